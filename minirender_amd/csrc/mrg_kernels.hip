@@ -1,0 +1,516 @@
+// HIP kernels for gfx950: ray queries and the wavefront path integrator.
+//
+// One batch renders a list of chunks (job, pixel range, sample range) for one or many jobs:
+//   reset -> generate -> { extend -> shade -> shadow } x (max depth + 1) -> accumulate
+// Queues: the extension queue (path ids, ping-pong by bounce) and the shadow queue (path id, shadow
+// ray, contribution) are compacted in `shade` with a wave ballot + mbcnt prefix and one atomic per
+// wave. All bounce kernels are persistent (fixed grid, counts read from device memory), so the
+// bounce loop can be captured once in a hipGraph.
+//
+// Reference: src/render/path_tracer.cpp:153-386 (trace, sample_pixel), scene.cpp:171-199 (camera),
+// render_session.cpp:39-74 (film commit order: per pixel, samples in index order, FP64 sums).
+#include <hip/hip_runtime.h>
+
+#include "mrg_kernels.h"
+#include "mrg_shade.h"
+#include "mrg_traverse.h"
+
+namespace mrg {
+
+// ---------------------------------------------------------------------------------------------
+// Wave helpers (wave64).
+__device__ __forceinline__ uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+
+__device__ __forceinline__ uint32_t prefix_in_mask(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+}
+
+// Reserves popcount(mask) consecutive slots of `counter` for the lanes in `mask`; returns each
+// lane's slot (all lanes of the wave must call).
+__device__ __forceinline__ uint32_t wave_reserve(uint32_t* counter, uint64_t mask) {
+    uint32_t base = 0;
+    if (mask) {
+        const uint32_t leader = static_cast<uint32_t>(__builtin_ctzll(mask));
+        if (lane_id() == leader) base = atomicAdd(counter, static_cast<uint32_t>(__builtin_popcountll(mask)));
+        base = __shfl(base, static_cast<int>(leader), 64);
+    }
+    return base + prefix_in_mask(mask);
+}
+
+__device__ __forceinline__ void wave_add64(uint64_t* counter, uint32_t value) {
+    // wave reduction then one atomic per wave
+    uint32_t v = value;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane_id() == 0 && v) atomicAdd(reinterpret_cast<unsigned long long*>(counter), static_cast<unsigned long long>(v));
+}
+
+__device__ __forceinline__ StackView make_stack(uint32_t* lds_ref, float* lds_t, const Pool& P) {
+    StackView s;
+    s.lds_ref   = lds_ref;
+    s.lds_t     = lds_t;
+    s.spill_ref = P.spill_ref;
+    s.spill_t   = P.spill_t;
+    s.lane      = threadIdx.x;
+    s.block     = blockDim.x;
+    s.glane     = blockIdx.x * blockDim.x + threadIdx.x;
+    s.lanes     = gridDim.x * blockDim.x;
+    return s;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Intersection microbenchmark / SceneBvh::intersect and ::occluded over a packed ray array.
+template <bool ANY, bool STATS>
+__global__ void __launch_bounds__(256) k_query(GpuScene sc, const float* __restrict__ rays, uint32_t n, float tmin, float tmax,
+                                               mrg_hit_raw* __restrict__ hits, float* __restrict__ bary,
+                                               uint8_t* __restrict__ occluded, uint32_t* __restrict__ work, Pool P,
+                                               uint64_t* __restrict__ stats) {
+    __shared__ uint32_t s_ref[kStackLds * 256];
+    __shared__ float    s_t[kStackLds * 256];
+    const StackView stk = make_stack(s_ref, s_t, P);
+    uint64_t        nb = 0, nt = 0;
+    for (;;) {
+        uint32_t base = 0;
+        if (lane_id() == 0) base = atomicAdd(work, 64u);
+        base = __shfl(base, 0, 64);
+        if (base >= n) break;
+        const uint32_t i = base + lane_id();
+        if (i < n) {
+            const float*  r  = rays + static_cast<size_t>(i) * 6;
+            const RayPre  rp = ray_setup(mk(r[0], r[1], r[2]), mk(r[3], r[4], r[5]));
+            const TraceResult res = trace_ray<ANY, STATS>(sc, rp, tmin, tmax, stk, &nb, &nt);
+            if (ANY) {
+                occluded[i] = res.slot != 0xffffffffu ? 1 : 0;
+            } else {
+                mrg_hit_raw h;
+                h.t         = res.slot != 0xffffffffu ? res.t : __builtin_inff();
+                h.primitive = res.slot != 0xffffffffu ? static_cast<int32_t>(res.prim) : -1;
+                hits[i]     = h;
+                if (bary) {
+                    bary[static_cast<size_t>(i) * 3 + 0] = res.b0;
+                    bary[static_cast<size_t>(i) * 3 + 1] = res.b1;
+                    bary[static_cast<size_t>(i) * 3 + 2] = res.b2;
+                }
+            }
+        }
+    }
+    if (STATS) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(stats), static_cast<unsigned long long>(nb));
+        atomicAdd(reinterpret_cast<unsigned long long*>(stats + 1), static_cast<unsigned long long>(nt));
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Batch start: clear counters (a fill kernel, so a captured graph holds no memset node).
+__global__ void k_reset(uint32_t* counters, uint32_t words, uint64_t* stats, uint32_t stat_words, uint32_t total_paths) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < words; i += gridDim.x * blockDim.x) counters[i] = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < stat_words; i += gridDim.x * blockDim.x) stats[i] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) counters[kCtrExt] = total_paths;
+}
+
+__device__ __forceinline__ uint32_t find_chunk_by_slot(const GpuChunk* chunks, uint32_t count, uint32_t slot) {
+    uint32_t lo = 0, hi = count;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (chunks[mid].slot_begin <= slot) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ uint32_t find_chunk_by_pixel(const GpuChunk* chunks, uint32_t count, uint32_t pix) {
+    uint32_t lo = 0, hi = count;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (chunks[mid].pixel_prefix <= pix) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// Camera rays (PathTracer::sample_pixel, path_tracer.cpp:368-386; Camera::generate_lens_ray).
+__global__ void __launch_bounds__(256) k_generate(Pool P, const GpuJob* __restrict__ jobs, const GpuChunk* __restrict__ chunks,
+                                                  uint32_t chunk_count, uint32_t total) {
+    for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < total; slot += gridDim.x * blockDim.x) {
+        const uint32_t  ci = find_chunk_by_slot(chunks, chunk_count, slot);
+        const GpuChunk  ch = chunks[ci];
+        const uint32_t  local  = slot - ch.slot_begin;
+        const uint32_t  s      = local / ch.pixel_count;
+        const uint32_t  p      = local - s * ch.pixel_count;
+        const uint32_t  pixel  = ch.pixel_begin + p;
+        const GpuJob&   job    = jobs[ch.job];
+        const uint32_t  x      = pixel % job.width;
+        const uint32_t  y      = pixel / job.width;
+        Pcg             rng    = for_sample(job.seed, static_cast<uint64_t>(y) * job.width + x, ch.sample_begin + s);
+        const float     fx     = next_1d(rng);
+        const float     fy     = next_1d(rng);
+        const float     lx     = next_1d(rng);
+        const float     ly     = next_1d(rng);
+        const GpuCamera& cam   = job.camera;
+        const float     uvx    = (static_cast<float>(x) + fx) / static_cast<float>(job.width);
+        const float     uvy    = (static_cast<float>(y) + fy) / static_cast<float>(job.height);
+        const float     aspect = static_cast<float>(job.width) / static_cast<float>(job.height);
+        const F3        fwd    = ld3(cam.forward), right = ld3(cam.right), up = ld3(cam.up), pos = ld3(cam.position);
+        const F3        pd = normalize((fwd + (((2.0f * uvx - 1.0f) * aspect) * cam.half_height) * right) +
+                                       ((1.0f - 2.0f * uvy) * cam.half_height) * up);
+        F3 o = pos, d = pd;
+        if (cam.aperture != 0.0f) {
+            float dx, dy;
+            sample_disk(lx, ly, dx, dy);
+            const float kx = dx * cam.aperture, ky = dy * cam.aperture;
+            const F3    focus_point = pos + pd * (cam.focus / dot(pd, fwd));
+            o                       = (pos + kx * right) + ky * up;
+            d                       = normalize(focus_point - o);
+        }
+        const float cosine = dot(d, fwd);
+        P.ray_o[slot] = make_float4(o.x, o.y, o.z, cam.near_plane / cosine);
+        P.ray_d[slot] = make_float4(d.x, d.y, d.z, cam.far_plane / cosine);
+        P.beta[slot]  = make_float4(1.0f, 1.0f, 1.0f, 0.0f);             // beta, previous pdf
+        P.rad[slot]   = make_float4(0.0f, 0.0f, 0.0f, 1.0f);             // radiance, eta scale
+        P.prev[slot]  = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(kPrevDelta | (ch.job << 16)));
+        P.rng[slot]   = make_uint4(static_cast<uint32_t>(rng.state), static_cast<uint32_t>(rng.state >> 32),
+                                   static_cast<uint32_t>(rng.inc), static_cast<uint32_t>(rng.inc >> 32));
+        P.q_ext0[slot] = slot;
+    }
+}
+
+// Closest hit for every path in the extension queue of this bounce.
+__global__ void __launch_bounds__(256) k_extend(Pool P, const GpuJob* __restrict__ jobs, uint32_t iter) {
+    __shared__ uint32_t s_ref[kStackLds * 256];
+    __shared__ float    s_t[kStackLds * 256];
+    const StackView stk   = make_stack(s_ref, s_t, P);
+    const uint32_t  n     = P.counters[kCtrExt + iter];
+    const uint32_t* queue = (iter & 1u) ? P.q_ext1 : P.q_ext0;
+    uint32_t*       work  = P.counters + kCtrWorkExt + iter;
+    uint64_t        dummy = 0;
+    for (;;) {
+        uint32_t base = 0;
+        if (lane_id() == 0) base = atomicAdd(work, 64u);
+        base = __shfl(base, 0, 64);
+        if (base >= n) break;
+        const uint32_t i = base + lane_id();
+        if (i < n) {
+            const uint32_t path = queue[i];
+            const float4   ro   = P.ray_o[path];
+            const float4   rd   = P.ray_d[path];
+            const uint32_t job  = __float_as_uint(P.prev[path].w) >> 16;
+            const RayPre   rp   = ray_setup(mk(ro.x, ro.y, ro.z), mk(rd.x, rd.y, rd.z));
+            const TraceResult res = trace_ray<false, false>(jobs[job].scene, rp, ro.w, rd.w, stk, &dummy, &dummy);
+            P.hit[path]      = make_float4(res.t, res.b0, res.b1, res.b2);
+            P.hit_slot[path] = res.slot;
+        }
+    }
+}
+
+// Shading of one bounce (PathTracer::trace loop body, path_tracer.cpp:164-363).
+__global__ void __launch_bounds__(256) k_shade(Pool P, const GpuJob* __restrict__ jobs, uint32_t iter) {
+    const uint32_t  n       = P.counters[kCtrExt + iter];
+    const uint32_t* queue   = (iter & 1u) ? P.q_ext1 : P.q_ext0;
+    uint32_t*       next_q  = (iter & 1u) ? P.q_ext0 : P.q_ext1;
+    uint32_t*       next_n  = P.counters + kCtrExt + iter + 1;
+    uint32_t*       sh_n    = P.counters + kCtrShadow + iter;
+    const uint32_t  stride  = gridDim.x * blockDim.x;
+    const uint32_t  wave0   = blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
+    const uint32_t  depth   = iter;
+    uint32_t        n_scatter = 0, n_rr = 0, n_shadow = 0;
+    for (uint32_t wb = wave0; wb < n; wb += stride) {
+        const uint32_t i        = wb + lane_id();
+        bool           cont     = false;
+        bool           emit     = false;
+        uint32_t       path     = 0;
+        float4         so = make_float4(0, 0, 0, 0), sd = so, sc4 = so;
+        if (i < n) {
+            path = queue[i];
+            const float4  pv    = P.prev[path];
+            const uint32_t flags = __float_as_uint(pv.w);
+            const GpuJob& job   = jobs[flags >> 16];
+            const GpuScene& sc  = job.scene;
+            const float4  ro = P.ray_o[path], rd = P.ray_d[path];
+            const F3      ray_d = mk(rd.x, rd.y, rd.z);
+            const float4  bt = P.beta[path];
+            const float4  lr = P.rad[path];
+            F3            beta = mk(bt.x, bt.y, bt.z);
+            float         prev_pdf  = bt.w;
+            F3            L         = mk(lr.x, lr.y, lr.z);
+            float         eta_scale = lr.w;
+            F3            prev_ref  = mk(pv.x, pv.y, pv.z);
+            bool          prev_delta = (flags & kPrevDelta) != 0u;
+            const uint4   rg = P.rng[path];
+            Pcg           rng{static_cast<uint64_t>(rg.x) | (static_cast<uint64_t>(rg.y) << 32),
+                              static_cast<uint64_t>(rg.z) | (static_cast<uint64_t>(rg.w) << 32)};
+            const float4  hv   = P.hit[path];
+            const uint32_t slot = P.hit_slot[path];
+            bool bad = false;
+            (void)ro;
+            do {
+                if (!finite3(beta) || beta.x < 0.0f || beta.y < 0.0f || beta.z < 0.0f) { bad = true; break; }
+                if (vmax(beta) == 0.0f) break;
+                const uint32_t mode = job.mode;
+                const bool unit_weight = depth == 0 || prev_delta || mode == 0u;
+                if (slot == 0xffffffffu) {
+                    float w = 1.0f;
+                    if (!unit_weight) w = mode == 2u ? power_heuristic(prev_pdf, sc.env_pdf, bad) : 0.0f;
+                    L = L + (beta * ld3(sc.env)) * w;
+                    break;
+                }
+                // surface_hit (bvh.cpp:302-370)
+                const float4* tp = reinterpret_cast<const float4*>(sc.tris + slot);
+                const float4  ta = tp[0], tb = tp[1], tc = tp[2];
+                const F3      v0 = mk(ta.x, ta.y, ta.z), v1 = mk(tb.x, tb.y, tb.z), v2 = mk(tc.x, tc.y, tc.z);
+                const uint32_t tflags = __float_as_uint(tb.w);
+                const float4* np = reinterpret_cast<const float4*>(sc.normals + slot);
+                const float4  na = np[0], nb = np[1], nc = np[2];
+                const uint32_t inst = sc.slot_instance[slot];
+                const GpuMaterial& mat = sc.materials[inst];
+                F3 geometric = normalize(cross(v1 - v0, v2 - v0));
+                if (tflags & kTriMirrored) geometric = -geometric;
+                const float b0 = hv.y, b1 = hv.z, b2 = hv.w;
+                F3 shading  = mk(0.0f, 0.0f, 0.0f);
+                F3 position = mk(0.0f, 0.0f, 0.0f);
+                F3 absolute = mk(0.0f, 0.0f, 0.0f);
+                shading = shading + b0 * normalize(mk(na.x, na.y, na.z));
+                { const F3 term = b0 * v0; position = position + term; absolute = absolute + vabs(term); }
+                shading = shading + b1 * normalize(mk(nb.x, nb.y, nb.z));
+                { const F3 term = b1 * v1; position = position + term; absolute = absolute + vabs(term); }
+                shading = shading + b2 * normalize(mk(nc.x, nc.y, nc.z));
+                { const F3 term = b2 * v2; position = position + term; absolute = absolute + vabs(term); }
+                if (dot(shading, shading) < 1e-24f) {
+                    shading = geometric;
+                } else {
+                    shading = normalize(shading);
+                    if (dot(shading, geometric) < 0.0f) shading = -shading;
+                }
+                const bool front_face = dot(ray_d, geometric) < 0.0f;
+                const F3   perr = absolute * (7.0f * kEpsF);
+                const F3   ng   = geometric;
+                // emission with its MIS weight
+                const F3 emission = ld3(mat.emission);
+                if (emission.x != 0.0f || emission.y != 0.0f || emission.z != 0.0f) {
+                    float w = 1.0f;
+                    if (!unit_weight) {
+                        w = mode == 2u ? power_heuristic(prev_pdf, area_pdf(sc, sc.slot_emitter[slot], prev_ref, position), bad) : 0.0f;
+                    }
+                    L = L + (beta * emission) * w;
+                }
+                if (depth == job.max_depth) break;
+                // shading frame and BSDF
+                F3 normal = mat.kind == 3u ? ng : shading;
+                const F3 wo_world = -ray_d;
+                if (dot(normal, wo_world) < 0.0f) normal = -normal;
+                const Frame f  = make_frame(normal);
+                const F3    wo = to_local(f, wo_world);
+                BsdfParams  bp;
+                bp.kind  = mat.kind;
+                bp.color = ld3(mat.color);
+                bp.alpha = mat.alpha;
+                bp.eta_i = front_face ? 1.0f : mat.ior;
+                bp.eta_t = front_face ? mat.ior : 1.0f;
+                const float light_u   = next_1d(rng);
+                const float su_x      = next_1d(rng);
+                const float su_y      = next_1d(rng);
+                const float component = next_1d(rng);
+                const float du_x      = next_1d(rng);
+                const float du_y      = next_1d(rng);
+                const F3    won       = normalize(wo);
+                if (!bp.is_delta()) {
+                    LightSample ls;
+                    if (sample_light(sc, position, light_u, su_x, su_y, ls) && (mode != 0u || ls.discrete)) {
+                        const F3   wi   = to_local(f, ls.dir);
+                        const bool side = dot(wo_world, ng) * dot(ls.dir, ng) > 0.0f;
+                        if (wi.z > 0.0f && side) {
+                            ++n_shadow;
+                            F3    origin = offset_origin(position, ng, perr, ls.dir);
+                            F3    sdir   = normalize(ls.dir);
+                            float limit  = __builtin_inff();
+                            bool  zero   = false;
+                            if (__builtin_isfinite(ls.distance)) {
+                                const F3 target = offset_origin(ls.position, ls.normal, ls.error, -ls.dir);
+                                const F3 diff   = target - origin;
+                                limit           = length(diff);
+                                if (limit == 0.0f) {
+                                    zero = true;
+                                } else {
+                                    sdir  = diff / limit;
+                                    limit = next_after(limit, 0.0f);
+                                }
+                            }
+                            if (!zero) {
+                                const F3 win = normalize(wi);
+                                float weight = 1.0f;
+                                if (mode == 2u && !ls.discrete) weight = power_heuristic(ls.probability, bsdf_pdf_n(bp, won, win), bad);
+                                const F3 c = (((beta * bsdf_eval_n(bp, won, win)) * wi.z) * ls.radiance) * (weight / ls.probability);
+                                emit = true;
+                                so   = make_float4(origin.x, origin.y, origin.z, limit);
+                                sd   = make_float4(sdir.x, sdir.y, sdir.z, 0.0f);
+                                sc4  = make_float4(c.x, c.y, c.z, 0.0f);
+                            }
+                        }
+                    }
+                }
+                BsdfSample bs;
+                if (!bsdf_sample(bp, won, component, du_x, du_y, bs)) break;
+                if (vmax(bs.weight) == 0.0f) break;
+                const F3    wi_world = normalize(to_world(f, bs.dir));
+                const float sides    = dot(wo_world, ng) * dot(wi_world, ng);
+                if ((bs.transmission && sides >= 0.0f) || (!bs.transmission && sides <= 0.0f)) break;
+                beta = beta * bs.weight;
+                if (!finite3(beta) || beta.x < 0.0f || beta.y < 0.0f || beta.z < 0.0f) { bad = true; break; }
+                prev_ref   = position;
+                prev_pdf   = bs.probability;
+                prev_delta = bs.discrete;
+                eta_scale  = eta_scale * (bs.eta * bs.eta);
+                ++n_scatter;
+                const F3 no = offset_origin(position, ng, perr, wi_world);
+                const F3 nd = normalize(wi_world);
+                const float roulette_u = next_1d(rng);
+                if (job.rr && depth + 1 >= job.rr_depth) {
+                    const float survival = sclamp(vmax(beta * eta_scale), 0.05f, 0.95f);
+                    if (roulette_u >= survival) {
+                        ++n_rr;
+                        break;
+                    }
+                    beta = beta / survival;
+                }
+                cont = true;
+                P.ray_o[path] = make_float4(no.x, no.y, no.z, 0.0f);
+                P.ray_d[path] = make_float4(nd.x, nd.y, nd.z, __builtin_inff());
+                P.beta[path]  = make_float4(beta.x, beta.y, beta.z, prev_pdf);
+                P.prev[path]  = make_float4(prev_ref.x, prev_ref.y, prev_ref.z,
+                                            __uint_as_float((flags & ~kPrevDelta) | (prev_delta ? kPrevDelta : 0u)));
+                P.rng[path]   = make_uint4(static_cast<uint32_t>(rng.state), static_cast<uint32_t>(rng.state >> 32),
+                                           static_cast<uint32_t>(rng.inc), static_cast<uint32_t>(rng.inc >> 32));
+            } while (false);
+            P.rad[path] = make_float4(L.x, L.y, L.z, eta_scale);
+            if (bad) atomicOr(job.error_flag, 1u);
+        }
+        const uint64_t cmask = __ballot(cont);
+        const uint32_t cpos  = wave_reserve(next_n, cmask);
+        if (cont) next_q[cpos] = path;
+        const uint64_t smask = __ballot(emit);
+        const uint32_t spos  = wave_reserve(sh_n, smask);
+        if (emit) {
+            P.q_sh[spos] = path;
+            P.sh_o[spos] = so;
+            P.sh_d[spos] = sd;
+            P.sh_c[spos] = sc4;
+        }
+    }
+    wave_add64(P.stats + kStatShadowRays, n_shadow);
+    wave_add64(P.stats + kStatScatter, n_scatter);
+    wave_add64(P.stats + kStatRoulette, n_rr);
+}
+
+// Any-hit for every shadow ray of this bounce; unoccluded rays add their contribution.
+__global__ void __launch_bounds__(256) k_shadow(Pool P, const GpuJob* __restrict__ jobs, uint32_t iter) {
+    __shared__ uint32_t s_ref[kStackLds * 256];
+    __shared__ float    s_t[kStackLds * 256];
+    const StackView stk  = make_stack(s_ref, s_t, P);
+    const uint32_t  n    = P.counters[kCtrShadow + iter];
+    uint32_t*       work = P.counters + kCtrWorkShadow + iter;
+    uint64_t        dummy = 0;
+    for (;;) {
+        uint32_t base = 0;
+        if (lane_id() == 0) base = atomicAdd(work, 64u);
+        base = __shfl(base, 0, 64);
+        if (base >= n) break;
+        const uint32_t i = base + lane_id();
+        if (i < n) {
+            const uint32_t path = P.q_sh[i];
+            const float4   o    = P.sh_o[i];
+            const float4   d    = P.sh_d[i];
+            const uint32_t job  = __float_as_uint(P.prev[path].w) >> 16;
+            const RayPre   rp   = ray_setup(mk(o.x, o.y, o.z), mk(d.x, d.y, d.z));
+            const TraceResult res = trace_ray<true, false>(jobs[job].scene, rp, 0.0f, o.w, stk, &dummy, &dummy);
+            if (res.slot == 0xffffffffu) {
+                const float4 c = P.sh_c[i];
+                float4       l = P.rad[path];
+                l.x = l.x + c.x;
+                l.y = l.y + c.y;
+                l.z = l.z + c.z;
+                P.rad[path] = l;
+            }
+        }
+    }
+}
+
+// Film commit: per pixel, samples in index order, FP64 sums (render_session.cpp:39-74).
+__global__ void __launch_bounds__(256) k_accumulate(Pool P, const GpuJob* __restrict__ jobs, const GpuChunk* __restrict__ chunks,
+                                                    uint32_t chunk_count, uint32_t total_pixels) {
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total_pixels; g += gridDim.x * blockDim.x) {
+        const uint32_t ci = find_chunk_by_pixel(chunks, chunk_count, g);
+        const GpuChunk ch = chunks[ci];
+        const uint32_t p  = g - ch.pixel_prefix;
+        const GpuJob&  job = jobs[ch.job];
+        const uint32_t pixel = ch.pixel_begin + p;
+        double sx = job.film_sums[static_cast<size_t>(pixel) * 3 + 0];
+        double sy = job.film_sums[static_cast<size_t>(pixel) * 3 + 1];
+        double sz = job.film_sums[static_cast<size_t>(pixel) * 3 + 2];
+        bool   bad = false;
+        for (uint32_t s = 0; s < ch.sample_count; ++s) {
+            const float4 v = P.rad[ch.slot_begin + s * ch.pixel_count + p];
+            if (!(__builtin_isfinite(v.x) && __builtin_isfinite(v.y) && __builtin_isfinite(v.z)) || v.x < 0.0f || v.y < 0.0f || v.z < 0.0f)
+                bad = true;
+            sx = sx + static_cast<double>(v.x);
+            sy = sy + static_cast<double>(v.y);
+            sz = sz + static_cast<double>(v.z);
+        }
+        const uint32_t count = job.film_counts[pixel] + ch.sample_count;
+        const double   fm    = 3.4028234663852886e38;
+        if (!(sx / count <= fm && sy / count <= fm && sz / count <= fm)) bad = true;
+        job.film_sums[static_cast<size_t>(pixel) * 3 + 0] = sx;
+        job.film_sums[static_cast<size_t>(pixel) * 3 + 1] = sy;
+        job.film_sums[static_cast<size_t>(pixel) * 3 + 2] = sz;
+        job.film_counts[pixel] = count;
+        if (bad) atomicOr(job.error_flag, 2u);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Host-side launchers.
+
+static uint32_t grid_for(const LaunchShape& s, uint32_t per_cu) { return s.compute_units * per_cu; }
+
+hipError_t launch_query(const GpuScene& sc, const float* rays, uint32_t n, float tmin, float tmax, mrg_hit_raw* hits, float* bary,
+                        uint8_t* occluded, uint32_t* work, const Pool& P, uint64_t* stats, bool any, bool with_stats,
+                        const LaunchShape& shape, hipStream_t stream) {
+    const dim3 grid(shape.traversal_blocks), block(256);
+    if (any) {
+        if (with_stats) hipLaunchKernelGGL((k_query<true, true>), grid, block, 0, stream, sc, rays, n, tmin, tmax, hits, bary, occluded, work, P, stats);
+        else hipLaunchKernelGGL((k_query<true, false>), grid, block, 0, stream, sc, rays, n, tmin, tmax, hits, bary, occluded, work, P, stats);
+    } else {
+        if (with_stats) hipLaunchKernelGGL((k_query<false, true>), grid, block, 0, stream, sc, rays, n, tmin, tmax, hits, bary, occluded, work, P, stats);
+        else hipLaunchKernelGGL((k_query<false, false>), grid, block, 0, stream, sc, rays, n, tmin, tmax, hits, bary, occluded, work, P, stats);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_reset(const Pool& P, uint32_t total_paths, hipStream_t stream) {
+    hipLaunchKernelGGL(k_reset, dim3(4), dim3(256), 0, stream, P.counters, kCounterWords, P.stats, kStatWords, total_paths);
+    return hipGetLastError();
+}
+
+hipError_t launch_generate(const Pool& P, const GpuJob* jobs, const GpuChunk* chunks, uint32_t chunk_count, uint32_t total,
+                           const LaunchShape& shape, hipStream_t stream) {
+    hipLaunchKernelGGL(k_generate, dim3(grid_for(shape, 8)), dim3(256), 0, stream, P, jobs, chunks, chunk_count, total);
+    return hipGetLastError();
+}
+
+hipError_t launch_extend(const Pool& P, const GpuJob* jobs, uint32_t iter, const LaunchShape& shape, hipStream_t stream) {
+    hipLaunchKernelGGL(k_extend, dim3(shape.traversal_blocks), dim3(256), 0, stream, P, jobs, iter);
+    return hipGetLastError();
+}
+
+hipError_t launch_shade(const Pool& P, const GpuJob* jobs, uint32_t iter, const LaunchShape& shape, hipStream_t stream) {
+    hipLaunchKernelGGL(k_shade, dim3(shape.shade_blocks), dim3(256), 0, stream, P, jobs, iter);
+    return hipGetLastError();
+}
+
+hipError_t launch_shadow(const Pool& P, const GpuJob* jobs, uint32_t iter, const LaunchShape& shape, hipStream_t stream) {
+    hipLaunchKernelGGL(k_shadow, dim3(shape.traversal_blocks), dim3(256), 0, stream, P, jobs, iter);
+    return hipGetLastError();
+}
+
+hipError_t launch_accumulate(const Pool& P, const GpuJob* jobs, const GpuChunk* chunks, uint32_t chunk_count, uint32_t total_pixels,
+                             const LaunchShape& shape, hipStream_t stream) {
+    hipLaunchKernelGGL(k_accumulate, dim3(grid_for(shape, 4)), dim3(256), 0, stream, P, jobs, chunks, chunk_count, total_pixels);
+    return hipGetLastError();
+}
+
+}  // namespace mrg

@@ -1,0 +1,145 @@
+// Device data layout shared by the host builder/scheduler (C++) and the HIP kernels.
+//
+// HBM layout of one resident scene:
+//   nodes      [node_count]    GpuNode, 64 B each (two per 128 B cache line): both child boxes of an
+//                              inner node plus the two child references, so one node fetch gives the
+//                              whole two-box test. Depth-first order, root = 0.
+//   tris       [n]             GpuTri, 48 B: three float4 vertices in leaf order; v0.w carries the
+//                              scene-order primitive index (the tie-break key of bvh.cpp:273-276).
+//   normals    [n]             GpuTri layout, per-corner world normals in leaf order.
+//   slot_info  [n]             uint2 (instance, area-light choice or -1) in leaf order.
+//   materials  [instances]     GpuMaterial.
+//   lights                     light choices, area emitters, directional lights and the selection
+//                              CDF, all float, built on the host exactly as the FP32 twin builds them.
+#pragma once
+
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define MRG_HD __host__ __device__
+#else
+#define MRG_HD
+#endif
+
+namespace mrg {
+
+constexpr uint32_t kLeafBit    = 0x80000000u;
+constexpr uint32_t kEmptyRef   = 0xffffffffu;
+constexpr uint32_t kMaxLeaf    = 16;   // triangles per leaf (count-1 packed in 4 bits)
+constexpr uint32_t kMaxDepth   = 30;   // builder guarantee: traversal stack never exceeds 32
+constexpr uint32_t kStackLds   = 16;   // stack entries per lane kept in LDS
+constexpr uint32_t kStackTotal = 32;   // LDS + global spill entries per lane
+
+MRG_HD inline uint32_t leaf_ref(uint32_t first, uint32_t count) { return kLeafBit | ((count - 1u) << 27) | first; }
+MRG_HD inline bool     is_leaf(uint32_t ref) { return (ref & kLeafBit) != 0u; }
+MRG_HD inline uint32_t leaf_first(uint32_t ref) { return ref & 0x07ffffffu; }
+MRG_HD inline uint32_t leaf_count(uint32_t ref) { return ((ref >> 27) & 15u) + 1u; }
+
+struct alignas(16) GpuNode {
+    float    lo0[3], hi0x;     // child 0 box (lo, hi.x)
+    float    hi0yz[2], lo1xy[2];
+    float    lo1z, hi1[3];
+    uint32_t ref0, ref1, pad0, pad1;
+};
+static_assert(sizeof(GpuNode) == 64, "GpuNode must be 64 bytes");
+
+struct alignas(16) GpuTri {
+    float v0[3];
+    uint32_t prim;
+    float v1[3];
+    uint32_t pad1;
+    float v2[3];
+    uint32_t pad2;
+};
+static_assert(sizeof(GpuTri) == 48, "GpuTri must be 48 bytes");
+
+struct GpuMaterial {
+    uint32_t kind;
+    float    color[3];
+    float    emission[3];
+    float    roughness;
+    float    alpha;       // roughness * roughness (float)
+    float    ior;
+    uint32_t double_sided;
+    uint32_t mirrored;
+};
+
+struct GpuAreaLight {
+    float    v0[3], v1[3], v2[3];
+    float    normal[3];
+    float    radiance[3];
+    float    area;
+    uint32_t double_sided;
+    uint32_t pad;
+};
+
+struct GpuLightChoice {
+    uint32_t kind;   // 0 area, 1 directional, 2 environment
+    uint32_t index;
+};
+
+struct GpuDirectional {
+    float direction[3];  // toward the light (-normalize(propagation))
+    float radiance[3];
+};
+
+// Per-scene pointers and constants handed to the kernels by value.
+struct GpuScene {
+    const GpuNode*        nodes;
+    const GpuTri*         tris;
+    const GpuTri*         normals;
+    const uint32_t*       slot_instance;
+    const int32_t*        slot_emitter;
+    const GpuMaterial*    materials;
+    const GpuLightChoice* choices;
+    const float*          cdf;          // [choice_count + 1]
+    const GpuAreaLight*   areas;
+    const GpuDirectional* directional;
+    uint32_t              node_count;
+    uint32_t              tri_count;
+    uint32_t              choice_count;
+    int32_t               env_choice;
+    float                 env[3];
+    float                 env_pdf;      // selection(env) * 1/(4 pi), 0 without environment
+};
+
+// Camera constants computed on the host with the FP32 twin's operations (scene.cpp:171-199).
+struct GpuCamera {
+    float position[3];
+    float forward[3];
+    float right[3];
+    float up[3];
+    float half_height;
+    float aperture;
+    float focus;
+    float near_plane;
+    float far_plane;
+};
+
+struct GpuJob {
+    GpuCamera   camera;
+    GpuScene    scene;
+    uint32_t    width, height;
+    uint32_t    mode, max_depth;
+    uint32_t    rr, rr_depth;
+    uint64_t    seed;
+    double*     film_sums;    // [w*h*3]
+    uint32_t*   film_counts;  // [w*h]
+    uint32_t*   error_flag;
+    uint32_t    pad;
+};
+
+// A contiguous range of samples for a contiguous range of pixels of one job. Path slot i of the
+// chunk maps to sample = sample_begin + i / pixel_count, pixel = pixel_begin + i % pixel_count.
+struct GpuChunk {
+    uint32_t job;
+    uint32_t pixel_begin;
+    uint32_t pixel_count;
+    uint32_t sample_begin;
+    uint32_t sample_count;
+    uint32_t slot_begin;  // first path slot of this chunk in the pool
+    uint32_t pad0, pad1;
+};
+
+}  // namespace mrg

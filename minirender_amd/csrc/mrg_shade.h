@@ -1,0 +1,283 @@
+// Surface interaction, BSDFs and light sampling in FP32, operation for operation as the oracle's FP32
+// twin. Reference: src/render/bvh.cpp:302-370 (surface_hit), src/render/bsdf.cpp (ShadingFrame,
+// diffuse / GGX conductor / mirror / dielectric), src/render/light.cpp:49-59, :266-340 (light
+// sampling, area pdf), src/render/path_tracer.cpp:73-109 (power heuristic, offset_origin).
+#pragma once
+
+#include "mrg_layout.h"
+#include "mrg_math.h"
+
+namespace mrg {
+
+__device__ __forceinline__ float power_heuristic(float a, float b, bool& bad) {
+    if (!__builtin_isfinite(a) || !__builtin_isfinite(b) || a < 0.0f || b < 0.0f) bad = true;
+    const float scale = smax(a, b);
+    if (scale == 0.0f) return 0.0f;
+    const float x = a / scale, y = b / scale;
+    return x * x / (x * x + y * y);
+}
+
+// offset_origin (path_tracer.cpp:87-105) with the FP32 numeric floor FLT_MIN.
+__device__ __forceinline__ F3 offset_origin(F3 position, F3 geometric_normal, F3 error, F3 dir) {
+    F3 n = normalize(geometric_normal);
+    if (dot(n, dir) < 0.0f) n = -n;
+    const float distance = smax(dot(vabs(n), error), kFltMin);
+    F3          o        = position + distance * n;
+    if (n.x != 0.0f) o.x = next_after(o.x, __builtin_copysignf(__builtin_inff(), n.x));
+    if (n.y != 0.0f) o.y = next_after(o.y, __builtin_copysignf(__builtin_inff(), n.y));
+    if (n.z != 0.0f) o.z = next_after(o.z, __builtin_copysignf(__builtin_inff(), n.z));
+    return o;
+}
+
+struct Frame {
+    F3 t, b, n;
+};
+
+__device__ __forceinline__ Frame make_frame(F3 normal) {
+    Frame     f;
+    const float m = max3(__builtin_fabsf(normal.x), __builtin_fabsf(normal.y), __builtin_fabsf(normal.z));
+    f.n           = normalize(normal / m);
+    const F3 helper = __builtin_fabsf(f.n.z) < 0.999f ? mk(0.0f, 0.0f, 1.0f) : mk(0.0f, 1.0f, 0.0f);
+    f.t             = normalize(cross(helper, f.n));
+    f.b             = cross(f.n, f.t);
+    return f;
+}
+
+__device__ __forceinline__ F3 to_local(const Frame& f, F3 w) { return mk(dot(w, f.t), dot(w, f.b), dot(w, f.n)); }
+__device__ __forceinline__ F3 to_world(const Frame& f, F3 l) { return (l.x * f.t + l.y * f.b) + l.z * f.n; }
+
+// --- warps (sampling.cpp:67-107), FP32 twin form
+__device__ __forceinline__ void sample_disk(float ux, float uy, float& dx, float& dy) {
+    const float sx = 2.0f * ux - 1.0f, sy = 2.0f * uy - 1.0f;
+    if (sx == 0.0f && sy == 0.0f) {
+        dx = 0.0f;
+        dy = 0.0f;
+        return;
+    }
+    const bool  horizontal = __builtin_fabsf(sx) > __builtin_fabsf(sy);
+    const float radius     = horizontal ? sx : sy;
+    const float turns      = horizontal ? 0.125f * (sy / sx) : 0.25f - 0.125f * (sx / sy);
+    float       s, c;
+    sincos_turns(turns, s, c);
+    dx = radius * c;
+    dy = radius * s;
+}
+
+__device__ __forceinline__ F3 sample_sphere(float ux, float uy) {
+    const float z = 1.0f - 2.0f * ux;
+    const float r = __builtin_sqrtf(smax(0.0f, 1.0f - z * z));
+    float       s, c;
+    sincos_turns(uy, s, c);
+    return mk(r * c, r * s, z);
+}
+
+__device__ __forceinline__ F3 sample_cosine_hemisphere(float ux, float uy) {
+    float dx, dy;
+    sample_disk(ux, uy, dx, dy);
+    return mk(dx, dy, __builtin_sqrtf(smax(0.0f, 1.0f - (dx * dx + dy * dy))));
+}
+
+__device__ __forceinline__ F3 sample_triangle(float ux, float uy) {
+    const float root = __builtin_sqrtf(ux);
+    return mk(1.0f - root, root * (1.0f - uy), root * uy);
+}
+
+// --- BSDF pieces (bsdf.cpp:36-85)
+__device__ __forceinline__ float ggx_d(F3 m, float alpha) {
+    const float a2    = alpha * alpha;
+    const float denom = (m.x * m.x + m.y * m.y) + a2 * m.z * m.z;
+    return m.z > 0.0f ? a2 / (kPiF * denom * denom) : 0.0f;
+}
+
+__device__ __forceinline__ float g1_over_cos(F3 v, float alpha) {
+    if (v.z <= 0.0f) return 0.0f;
+    return 2.0f / (v.z + __builtin_sqrtf(v.z * v.z + alpha * alpha * (v.x * v.x + v.y * v.y)));
+}
+
+__device__ __forceinline__ float fresnel_ft(float cosine, float sin2, float eta_i, float eta_t) {
+    if (eta_i == eta_t) return 0.0f;
+    if (sin2 >= 1.0f) return 1.0f;
+    const float tc  = __builtin_sqrtf(1.0f - sin2);
+    const float par = (eta_t * cosine - eta_i * tc) / (eta_t * cosine + eta_i * tc);
+    const float per = (eta_i * cosine - eta_t * tc) / (eta_i * cosine + eta_t * tc);
+    return 0.5f * (par * par + per * per);
+}
+
+__device__ __forceinline__ F3 schlick(F3 f0, float cosine) {
+    const float t  = 1.0f - sclamp(cosine, 0.0f, 1.0f);
+    const float t5 = (((t * t) * t) * t) * t;
+    return f0 + (mk(1.0f, 1.0f, 1.0f) - f0) * t5;
+}
+
+__device__ __forceinline__ F3 sample_vndf(F3 wo, float ux, float uy, float alpha) {
+    const F3    view  = normalize(mk(alpha * wo.x, alpha * wo.y, wo.z));
+    const float lensq = view.x * view.x + view.y * view.y;
+    const F3    t1    = lensq > 0.0f ? mk(-view.y, view.x, 0.0f) / __builtin_sqrtf(lensq) : mk(1.0f, 0.0f, 0.0f);
+    const F3    t2    = cross(view, t1);
+    const float radius = __builtin_sqrtf(ux);
+    float       s, c;
+    sincos_turns(uy, s, c);
+    const float x     = radius * c;
+    const float blend = 0.5f * (1.0f + view.z);
+    const float y     = (1.0f - blend) * __builtin_sqrtf(smax(0.0f, 1.0f - x * x)) + blend * radius * s;
+    const F3    n     = (x * t1 + y * t2) + __builtin_sqrtf(smax(0.0f, (1.0f - x * x) - y * y)) * view;
+    return normalize(mk(alpha * n.x, alpha * n.y, smax(0.0f, n.z)));
+}
+
+struct BsdfParams {
+    uint32_t kind;
+    F3       color;
+    float    alpha, eta_i, eta_t;
+    __device__ __forceinline__ bool is_delta() const { return kind == 2u || kind == 3u || (kind == 1u && alpha <= 1e-6f); }
+};
+
+__device__ __forceinline__ F3 bsdf_eval_n(const BsdfParams& b, F3 wo, F3 wi) {
+    if (wo.z <= 0.0f || wi.z <= 0.0f || b.is_delta()) return mk(0.0f, 0.0f, 0.0f);
+    if (b.kind == 0u) return b.color / kPiF;
+    const F3 m = normalize(wo + wi);
+    return (((schlick(b.color, dot(wo, m)) * ggx_d(m, b.alpha)) * g1_over_cos(wo, b.alpha)) * g1_over_cos(wi, b.alpha)) / 4.0f;
+}
+
+__device__ __forceinline__ float bsdf_pdf_n(const BsdfParams& b, F3 wo, F3 wi) {
+    if (wo.z <= 0.0f || wi.z <= 0.0f || b.is_delta()) return 0.0f;
+    if (b.kind == 0u) return smax(0.0f, wi.z) / kPiF;
+    const F3 m = normalize(wo + wi);
+    return ggx_d(m, b.alpha) * g1_over_cos(wo, b.alpha) / 4.0f;
+}
+
+struct BsdfSample {
+    F3    dir, weight;
+    float probability, eta;
+    bool  discrete, transmission;
+};
+
+// Bsdf::sample (bsdf.cpp:198-277) with wo already normalised.
+__device__ __forceinline__ bool bsdf_sample(const BsdfParams& b, F3 wo, float component, float ux, float uy, BsdfSample& s) {
+    if (wo.z <= 0.0f) return false;
+    const F3 reflected = mk(-wo.x, -wo.y, wo.z);
+    s.transmission     = false;
+    if (b.kind == 3u) {
+        const float ratio = b.eta_i / b.eta_t;
+        const float sin2  = ratio * ratio * (wo.x * wo.x + wo.y * wo.y);
+        const float refl  = fresnel_ft(sclamp(wo.z, 0.0f, 1.0f), sin2, b.eta_i, b.eta_t);
+        if (component < refl) {
+            s.dir         = reflected;
+            s.weight      = mk(1.0f, 1.0f, 1.0f);
+            s.probability = refl;
+            s.discrete    = true;
+            s.eta         = 1.0f;
+            return true;
+        }
+        s.dir          = mk(-ratio * wo.x, -ratio * wo.y, -__builtin_sqrtf(smax(0.0f, 1.0f - sin2)));
+        s.weight       = b.color * (ratio * ratio);
+        s.probability  = 1.0f - refl;
+        s.discrete     = true;
+        s.transmission = true;
+        s.eta          = b.eta_t / b.eta_i;
+        return true;
+    }
+    if (b.is_delta()) {
+        s.dir         = reflected;
+        s.weight      = b.kind == 2u ? b.color : schlick(b.color, wo.z);
+        s.probability = 1.0f;
+        s.discrete    = true;
+        s.eta         = 1.0f;
+        return true;
+    }
+    F3 wi;
+    if (b.kind == 0u) {
+        wi = sample_cosine_hemisphere(ux, uy);
+    } else {
+        const F3 m = sample_vndf(wo, ux, uy, b.alpha);
+        wi         = normalize((2.0f * dot(wo, m)) * m - wo);
+    }
+    if (wi.z <= 0.0f) return false;
+    const F3    ewi     = normalize(wi);
+    const float density = bsdf_pdf_n(b, wo, ewi);
+    s.weight            = bsdf_eval_n(b, wo, ewi) * wi.z / density;
+    s.dir               = wi;
+    s.probability       = density;
+    s.discrete          = false;
+    s.eta               = 1.0f;
+    return true;
+}
+
+// --- lights (light.cpp)
+__device__ __forceinline__ float area_density(F3 ref, F3 point, F3 normal, float area) {
+    const F3    diff = point - ref;
+    const float d2   = dot(diff, diff);
+    if (d2 == 0.0f) return 0.0f;
+    const float cosine = __builtin_fabsf(dot(normal, (-diff) / __builtin_sqrtf(d2)));
+    return cosine > 0.0f ? d2 / (area * cosine) : 0.0f;
+}
+
+struct LightSample {
+    F3    dir, radiance;
+    float distance, probability;
+    bool  discrete;
+    F3    position, normal, error;
+};
+
+// DiscreteDistribution::sample index (upper_bound over the CDF) and mass.
+__device__ __forceinline__ uint32_t select_light(const GpuScene& sc, float u, float& mass) {
+    uint32_t lo = 0, hi = sc.choice_count + 1;  // first index with u < cdf[index]
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (u < sc.cdf[mid]) hi = mid; else lo = mid + 1;
+    }
+    const uint32_t index = lo - 1;
+    mass                 = sc.cdf[index + 1] - sc.cdf[index];
+    return index;
+}
+
+__device__ __forceinline__ bool sample_light(const GpuScene& sc, F3 ref, float light_u, float sux, float suy, LightSample& out) {
+    if (sc.choice_count == 0) return false;
+    float          mass;
+    const uint32_t idx = select_light(sc, light_u, mass);
+    const GpuLightChoice ch = sc.choices[idx];
+    if (ch.kind == 1u) {
+        const GpuDirectional d = sc.directional[ch.index];
+        out.dir         = ld3(d.direction);
+        out.radiance    = ld3(d.radiance);
+        out.distance    = __builtin_inff();
+        out.probability = mass;
+        out.discrete    = true;
+        return true;
+    }
+    if (ch.kind == 2u) {
+        out.dir         = sample_sphere(sux, suy);
+        out.radiance    = ld3(sc.env);
+        out.distance    = __builtin_inff();
+        out.probability = mass * (1.0f / (4.0f * kPiF));
+        out.discrete    = false;
+        return true;
+    }
+    const GpuAreaLight& a  = sc.areas[ch.index];
+    const F3            v0 = ld3(a.v0), v1 = ld3(a.v1), v2 = ld3(a.v2), nrm = ld3(a.normal);
+    const F3            b  = sample_triangle(sux, suy);
+    const F3            pt = (b.x * v0 + b.y * v1) + b.z * v2;
+    if (!a.double_sided && dot(nrm, ref - pt) <= 0.0f) return false;
+    const float density = area_density(ref, pt, nrm, a.area);
+    if (density == 0.0f) return false;
+    const F3    dir      = pt - ref;
+    const float distance = length(dir);
+    out.error       = ((vabs(b.x * v0) + vabs(b.y * v1)) + vabs(b.z * v2)) * (7.0f * kEpsF);
+    out.dir         = dir / distance;
+    out.radiance    = ld3(a.radiance);
+    out.distance    = distance;
+    out.probability = mass * density;
+    out.discrete    = false;
+    out.position    = pt;
+    out.normal      = nrm;
+    return true;
+}
+
+__device__ __forceinline__ float area_pdf(const GpuScene& sc, int32_t choice, F3 ref, F3 position) {
+    if (choice < 0) return 0.0f;
+    const GpuAreaLight& a   = sc.areas[sc.choices[choice].index];
+    const F3            nrm = ld3(a.normal);
+    if (!a.double_sided && dot(nrm, ref - position) <= 0.0f) return 0.0f;
+    return (sc.cdf[choice + 1] - sc.cdf[choice]) * area_density(ref, position, nrm, a.area);
+}
+
+}  // namespace mrg

@@ -1,0 +1,243 @@
+// BVH traversal for one ray per lane: two-box nodes from HBM, the current node in VGPRs, a stack of
+// (child reference, entry distance) pairs in LDS (kStackLds per lane) that spills to a per-lane
+// global region for deeper entries (the builder bounds the depth, so 32 entries always suffice).
+//
+// Box test: conservative slab test (each slab bound computed as (bound - origin) * (1/d), then the
+// entry/exit distances widened by 2^-20 relative), so a box is never culled when the watertight
+// triangle test would accept a triangle inside it; closest-hit results therefore do not depend on
+// the tree (nearest t, ties to the lower scene-order index; bvh.cpp:273-276).
+// Triangle test: the FP32 form of RayQuery::intersect (intersection.cpp:108-152), operation for
+// operation.
+#pragma once
+
+#include "mrg_layout.h"
+#include "mrg_math.h"
+
+namespace mrg {
+
+struct RayPre {
+    F3    o, d, inv;
+    int   kx, ky, kz;
+    float sx, sy, sz;
+};
+
+__device__ __forceinline__ RayPre ray_setup(F3 o, F3 d) {
+    RayPre r;
+    r.o = o;
+    r.d = d;
+    r.inv.x = d.x != 0.0f ? 1.0f / d.x : __builtin_copysignf(1e30f, d.x);
+    r.inv.y = d.y != 0.0f ? 1.0f / d.y : __builtin_copysignf(1e30f, d.y);
+    r.inv.z = d.z != 0.0f ? 1.0f / d.z : __builtin_copysignf(1e30f, d.z);
+    const F3 ad = vabs(d);
+    int kz = ad.x > ad.y ? 0 : 1;
+    if (ad.z > (kz == 0 ? ad.x : ad.y)) kz = 2;
+    int kx = kz == 2 ? 0 : kz + 1;
+    int ky = kx == 2 ? 0 : kx + 1;
+    if (comp(d, kz) < 0.0f) {
+        const int t = kx;
+        kx          = ky;
+        ky          = t;
+    }
+    r.kx = kx;
+    r.ky = ky;
+    r.kz = kz;
+    const float dz = comp(d, kz);
+    r.sx = comp(d, kx) / dz;
+    r.sy = comp(d, ky) / dz;
+    r.sz = 1.0f / dz;
+    return r;
+}
+
+// Conservative slab test; returns the widened entry distance or +inf for a miss.
+__device__ __forceinline__ float box_enter(const RayPre& r, float lx, float ly, float lz, float hx, float hy, float hz,
+                                           float tmin, float tmax) {
+    const float ax = (lx - r.o.x) * r.inv.x, bx = (hx - r.o.x) * r.inv.x;
+    const float ay = (ly - r.o.y) * r.inv.y, by = (hy - r.o.y) * r.inv.y;
+    const float az = (lz - r.o.z) * r.inv.z, bz = (hz - r.o.z) * r.inv.z;
+    float tn = __builtin_fmaxf(__builtin_fmaxf(__builtin_fminf(ax, bx), __builtin_fminf(ay, by)),
+                               __builtin_fmaxf(__builtin_fminf(az, bz), tmin));
+    float tf = __builtin_fminf(__builtin_fminf(__builtin_fmaxf(ax, bx), __builtin_fmaxf(ay, by)),
+                               __builtin_fminf(__builtin_fmaxf(az, bz), tmax));
+    constexpr float lo = 1.0f - 0x1p-20f, hi = 1.0f + 0x1p-20f;
+    tn *= tn > 0.0f ? lo : hi;
+    tf *= tf > 0.0f ? hi : lo;
+    return tn <= tf ? tn : __builtin_inff();
+}
+
+__device__ __forceinline__ float dop(float a, float b, float c, float d) {
+    const float cd   = c * d;
+    const float corr = __builtin_fmaf(-c, d, cd);
+    return __builtin_fmaf(a, b, -cd) + corr;
+}
+
+struct TriHit {
+    float t, b0, b1, b2;
+};
+
+__device__ __forceinline__ void permute(const RayPre& r, F3 v, float& px, float& py, float& pz) {
+    const F3    t  = v - r.o;
+    const float tx = comp(t, r.kx), ty = comp(t, r.ky), tz = comp(t, r.kz);
+    px = tx - r.sx * tz;
+    py = ty - r.sy * tz;
+    pz = tz * r.sz;
+}
+
+// FP32 watertight test with the reference's error-bound rejection. Returns false on a miss.
+__device__ __forceinline__ bool tri_test(const RayPre& r, F3 v0, F3 v1, F3 v2, float tmin, float tmax, TriHit& out) {
+    float p0x, p0y, p0z, p1x, p1y, p1z, p2x, p2y, p2z;
+    permute(r, v0, p0x, p0y, p0z);
+    permute(r, v1, p1x, p1y, p1z);
+    permute(r, v2, p2x, p2y, p2z);
+    const float a = dop(p1x, p2y, p1y, p2x);
+    const float b = dop(p2x, p0y, p2y, p0x);
+    const float c = dop(p0x, p1y, p0y, p1x);
+    if ((a < 0.0f || b < 0.0f || c < 0.0f) && (a > 0.0f || b > 0.0f || c > 0.0f)) return false;
+    const float det = (a + b) + c;
+    if (det == 0.0f) return false;
+    const float t = ((a * p0z + b * p1z) + c * p2z) / det;
+    if (t < tmin || t > tmax) return false;
+    constexpr float g2 = (2.0f * 0x1p-24f) / (1.0f - 2.0f * 0x1p-24f);
+    constexpr float g3 = (3.0f * 0x1p-24f) / (1.0f - 3.0f * 0x1p-24f);
+    constexpr float g5 = (5.0f * 0x1p-24f) / (1.0f - 5.0f * 0x1p-24f);
+    const float x    = max3(__builtin_fabsf(p0x), __builtin_fabsf(p1x), __builtin_fabsf(p2x));
+    const float y    = max3(__builtin_fabsf(p0y), __builtin_fabsf(p1y), __builtin_fabsf(p2y));
+    const float z    = max3(__builtin_fabsf(p0z), __builtin_fabsf(p1z), __builtin_fabsf(p2z));
+    const float ex   = g5 * (x + z);
+    const float ey   = g5 * (y + z);
+    const float ee   = 2.0f * ((g2 * x * y + ey * x) + ex * y);
+    const float edge = max3(__builtin_fabsf(a), __builtin_fabsf(b), __builtin_fabsf(c));
+    const float et   = 3.0f * ((g3 * edge * z + ee * z) + g3 * z * edge) / __builtin_fabsf(det);
+    if (t <= et) return false;
+    out.t  = t;
+    out.b0 = a / det;
+    out.b1 = b / det;
+    out.b2 = c / det;
+    return true;
+}
+
+// Triangle record flags (GpuTri::pad1): bit 0 single-sided, bit 1 mirrored instance.
+constexpr uint32_t kTriSingleSided = 1u;
+constexpr uint32_t kTriMirrored    = 2u;
+
+struct StackView {
+    uint32_t* lds_ref;   // [kStackLds][block]
+    float*    lds_t;
+    uint32_t* spill_ref; // [kStackTotal - kStackLds][lanes]
+    float*    spill_t;
+    uint32_t  lane;      // threadIdx.x
+    uint32_t  block;     // blockDim.x
+    uint32_t  glane;     // global lane id (spill column)
+    uint32_t  lanes;     // total lanes of the grid (spill row pitch)
+};
+
+__device__ __forceinline__ void stack_push(const StackView& s, uint32_t& sp, uint32_t ref, float t) {
+    if (sp < kStackLds) {
+        s.lds_ref[sp * s.block + s.lane] = ref;
+        s.lds_t[sp * s.block + s.lane]   = t;
+    } else {
+        const uint32_t k = (sp - kStackLds) * s.lanes + s.glane;
+        s.spill_ref[k]   = ref;
+        s.spill_t[k]     = t;
+    }
+    ++sp;
+}
+
+__device__ __forceinline__ void stack_pop(const StackView& s, uint32_t& sp, uint32_t& ref, float& t) {
+    --sp;
+    if (sp < kStackLds) {
+        ref = s.lds_ref[sp * s.block + s.lane];
+        t   = s.lds_t[sp * s.block + s.lane];
+    } else {
+        const uint32_t k = (sp - kStackLds) * s.lanes + s.glane;
+        ref              = s.spill_ref[k];
+        t                = s.spill_t[k];
+    }
+}
+
+struct TraceResult {
+    float    t, b0, b1, b2;
+    uint32_t slot;  // leaf-order triangle slot, 0xffffffff on a miss
+    uint32_t prim;  // scene-order primitive index
+};
+
+// Closest hit (ANY = false) or any hit (ANY = true) in [tmin, tmax].
+template <bool ANY, bool STATS>
+__device__ __forceinline__ TraceResult trace_ray(const GpuScene& sc, const RayPre& r, float tmin, float tmax,
+                                                 const StackView& stk, uint64_t* bounds_tests, uint64_t* tri_tests) {
+    TraceResult res;
+    res.t    = tmax;
+    res.slot = 0xffffffffu;
+    res.prim = 0xffffffffu;
+    res.b0 = res.b1 = res.b2 = 0.0f;
+    if (sc.node_count == 0) return res;
+    uint32_t sp  = 0;
+    uint32_t ref = 0;
+    for (;;) {
+        if (!is_leaf(ref)) {
+            const float4* np = reinterpret_cast<const float4*>(sc.nodes + ref);
+            const float4  na = np[0];
+            const float4  nb = np[1];
+            const float4  nc = np[2];
+            const uint4   nd = reinterpret_cast<const uint4*>(np)[3];
+            const float   tl = box_enter(r, na.x, na.y, na.z, na.w, nb.x, nb.y, tmin, res.t);
+            const float   tr = nd.y != kEmptyRef ? box_enter(r, nb.z, nb.w, nc.x, nc.y, nc.z, nc.w, tmin, res.t)
+                                                 : __builtin_inff();
+            if (STATS) *bounds_tests += 2;
+            const bool hl = tl != __builtin_inff();
+            const bool hr = tr != __builtin_inff();
+            if (hl && hr) {
+                const bool ln = tl <= tr;
+                stack_push(stk, sp, ln ? nd.y : nd.x, ln ? tr : tl);
+                ref = ln ? nd.x : nd.y;
+                continue;
+            }
+            if (hl) {
+                ref = nd.x;
+                continue;
+            }
+            if (hr) {
+                ref = nd.y;
+                continue;
+            }
+        } else {
+            const uint32_t first = leaf_first(ref), count = leaf_count(ref);
+            for (uint32_t k = 0; k < count; ++k) {
+                const uint32_t slot = first + k;
+                const float4*  tp   = reinterpret_cast<const float4*>(sc.tris + slot);
+                const float4   a    = tp[0];
+                const float4   b    = tp[1];
+                const float4   c    = tp[2];
+                if (STATS) *tri_tests += 1;
+                TriHit h;
+                const F3 v0 = mk(a.x, a.y, a.z), v1 = mk(b.x, b.y, b.z), v2 = mk(c.x, c.y, c.z);
+                if (!tri_test(r, v0, v1, v2, tmin, res.t, h)) continue;
+                const uint32_t flags = __float_as_uint(b.w);
+                if (flags & kTriSingleSided) {
+                    const F3 e1      = v1 - v0;
+                    const F3 e2      = v2 - v0;
+                    F3       outward = cross(e1, e2) * ((flags & kTriMirrored) ? -1.0f : 1.0f);
+                    if (dot(r.d, outward) >= 0.0f) continue;
+                }
+                const uint32_t prim = __float_as_uint(a.w);
+                if (res.slot == 0xffffffffu || h.t < res.t || (h.t == res.t && prim < res.prim)) {
+                    res.t    = h.t;
+                    res.b0   = h.b0;
+                    res.b1   = h.b1;
+                    res.b2   = h.b2;
+                    res.slot = slot;
+                    res.prim = prim;
+                    if (ANY) return res;
+                }
+            }
+        }
+        // pop, skipping entries that start beyond the current closest hit
+        for (;;) {
+            if (sp == 0) return res;
+            float t;
+            stack_pop(stk, sp, ref, t);
+            if (!(t > res.t)) break;
+        }
+    }
+}
+
+}  // namespace mrg
