@@ -1,0 +1,1273 @@
+// Host side of libmrg: device context, scene upload (BVH build, light table, dedup cache), ray
+// queries, and the job scheduler that renders RenderSession-style jobs in batched wavefront passes.
+//
+// Reference interfaces replaced (see include/mrg.h): SceneBvh (src/render/bvh.cpp:31-99, 372-384),
+// LightSampler construction (src/render/light.cpp:197-264), PathTracer::render / sample_pixel
+// (src/render/path_tracer.cpp:368-405) and RenderSession (src/render/render_session.cpp:200-656).
+#include "../../include/mrg.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <limits>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "mrg_bvh.h"
+#include "mrg_kernels.h"
+#include "mrg_layout.h"
+
+using namespace mrg;
+
+namespace {
+
+thread_local std::string g_error;
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void check_hip(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw Error(MRG_DEVICE_ERROR, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class F> int guarded(F&& f) {
+    try {
+        f();
+        return MRG_OK;
+    } catch (const Error& e) {
+        g_error = e.what();
+        return e.code;
+    } catch (const std::invalid_argument& e) {
+        g_error = e.what();
+        return MRG_INVALID_ARGUMENT;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return MRG_DEVICE_ERROR;
+    }
+}
+
+using Clock = std::chrono::steady_clock;
+double seconds_since(Clock::time_point t) { return std::chrono::duration<double>(Clock::now() - t).count(); }
+
+// Device allocation owned by RAII.
+struct DevBuf {
+    void*  ptr   = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t n) { alloc(n); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void alloc(size_t n) {
+        release();
+        if (n == 0) return;
+        check_hip(hipMalloc(&ptr, n), "hipMalloc");
+        bytes = n;
+    }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr   = nullptr;
+        bytes = 0;
+    }
+    template <class T> T* as() const { return static_cast<T*>(ptr); }
+};
+
+// ---------------------------------------------------------------------------------------------
+// Float vector arithmetic in the FP32 twin's order (host side: camera constants, light table).
+struct V3f {
+    float x, y, z;
+};
+V3f   sub(V3f a, V3f b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+V3f   mul(V3f a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+V3f   divs(V3f a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+V3f   neg(V3f a) { return {-a.x, -a.y, -a.z}; }
+float dot(V3f a, V3f b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+V3f   cross(V3f a, V3f b) { return {a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y}; }
+V3f   normalize(V3f v) { return mul(v, 1.0f / std::sqrt(dot(v, v))); }
+float length(V3f v) { return std::sqrt(dot(v, v)); }
+V3f   ld(const float* p) { return {p[0], p[1], p[2]}; }
+float luminance(V3f c) { return dot(c, V3f{0.2126f, 0.7152f, 0.0722f}); }
+void  st(float* d, V3f v) { d[0] = v.x; d[1] = v.y; d[2] = v.z; }
+
+constexpr float  kPiF = 3.14159274101257324219f;
+constexpr double kPi  = 3.14159265358979323846;
+
+// 128-bit content hash (two independent 64-bit multiply-mix lanes over the bytes).
+struct Hasher {
+    uint64_t a = 0x243f6a8885a308d3ull, b = 0x13198a2e03707344ull;
+    static uint64_t mix(uint64_t v) {
+        v = (v ^ (v >> 30)) * 0xbf58476d1ce4e5b9ull;
+        v = (v ^ (v >> 27)) * 0x94d049bb133111ebull;
+        return v ^ (v >> 31);
+    }
+    void bytes(const void* p, size_t n) {
+        const auto* c = static_cast<const unsigned char*>(p);
+        size_t      i = 0;
+        for (; i + 8 <= n; i += 8) {
+            uint64_t w;
+            std::memcpy(&w, c + i, 8);
+            a = mix(a ^ w) + 0x9e3779b97f4a7c15ull;
+            b = mix(b + w * 0xff51afd7ed558ccdull) ^ (b >> 17);
+        }
+        uint64_t w = n;
+        for (; i < n; ++i) w = (w << 8) | c[i];
+        a = mix(a ^ w);
+        b = mix(b + w);
+    }
+    std::string hex() const {
+        char s[33];
+        std::snprintf(s, sizeof s, "%016llx%016llx", static_cast<unsigned long long>(a), static_cast<unsigned long long>(b));
+        return s;
+    }
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------
+struct mrg_scene {
+    mrg_device*    device = nullptr;
+    std::string    hash;
+    int            refs = 1;
+    GpuScene       gpu{};
+    mrg_scene_info info{};
+    DevBuf         nodes, tris, normals, slot_instance, slot_emitter, materials, choices, cdf, areas, directional;
+};
+
+struct mrg_job {
+    mrg_device*   device = nullptr;
+    mrg_scene*    scene  = nullptr;
+    mrg_job_desc  desc{};
+    GpuCamera     camera{};
+    uint64_t      id = 0;
+    // guarded by device->mutex
+    uint32_t      state = MRG_STATE_PREPARING;
+    bool          admitted = false, cancel_requested = false, pause_requested = false, in_flight = false;
+    int32_t       status = MRG_OK;
+    std::string   error;
+    uint32_t      next_sample = 0;   // first sample of the next chunk
+    uint32_t      next_pixel  = 0;   // pixel cursor inside the current sample range
+    uint32_t      range_samples = 0; // sample count of the current range (when pixels are split)
+    uint64_t      recorded = 0, completed_pixels = 0;
+    uint64_t      paths = 0, ext_rays = 0, shadow_rays = 0, scatter = 0, rr_term = 0;
+    Clock::time_point submitted, admitted_at, first_pass_at, finished_at;
+    bool          first_pass_done = false, finished = false;
+    uint64_t      film_bytes = 0;
+    int           refs = 1;  // caller + scheduler
+    DevBuf        film_sums, film_counts, error_flag;
+    std::mutex    film_mutex;  // held while a batch that writes this film is in flight
+};
+
+struct mrg_device {
+    int                ordinal = 0;
+    mrg_device_options opt{};
+    hipStream_t        stream = nullptr;
+    hipStream_t        copy_stream = nullptr;
+    hipDeviceProp_t    props{};
+    LaunchShape        shape{};
+    // path pool
+    Pool               pool{};
+    DevBuf             pool_mem, spill_ref, spill_t, counters, stats, header, jobs_buf, chunks_buf, query_work;
+    uint32_t           max_chunks = 0, max_jobs = 0;
+    uint64_t           pool_bytes = 0;
+    // scene cache + scheduler
+    std::mutex                         mutex;
+    std::condition_variable            cv;
+    std::map<std::string, mrg_scene*>  scenes;
+    std::deque<mrg_job*>               jobs;  // admission order
+    uint64_t                           next_job_id = 1;
+    bool                               shutdown = false;
+    std::thread                        worker;
+    uint64_t                           resident_bytes = 0;  // scenes + admitted films
+    mrg_kernel_times                   times{};
+    std::vector<hipEvent_t>            events;
+    hipEvent_t                         query_events[2] = {nullptr, nullptr};
+    std::map<uint32_t, hipGraphExec_t> graphs;  // by bounce count
+};
+
+namespace {
+
+uint64_t budget(const mrg_device* d) { return d->opt.hbm_budget_bytes ? d->opt.hbm_budget_bytes : (uint64_t(64) << 30); }
+
+void alloc_pool(mrg_device* d) {
+    const uint64_t P = d->opt.path_capacity;
+    // per slot: ray_o, ray_d, hit, beta, rad, prev, rng, sh_o, sh_d, sh_c (16 B each) + 4 u32
+    const uint64_t per = 10 * 16 + 4 * 4;
+    d->pool_mem.alloc(P * per);
+    char* base = d->pool_mem.as<char>();
+    auto take  = [&](uint64_t bytes) {
+        char* p = base;
+        base += bytes;
+        return p;
+    };
+    Pool& pl   = d->pool;
+    pl.ray_o   = reinterpret_cast<float4*>(take(P * 16));
+    pl.ray_d   = reinterpret_cast<float4*>(take(P * 16));
+    pl.hit     = reinterpret_cast<float4*>(take(P * 16));
+    pl.beta    = reinterpret_cast<float4*>(take(P * 16));
+    pl.rad     = reinterpret_cast<float4*>(take(P * 16));
+    pl.prev    = reinterpret_cast<float4*>(take(P * 16));
+    pl.rng     = reinterpret_cast<uint4*>(take(P * 16));
+    pl.sh_o    = reinterpret_cast<float4*>(take(P * 16));
+    pl.sh_d    = reinterpret_cast<float4*>(take(P * 16));
+    pl.sh_c    = reinterpret_cast<float4*>(take(P * 16));
+    pl.hit_slot = reinterpret_cast<uint32_t*>(take(P * 4));
+    pl.q_ext0  = reinterpret_cast<uint32_t*>(take(P * 4));
+    pl.q_ext1  = reinterpret_cast<uint32_t*>(take(P * 4));
+    pl.q_sh    = reinterpret_cast<uint32_t*>(take(P * 4));
+    const uint64_t lanes = uint64_t(std::max(d->shape.traversal_blocks, 1u)) * 256;
+    d->spill_ref.alloc(lanes * (kStackTotal - kStackLds) * 4);
+    d->spill_t.alloc(lanes * (kStackTotal - kStackLds) * 4);
+    d->counters.alloc(kCounterWords * 4);
+    d->stats.alloc(kStatWords * 8);
+    d->header.alloc(kHdrWords * 4);
+    pl.spill_ref   = d->spill_ref.as<uint32_t>();
+    pl.spill_t     = d->spill_t.as<float>();
+    pl.spill_lanes = uint32_t(lanes);
+    pl.counters    = d->counters.as<uint32_t>();
+    pl.stats       = d->stats.as<uint64_t>();
+    pl.header      = d->header.as<uint32_t>();
+    pl.capacity    = uint32_t(P);
+    d->max_jobs    = std::max(1u, d->opt.max_jobs_in_flight);
+    d->max_chunks  = d->max_jobs * 4;
+    d->jobs_buf.alloc(sizeof(GpuJob) * d->max_jobs);
+    d->chunks_buf.alloc(sizeof(GpuChunk) * d->max_chunks);
+    d->query_work.alloc(64);
+    d->pool_bytes = d->pool_mem.bytes + d->spill_ref.bytes + d->spill_t.bytes;
+}
+
+// Camera constants: CameraRays<float> of the oracle (scene.cpp:171-199 in FP32; tan in FP64).
+GpuCamera make_camera(const mrg_camera& c) {
+    const V3f p = ld(c.position), t = ld(c.target), u = ld(c.up);
+    if (!(std::isfinite(p.x) && std::isfinite(p.y) && std::isfinite(p.z) && std::isfinite(t.x) && std::isfinite(t.y) &&
+          std::isfinite(t.z) && std::isfinite(u.x) && std::isfinite(u.y) && std::isfinite(u.z)))
+        throw std::invalid_argument("Camera vectors must be finite");
+    if (!(length(sub(t, p)) > 1e-6f)) throw std::invalid_argument("Camera position and target must differ");
+    if (!(length(cross(sub(t, p), u)) > 1e-6f)) throw std::invalid_argument("Camera up must not be parallel to view direction");
+    if (!(c.vertical_fov_degrees > 0 && c.vertical_fov_degrees < 179)) throw std::invalid_argument("Camera FOV must be between 0 and 179 degrees");
+    if (!(std::isfinite(c.near_plane) && std::isfinite(c.far_plane) && c.near_plane > 0 && c.far_plane > c.near_plane))
+        throw std::invalid_argument("Camera clipping planes must satisfy 0 < near < far");
+    if (!(std::isfinite(c.aperture_radius) && c.aperture_radius >= 0 && std::isfinite(c.focus_distance) && c.focus_distance > 0))
+        throw std::invalid_argument("Camera lens requires nonnegative aperture and positive focus distance");
+    GpuCamera g{};
+    const V3f fwd   = normalize(sub(t, p));
+    const V3f right = normalize(cross(fwd, u));
+    const V3f up    = cross(right, fwd);
+    st(g.position, p);
+    st(g.forward, fwd);
+    st(g.right, right);
+    st(g.up, up);
+    g.half_height = static_cast<float>(std::tan(static_cast<double>(c.vertical_fov_degrees) * (kPi / 180.0) * 0.5));
+    g.aperture    = c.aperture_radius;
+    g.focus       = c.focus_distance;
+    g.near_plane  = c.near_plane;
+    g.far_plane   = c.far_plane;
+    return g;
+}
+
+template <class T> void upload(DevBuf& b, const std::vector<T>& v) {
+    b.alloc(std::max<size_t>(v.size() * sizeof(T), 16));
+    if (!v.empty()) check_hip(hipMemcpy(b.ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy");
+}
+
+mrg_scene* build_scene(mrg_device* d, const mrg_scene_desc& s, const std::string& hash) {
+    const auto t0 = Clock::now();
+    const uint64_t n = s.triangle_count;
+    if (n && (!s.positions || !s.normals || !s.instance || !s.triangle)) throw std::invalid_argument("scene arrays are missing");
+    if (s.instance_count && !s.materials) throw std::invalid_argument("materials are missing");
+    if (s.directional_count && !s.directional) throw std::invalid_argument("directional lights are missing");
+    for (uint64_t p = 0; p < n; ++p) {
+        if (s.instance[p] >= s.instance_count) throw std::invalid_argument("instance index out of range");
+        const float* v = s.positions + p * 9;
+        const double e1[3] = {double(v[3]) - v[0], double(v[4]) - v[1], double(v[5]) - v[2]};
+        const double e2[3] = {double(v[6]) - v[0], double(v[7]) - v[1], double(v[8]) - v[2]};
+        const double c[3]  = {e1[1] * e2[2] - e2[1] * e1[2], e1[2] * e2[0] - e2[2] * e1[0], e1[0] * e2[1] - e2[0] * e1[1]};
+        for (int k = 0; k < 9; ++k)
+            if (!std::isfinite(v[k])) throw std::invalid_argument("Transformed triangle position is not finite");
+        if ((c[0] * c[0] + c[1] * c[1]) + c[2] * c[2] == 0) throw std::invalid_argument("Transform collapses a triangle at float precision");
+    }
+    for (uint32_t i = 0; i < s.instance_count; ++i) {
+        const mrg_material& m = s.materials[i];
+        if (m.kind > 3) throw std::invalid_argument("Unsupported material kind");
+        for (int k = 0; k < 3; ++k) {
+            if (!(m.base_color[k] >= 0 && m.base_color[k] <= 1)) throw std::invalid_argument("Material base color must be in [0,1]");
+            if (!(m.emission[k] >= 0 && std::isfinite(m.emission[k]))) throw std::invalid_argument("Material emission must be nonnegative");
+        }
+        if (!(m.roughness >= 0 && m.roughness <= 1)) throw std::invalid_argument("Roughness must be in [0,1]");
+        if (!(m.ior > 0 && std::isfinite(m.ior))) throw std::invalid_argument("IOR must be positive");
+    }
+    auto sc    = std::make_unique<mrg_scene>();
+    sc->device = d;
+    sc->hash   = hash;
+    const BvhBuildResult bvh = build_bvh(s.positions, n, 0);
+    if (bvh.max_depth > kMaxDepth) throw Error(MRG_DEVICE_ERROR, "BVH depth exceeds the traversal stack bound");
+    // leaf-order triangle records
+    std::vector<GpuTri>   tris(n), nrms(n);
+    std::vector<uint32_t> slot_instance(n);
+    std::vector<int32_t>  slot_emitter(n, -1);
+    // light table (LightSampler<float> of the oracle, light.cpp:197-264)
+    std::vector<GpuLightChoice> choices;
+    std::vector<GpuAreaLight>   areas;
+    std::vector<GpuDirectional> dirs;
+    std::vector<float>          weights;
+    std::vector<int32_t>        area_choice(n, -1);
+    const V3f env      = ld(s.environment);
+    const float env_integral = 4.0f * kPiF * luminance(env);
+    float lo[3] = {std::numeric_limits<float>::infinity(), std::numeric_limits<float>::infinity(), std::numeric_limits<float>::infinity()};
+    float hi[3] = {-lo[0], -lo[1], -lo[2]};
+    for (uint64_t p = 0; p < n; ++p) {
+        const mrg_material& m = s.materials[s.instance[p]];
+        const V3f radiance    = ld(m.emission);
+        V3f v[3];
+        for (int k = 0; k < 3; ++k) {
+            v[k] = ld(s.positions + p * 9 + k * 3);
+            lo[0] = std::min(lo[0], v[k].x); lo[1] = std::min(lo[1], v[k].y); lo[2] = std::min(lo[2], v[k].z);
+            hi[0] = std::max(hi[0], v[k].x); hi[1] = std::max(hi[1], v[k].y); hi[2] = std::max(hi[2], v[k].z);
+        }
+        if (luminance(radiance) == 0) continue;
+        const V3f   c    = cross(sub(v[1], v[0]), sub(v[2], v[0]));
+        const float area = length(c) * 0.5f;
+        if (area == 0) continue;
+        area_choice[p] = int32_t(choices.size());
+        choices.push_back({0u, uint32_t(areas.size())});
+        GpuAreaLight a{};
+        st(a.v0, v[0]);
+        st(a.v1, v[1]);
+        st(a.v2, v[2]);
+        st(a.normal, divs(mul(c, m.mirrored ? -1.0f : 1.0f), 2.0f * area));
+        st(a.radiance, radiance);
+        a.area         = area;
+        a.double_sided = m.double_sided ? 1u : 0u;
+        areas.push_back(a);
+        weights.push_back(float(m.double_sided ? 2 : 1) * kPiF * area * luminance(radiance));
+    }
+    const bool empty_bounds = n == 0;
+    const V3f  extent       = empty_bounds ? V3f{0, 0, 0} : V3f{hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};
+    float      q            = dot(extent, extent) * 0.25f;
+    q                       = q < 1e-12f ? 1e-12f : q;
+    const float projected   = kPiF * (empty_bounds ? 1.0f : q);
+    for (uint32_t i = 0; i < s.directional_count; ++i) {
+        const mrg_directional_light& l = s.directional[i];
+        const V3f radiance = mul(ld(l.color), l.intensity);
+        if (luminance(radiance) > 0) {
+            choices.push_back({1u, uint32_t(dirs.size())});
+            GpuDirectional g{};
+            st(g.direction, neg(normalize(ld(l.direction))));
+            st(g.radiance, radiance);
+            dirs.push_back(g);
+            weights.push_back(projected * luminance(radiance));
+        }
+    }
+    int32_t env_choice = -1;
+    if (env_integral > 0) {
+        env_choice = int32_t(choices.size());
+        choices.push_back({2u, 0u});
+        weights.push_back(projected * env_integral);
+    }
+    // DiscreteDistribution (sampling.cpp:109-130) in float
+    std::vector<float> cdf(weights.size() + 1, 0.0f);
+    float maximum = 0;
+    for (float w : weights) {
+        if (!std::isfinite(w) || w < 0) throw std::invalid_argument("Distribution weights must be finite and nonnegative");
+        maximum = maximum < w ? w : maximum;
+    }
+    for (size_t i = 0; i < weights.size(); ++i) cdf[i + 1] = cdf[i] + (maximum > 0 ? weights[i] / maximum : 1.0f);
+    if (!weights.empty()) {
+        const float integral = cdf.back();
+        for (auto& v : cdf) v /= integral;
+    }
+    for (uint64_t slot = 0; slot < n; ++slot) {
+        const uint32_t p = bvh.order[slot];
+        const mrg_material& m = s.materials[s.instance[p]];
+        const float* v = s.positions + uint64_t(p) * 9;
+        const float* nm = s.normals + uint64_t(p) * 9;
+        GpuTri& t = tris[slot];
+        t.v0[0] = v[0]; t.v0[1] = v[1]; t.v0[2] = v[2];
+        t.v1[0] = v[3]; t.v1[1] = v[4]; t.v1[2] = v[5];
+        t.v2[0] = v[6]; t.v2[1] = v[7]; t.v2[2] = v[8];
+        t.prim = p;
+        t.pad1 = (m.double_sided ? 0u : 1u) | (m.mirrored ? 2u : 0u);
+        t.pad2 = 0;
+        GpuTri& r = nrms[slot];
+        r.v0[0] = nm[0]; r.v0[1] = nm[1]; r.v0[2] = nm[2];
+        r.v1[0] = nm[3]; r.v1[1] = nm[4]; r.v1[2] = nm[5];
+        r.v2[0] = nm[6]; r.v2[1] = nm[7]; r.v2[2] = nm[8];
+        r.prim = p;
+        r.pad1 = r.pad2 = 0;
+        slot_instance[slot] = s.instance[p];
+        slot_emitter[slot]  = area_choice[p];
+    }
+    std::vector<GpuMaterial> mats(s.instance_count);
+    for (uint32_t i = 0; i < s.instance_count; ++i) {
+        const mrg_material& m = s.materials[i];
+        GpuMaterial&        g = mats[i];
+        g.kind = m.kind;
+        std::memcpy(g.color, m.base_color, 12);
+        std::memcpy(g.emission, m.emission, 12);
+        g.roughness    = m.roughness;
+        g.alpha        = m.roughness * m.roughness;
+        g.ior          = m.ior;
+        g.double_sided = m.double_sided;
+        g.mirrored     = m.mirrored;
+    }
+    check_hip(hipSetDevice(d->ordinal), "hipSetDevice");
+    upload(sc->nodes, bvh.nodes);
+    upload(sc->tris, tris);
+    upload(sc->normals, nrms);
+    upload(sc->slot_instance, slot_instance);
+    upload(sc->slot_emitter, slot_emitter);
+    upload(sc->materials, mats);
+    upload(sc->choices, choices);
+    upload(sc->cdf, cdf);
+    upload(sc->areas, areas);
+    upload(sc->directional, dirs);
+    GpuScene& g     = sc->gpu;
+    g.nodes         = sc->nodes.as<GpuNode>();
+    g.tris          = sc->tris.as<GpuTri>();
+    g.normals       = sc->normals.as<GpuTri>();
+    g.slot_instance = sc->slot_instance.as<uint32_t>();
+    g.slot_emitter  = sc->slot_emitter.as<int32_t>();
+    g.materials     = sc->materials.as<GpuMaterial>();
+    g.choices       = sc->choices.as<GpuLightChoice>();
+    g.cdf           = sc->cdf.as<float>();
+    g.areas         = sc->areas.as<GpuAreaLight>();
+    g.directional   = sc->directional.as<GpuDirectional>();
+    g.node_count    = uint32_t(bvh.nodes.size());
+    g.tri_count     = uint32_t(n);
+    g.choice_count  = uint32_t(choices.size());
+    g.env_choice    = env_choice;
+    st(g.env, env);
+    g.env_pdf = env_choice >= 0 ? (cdf[env_choice + 1] - cdf[env_choice]) * (1.0f / (4.0f * kPiF)) : 0.0f;
+    mrg_scene_info& info = sc->info;
+    info.triangles       = n;
+    info.nodes           = bvh.nodes.size();
+    info.leaves          = bvh.leaves;
+    info.maximum_depth   = bvh.max_depth;
+    info.device_bytes    = sc->nodes.bytes + sc->tris.bytes + sc->normals.bytes + sc->slot_instance.bytes + sc->slot_emitter.bytes +
+                        sc->materials.bytes + sc->choices.bytes + sc->cdf.bytes + sc->areas.bytes + sc->directional.bytes;
+    info.build_seconds = seconds_since(t0);
+    std::snprintf(info.content_hash, sizeof info.content_hash, "%s", hash.c_str());
+    return sc.release();
+}
+
+// ---------------------------------------------------------------------------------------------
+// Scheduler.
+
+bool terminal(uint32_t s) {
+    return s == MRG_STATE_IDLE || s == MRG_STATE_CANCELLED || s == MRG_STATE_COMPLETED || s == MRG_STATE_FAILED;
+}
+
+uint64_t film_bytes_of(const mrg_job_desc& d) { return uint64_t(d.width) * d.height * (24 + 4); }
+
+void release_job_locked(mrg_job* j);
+
+void finish_job_locked(mrg_device* d, mrg_job* j, uint32_t state) {
+    j->state       = state;
+    j->finished    = true;
+    j->finished_at = Clock::now();
+    if (j->admitted) {
+        d->resident_bytes -= std::min(d->resident_bytes, j->film_bytes);
+        j->admitted = false;
+    }
+    d->cv.notify_all();
+}
+
+// Admission in submission order while the budget allows (films; scenes are already resident).
+void admit_locked(mrg_device* d) {
+    for (mrg_job* j : d->jobs) {
+        if (j->admitted || j->finished) continue;
+        if (j->cancel_requested) {
+            finish_job_locked(d, j, MRG_STATE_CANCELLED);
+            continue;
+        }
+        if (d->resident_bytes + j->film_bytes + d->pool_bytes > budget(d)) break;  // FIFO: later jobs wait too
+        j->film_sums.alloc(uint64_t(j->desc.width) * j->desc.height * 3 * 8);
+        j->film_counts.alloc(uint64_t(j->desc.width) * j->desc.height * 4);
+        j->error_flag.alloc(16);
+        check_hip(hipMemsetAsync(j->film_sums.ptr, 0, j->film_sums.bytes, d->stream), "hipMemsetAsync");
+        check_hip(hipMemsetAsync(j->film_counts.ptr, 0, j->film_counts.bytes, d->stream), "hipMemsetAsync");
+        check_hip(hipMemsetAsync(j->error_flag.ptr, 0, 16, d->stream), "hipMemsetAsync");
+        j->admitted    = true;
+        j->admitted_at = Clock::now();
+        d->resident_bytes += j->film_bytes;
+        if (j->state == MRG_STATE_PREPARING) j->state = j->pause_requested ? MRG_STATE_PAUSED : MRG_STATE_RUNNING;
+    }
+}
+
+struct BatchPlan {
+    std::vector<mrg_job*>  jobs;
+    std::vector<GpuJob>    gjobs;
+    std::vector<GpuChunk>  chunks;
+    uint32_t               slots = 0, pixels = 0, max_depth = 0;
+};
+
+bool runnable(const mrg_job* j) {
+    return j->admitted && !j->finished && !j->cancel_requested && !j->pause_requested && j->state == MRG_STATE_RUNNING &&
+           j->next_sample < j->desc.samples_per_pixel;
+}
+
+uint32_t pass_of(const mrg_job* j) {
+    return j->desc.pass_samples ? j->desc.pass_samples : 16u;
+}
+
+// Adds the next chunk of job `j` within `room` path slots; returns false when nothing fits.
+bool plan_chunk(mrg_job* j, uint32_t job_slot, uint32_t room, bool fill, BatchPlan& b) {
+    const uint32_t pixels    = j->desc.width * j->desc.height;
+    const uint32_t remaining = j->desc.samples_per_pixel - j->next_sample;
+    GpuChunk c{};
+    c.job = job_slot;
+    if (j->next_pixel == 0 && pixels <= room) {
+        uint32_t k = std::min(remaining, pass_of(j));
+        if (fill) k = std::min(remaining, std::max(k, room / pixels));
+        k = std::min(k, room / pixels);
+        if (k == 0) return false;
+        c.pixel_begin  = 0;
+        c.pixel_count  = pixels;
+        c.sample_begin = j->next_sample;
+        c.sample_count = k;
+        j->next_sample += k;
+    } else {
+        // pixel-split range: fixed sample count for the whole range
+        if (j->next_pixel == 0) j->range_samples = std::min(remaining, pass_of(j));
+        const uint32_t k     = j->range_samples;
+        const uint32_t count = std::min(pixels - j->next_pixel, room / k);
+        if (count == 0) return false;
+        c.pixel_begin  = j->next_pixel;
+        c.pixel_count  = count;
+        c.sample_begin = j->next_sample;
+        c.sample_count = k;
+        j->next_pixel += count;
+        if (j->next_pixel == pixels) {
+            j->next_pixel = 0;
+            j->next_sample += k;
+        }
+    }
+    c.slot_begin   = b.slots;
+    c.pixel_prefix = b.pixels;
+    b.slots += c.pixel_count * c.sample_count;
+    b.pixels += c.pixel_count;
+    b.chunks.push_back(c);
+    return true;
+}
+
+GpuJob gpu_job(mrg_job* j) {
+    GpuJob g{};
+    g.camera      = j->camera;
+    g.scene       = j->scene->gpu;
+    g.width       = j->desc.width;
+    g.height      = j->desc.height;
+    g.mode        = j->desc.mode;
+    g.max_depth   = j->desc.maximum_depth;
+    g.rr          = j->desc.russian_roulette;
+    g.rr_depth    = j->desc.roulette_depth;
+    g.seed        = j->desc.seed;
+    g.film_sums   = j->film_sums.as<double>();
+    g.film_counts = j->film_counts.as<uint32_t>();
+    g.error_flag  = j->error_flag.as<uint32_t>();
+    return g;
+}
+
+BatchPlan plan_batch_locked(mrg_device* d) {
+    BatchPlan b;
+    const uint32_t cap      = d->pool.capacity;
+    const bool     batched  = d->opt.schedule == MRG_SCHEDULE_BATCHED;
+    const uint32_t max_jobs = batched ? d->max_jobs : 1u;
+    for (mrg_job* j : d->jobs) {
+        if (b.jobs.size() >= max_jobs || b.slots >= cap) break;
+        if (!runnable(j)) continue;
+        const uint32_t slot = uint32_t(b.jobs.size());
+        // one chunk per job per batch: a pixel never appears twice in one accumulate launch
+        if (!plan_chunk(j, slot, cap - b.slots, !batched, b)) {
+            if (b.jobs.empty()) continue;
+            break;
+        }
+        b.jobs.push_back(j);
+        b.gjobs.push_back(gpu_job(j));
+        b.max_depth = std::max(b.max_depth, j->desc.maximum_depth);
+    }
+    if (batched && !b.jobs.empty() && b.slots < cap) {
+        // spread the remaining capacity over whole-image chunks by raising their sample counts
+        bool grew = true;
+        while (grew && b.slots < cap) {
+            grew = false;
+            for (size_t i = 0; i < b.chunks.size(); ++i) {
+                GpuChunk& c = b.chunks[i];
+                mrg_job*  j = b.jobs[c.job];
+                const uint32_t pixels = j->desc.width * j->desc.height;
+                if (c.pixel_count != pixels || j->next_sample >= j->desc.samples_per_pixel) continue;
+                const uint32_t add = std::min({pass_of(j), j->desc.samples_per_pixel - j->next_sample, (cap - b.slots) / pixels});
+                if (add == 0) continue;
+                c.sample_count += add;
+                j->next_sample += add;
+                b.slots += add * pixels;
+                grew = true;
+            }
+        }
+        // recompute slot offsets
+        uint32_t s = 0;
+        for (auto& c : b.chunks) {
+            c.slot_begin = s;
+            s += c.pixel_count * c.sample_count;
+        }
+    }
+    for (mrg_job* j : b.jobs) {
+        j->in_flight = true;
+        j->refs++;
+    }
+    return b;
+}
+
+void launch_iteration_kernels(mrg_device* d, uint32_t iters, bool timed, std::vector<std::pair<int, int>>& marks) {
+    const GpuJob*   jobs   = d->jobs_buf.as<GpuJob>();
+    const GpuChunk* chunks = d->chunks_buf.as<GpuChunk>();
+    int             ev     = 0;
+    auto            rec    = [&](int cls) {
+        if (!timed) return;
+        check_hip(hipEventRecord(d->events[ev], d->stream), "hipEventRecord");
+        marks.push_back({ev, cls});
+        ++ev;
+    };
+    check_hip(launch_reset(d->pool, d->stream), "k_reset");
+    rec(-1);
+    check_hip(launch_generate(d->pool, jobs, chunks, d->shape, d->stream), "k_generate");
+    rec(0);
+    for (uint32_t it = 0; it < iters; ++it) {
+        check_hip(launch_extend(d->pool, jobs, it, d->shape, d->stream), "k_extend");
+        rec(1);
+        check_hip(launch_shade(d->pool, jobs, it, d->shape, d->stream), "k_shade");
+        rec(2);
+        check_hip(launch_shadow(d->pool, jobs, it, d->shape, d->stream), "k_shadow");
+        rec(3);
+    }
+    check_hip(launch_accumulate(d->pool, jobs, chunks, d->shape, d->stream), "k_accumulate");
+    rec(4);
+}
+
+void run_batch(mrg_device* d, BatchPlan& b) {
+    const uint32_t iters = b.max_depth + 1;
+    uint32_t       hdr[kHdrWords] = {uint32_t(b.chunks.size()), b.slots, b.pixels, iters};
+    check_hip(hipMemcpyAsync(d->jobs_buf.ptr, b.gjobs.data(), b.gjobs.size() * sizeof(GpuJob), hipMemcpyHostToDevice, d->stream), "upload jobs");
+    check_hip(hipMemcpyAsync(d->chunks_buf.ptr, b.chunks.data(), b.chunks.size() * sizeof(GpuChunk), hipMemcpyHostToDevice, d->stream), "upload chunks");
+    check_hip(hipMemcpyAsync(d->header.ptr, hdr, sizeof hdr, hipMemcpyHostToDevice, d->stream), "upload header");
+    std::vector<std::pair<int, int>> marks;
+    const bool timed = d->opt.use_graph == 0;
+    if (d->opt.use_graph) {
+        auto it = d->graphs.find(iters);
+        if (it == d->graphs.end()) {
+            hipGraph_t graph;
+            check_hip(hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+            launch_iteration_kernels(d, iters, false, marks);
+            check_hip(hipStreamEndCapture(d->stream, &graph), "hipStreamEndCapture");
+            hipGraphExec_t exec;
+            check_hip(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+            (void)hipGraphDestroy(graph);
+            it = d->graphs.emplace(iters, exec).first;
+        }
+        check_hip(hipGraphLaunch(it->second, d->stream), "hipGraphLaunch");
+    } else {
+        launch_iteration_kernels(d, iters, timed, marks);
+    }
+    std::vector<uint32_t> ctr(kCounterWords);
+    uint64_t              st[kStatWords];
+    check_hip(hipMemcpyAsync(ctr.data(), d->counters.ptr, kCounterWords * 4, hipMemcpyDeviceToHost, d->stream), "read counters");
+    check_hip(hipMemcpyAsync(st, d->stats.ptr, sizeof st, hipMemcpyDeviceToHost, d->stream), "read stats");
+    std::vector<uint32_t> flags(b.jobs.size() * 4);
+    for (size_t i = 0; i < b.jobs.size(); ++i)
+        check_hip(hipMemcpyAsync(&flags[i * 4], b.jobs[i]->error_flag.ptr, 16, hipMemcpyDeviceToHost, d->stream), "read flags");
+    check_hip(hipStreamSynchronize(d->stream), "hipStreamSynchronize");
+    uint64_t ext = 0, sh = 0;
+    for (uint32_t i = 0; i < iters; ++i) {
+        ext += ctr[kCtrExt + i];
+        sh += ctr[kCtrShadow + i];
+    }
+    double acc[5] = {0, 0, 0, 0, 0};
+    if (timed && !marks.empty()) {
+        for (size_t i = 1; i < marks.size(); ++i) {
+            float ms = 0;
+            check_hip(hipEventElapsedTime(&ms, d->events[marks[i - 1].first], d->events[marks[i].first]), "hipEventElapsedTime");
+            acc[marks[i].second] += ms;
+        }
+    }
+    std::lock_guard<std::mutex> lock(d->mutex);
+    d->times.generate_ms += acc[0];
+    d->times.extend_ms += acc[1];
+    d->times.shade_ms += acc[2];
+    d->times.shadow_ms += acc[3];
+    d->times.accumulate_ms += acc[4];
+    d->times.batches++;
+    d->times.extension_rays += ext;
+    d->times.shadow_rays += sh;
+    const bool shared = b.jobs.size() > 1;
+    for (size_t i = 0; i < b.jobs.size(); ++i) {
+        mrg_job* j = b.jobs[i];
+        uint64_t samples = 0;
+        for (const auto& c : b.chunks)
+            if (c.job == i) samples += uint64_t(c.pixel_count) * c.sample_count;
+        j->recorded += samples;
+        j->paths += samples;
+        if (!shared) {
+            j->ext_rays += ext;
+            j->shadow_rays += st[kStatShadowRays];
+            j->scatter += st[kStatScatter];
+            j->rr_term += st[kStatRoulette];
+        }
+        const uint64_t pixels = uint64_t(j->desc.width) * j->desc.height;
+        j->completed_pixels   = j->next_pixel == 0 && j->next_sample == j->desc.samples_per_pixel ? pixels : j->completed_pixels;
+        if (!j->first_pass_done && (j->next_sample >= std::min(pass_of(j), j->desc.samples_per_pixel)) && j->next_pixel == 0) {
+            j->first_pass_done = true;
+            j->first_pass_at   = Clock::now();
+        }
+        j->in_flight = false;
+        if (flags[i * 4] != 0) {
+            j->status = MRG_NONFINITE;
+            j->error  = (flags[i * 4] & 1u) ? "Path contribution is nonfinite or negative" : "Render sample exceeds the float image range";
+            finish_job_locked(d, j, MRG_STATE_FAILED);
+        } else if (j->cancel_requested) {
+            j->status = MRG_CANCELLED;
+            finish_job_locked(d, j, MRG_STATE_CANCELLED);
+        } else if (j->next_sample >= j->desc.samples_per_pixel && j->next_pixel == 0) {
+            j->state = MRG_STATE_FINALIZING;
+            finish_job_locked(d, j, MRG_STATE_COMPLETED);
+        } else if (j->pause_requested) {
+            j->state = MRG_STATE_PAUSED;
+        }
+        j->film_mutex.unlock();
+        release_job_locked(j);
+    }
+    d->cv.notify_all();
+}
+
+void scheduler(mrg_device* d) {
+    (void)hipSetDevice(d->ordinal);
+    for (;;) {
+        BatchPlan b;
+        {
+            std::unique_lock<std::mutex> lock(d->mutex);
+            for (;;) {
+                if (d->shutdown) return;
+                try {
+                    admit_locked(d);
+                } catch (const std::exception& e) {
+                    for (mrg_job* j : d->jobs)
+                        if (!j->admitted && !j->finished) {
+                            j->status = MRG_DEVICE_ERROR;
+                            j->error  = e.what();
+                            finish_job_locked(d, j, MRG_STATE_FAILED);
+                            break;
+                        }
+                }
+                // settle pause/cancel of jobs not in flight
+                for (mrg_job* j : d->jobs) {
+                    if (j->finished || j->in_flight) continue;
+                    if (j->cancel_requested && j->admitted) {
+                        j->status = MRG_CANCELLED;
+                        finish_job_locked(d, j, MRG_STATE_CANCELLED);
+                    } else if (j->pause_requested && j->state == MRG_STATE_PAUSING) {
+                        j->state = MRG_STATE_PAUSED;
+                        d->cv.notify_all();
+                    }
+                }
+                // drop finished jobs from the queue (the caller's handle stays valid)
+                for (auto it = d->jobs.begin(); it != d->jobs.end();) {
+                    if ((*it)->finished) {
+                        mrg_job* j = *it;
+                        it         = d->jobs.erase(it);
+                        release_job_locked(j);
+                    } else {
+                        ++it;
+                    }
+                }
+                b = plan_batch_locked(d);
+                if (!b.jobs.empty()) break;
+                d->cv.wait(lock);
+            }
+        }
+        for (mrg_job* j : b.jobs) j->film_mutex.lock();
+        try {
+            run_batch(d, b);
+        } catch (const std::exception& e) {
+            std::lock_guard<std::mutex> lock(d->mutex);
+            for (mrg_job* j : b.jobs) {
+                j->in_flight = false;
+                j->status    = MRG_DEVICE_ERROR;
+                j->error     = e.what();
+                if (!j->finished) finish_job_locked(d, j, MRG_STATE_FAILED);
+                j->film_mutex.unlock();
+                release_job_locked(j);
+            }
+        }
+    }
+}
+
+void release_job_locked(mrg_job* j) {
+    if (--j->refs == 0) {
+        mrg_scene* s = j->scene;
+        delete j;
+        if (s && --s->refs == 0) {
+            s->device->scenes.erase(s->hash);
+            s->device->resident_bytes -= std::min(s->device->resident_bytes, s->info.device_bytes);
+            delete s;
+        }
+    }
+}
+
+void validate_job(const mrg_job_desc& d) {
+    if (d.width == 0 || d.height == 0 || d.width > 16384 || d.height > 16384 || uint64_t(d.width) * d.height > 64ull * 1024 * 1024 ||
+        d.samples_per_pixel == 0)
+        throw std::invalid_argument("Render dimensions or sample count exceed the image budget");
+    if (d.mode > 2) throw std::invalid_argument("Sampling mode must be bsdf, light or mis");
+    if (d.maximum_depth > 128 || d.roulette_depth > 128) throw std::invalid_argument("Path and roulette depths must not exceed 128");
+}
+
+}  // namespace
+
+// =============================================================================================
+extern "C" {
+
+const char* mrg_last_error(void) { return g_error.c_str(); }
+const char* mrg_version(void) { return "mrg 0.1 (gfx950 wavefront path tracer)"; }
+
+int mrg_device_open(int ordinal, const mrg_device_options* options, mrg_device** out) {
+    return guarded([&] {
+        if (!out) throw std::invalid_argument("out is null");
+        int count = 0;
+        check_hip(hipGetDeviceCount(&count), "hipGetDeviceCount");
+        if (ordinal < 0 || ordinal >= count) throw std::invalid_argument("device ordinal out of range");
+        auto d     = std::make_unique<mrg_device>();
+        d->ordinal = ordinal;
+        if (options) d->opt = *options;
+        check_hip(hipSetDevice(ordinal), "hipSetDevice");
+        check_hip(hipGetDeviceProperties(&d->props, ordinal), "hipGetDeviceProperties");
+        check_hip(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking), "hipStreamCreate");
+        check_hip(hipStreamCreateWithFlags(&d->copy_stream, hipStreamNonBlocking), "hipStreamCreate");
+        const uint32_t cus = uint32_t(std::max(1, d->props.multiProcessorCount));
+        const uint32_t per = d->opt.persistent_blocks_per_cu ? d->opt.persistent_blocks_per_cu : 4u;
+        d->shape           = LaunchShape{cus, cus * per, cus * 8};
+        if (d->opt.path_capacity == 0) d->opt.path_capacity = uint64_t(1) << 23;
+        if (d->opt.path_capacity > (uint64_t(1) << 30)) throw std::invalid_argument("path capacity too large");
+        if (d->opt.max_jobs_in_flight == 0) d->opt.max_jobs_in_flight = 64;
+        if (d->opt.max_jobs_in_flight > 4096) throw std::invalid_argument("max_jobs_in_flight too large");
+        alloc_pool(d.get());
+        d->events.resize(2 * (3 * kMaxBounces + 8));
+        for (auto& e : d->events) check_hip(hipEventCreate(&e), "hipEventCreate");
+        for (auto& e : d->query_events) check_hip(hipEventCreate(&e), "hipEventCreate");
+        d->resident_bytes = 0;
+        mrg_device* raw   = d.release();
+        raw->worker       = std::thread(scheduler, raw);
+        *out              = raw;
+    });
+}
+
+int mrg_device_close(mrg_device* d) {
+    return guarded([&] {
+        if (!d) return;
+        {
+            std::lock_guard<std::mutex> lock(d->mutex);
+            for (mrg_job* j : d->jobs) j->cancel_requested = true;
+            d->shutdown = true;
+            d->cv.notify_all();
+        }
+        if (d->worker.joinable()) d->worker.join();
+        (void)hipSetDevice(d->ordinal);
+        (void)hipStreamSynchronize(d->stream);
+        for (auto& g : d->graphs) (void)hipGraphExecDestroy(g.second);
+        for (auto& e : d->events) (void)hipEventDestroy(e);
+        for (auto& e : d->query_events) (void)hipEventDestroy(e);
+        (void)hipStreamDestroy(d->stream);
+        (void)hipStreamDestroy(d->copy_stream);
+        delete d;  // remaining scenes/jobs owned by callers are leaked intentionally only if unreleased
+    });
+}
+
+int mrg_device_query(mrg_device* d, char* name, size_t cap, uint32_t* cus, uint64_t* hbm, uint64_t* l2) {
+    return guarded([&] {
+        if (!d) throw std::invalid_argument("device is null");
+        if (name && cap) std::snprintf(name, cap, "%s (%s)", d->props.name, d->props.gcnArchName);
+        if (cus) *cus = uint32_t(d->props.multiProcessorCount);
+        if (hbm) *hbm = d->props.totalGlobalMem;
+        if (l2) *l2 = uint64_t(d->props.l2CacheSize);
+    });
+}
+
+int mrg_scene_upload(mrg_device* d, const mrg_scene_desc* s, mrg_scene** out) {
+    return guarded([&] {
+        if (!d || !s || !out) throw std::invalid_argument("null argument");
+        Hasher h;
+        const uint64_t n = s->triangle_count;
+        h.bytes(&n, 8);
+        if (n) {
+            h.bytes(s->positions, n * 36);
+            h.bytes(s->normals, n * 36);
+            h.bytes(s->instance, n * 4);
+            h.bytes(s->triangle, n * 4);
+        }
+        h.bytes(&s->instance_count, 4);
+        if (s->instance_count) h.bytes(s->materials, s->instance_count * sizeof(mrg_material));
+        h.bytes(s->environment, 12);
+        h.bytes(&s->directional_count, 4);
+        if (s->directional_count) h.bytes(s->directional, s->directional_count * sizeof(mrg_directional_light));
+        const std::string key = h.hex();
+        {
+            std::lock_guard<std::mutex> lock(d->mutex);
+            auto it = d->scenes.find(key);
+            if (it != d->scenes.end()) {
+                it->second->refs++;
+                *out = it->second;
+                return;
+            }
+        }
+        mrg_scene* sc = build_scene(d, *s, key);
+        std::lock_guard<std::mutex> lock(d->mutex);
+        auto it = d->scenes.find(key);
+        if (it != d->scenes.end()) {  // raced with an identical upload
+            delete sc;
+            it->second->refs++;
+            *out = it->second;
+            return;
+        }
+        d->scenes[key] = sc;
+        d->resident_bytes += sc->info.device_bytes;
+        *out = sc;
+    });
+}
+
+int mrg_scene_release(mrg_scene* s) {
+    return guarded([&] {
+        if (!s) return;
+        mrg_device*                 d = s->device;
+        std::lock_guard<std::mutex> lock(d->mutex);
+        if (--s->refs == 0) {
+            d->scenes.erase(s->hash);
+            d->resident_bytes -= std::min(d->resident_bytes, s->info.device_bytes);
+            delete s;
+        }
+    });
+}
+
+int mrg_scene_info_get(mrg_scene* s, mrg_scene_info* out) {
+    return guarded([&] {
+        if (!s || !out) throw std::invalid_argument("null argument");
+        std::lock_guard<std::mutex> lock(s->device->mutex);
+        *out                 = s->info;
+        out->reference_count = uint64_t(s->refs);
+    });
+}
+
+static void run_query(mrg_scene* s, const void* d_rays, uint64_t n, float tmin, float tmax, void* d_hits, float* d_bary,
+                      uint8_t* d_occ, bool any, mrg_trace_stats* stats, float* kernel_ms) {
+    if (!(tmin >= 0) || !std::isfinite(tmin) || std::isnan(tmax) || tmax < tmin)
+        throw std::invalid_argument("Ray interval must satisfy 0 <= minimum <= maximum");
+    if (n >= (uint64_t(1) << 32) - 64) throw std::invalid_argument("too many rays for one call");
+    mrg_device* d = s->device;
+    std::lock_guard<std::mutex> lock(d->mutex);  // the query buffers and spill area are shared
+    check_hip(hipSetDevice(d->ordinal), "hipSetDevice");
+    DevBuf st(16);
+    check_hip(hipMemsetAsync(st.ptr, 0, 16, d->stream), "hipMemsetAsync");
+    check_hip(hipMemsetAsync(d->query_work.ptr, 0, 64, d->stream), "hipMemsetAsync");
+    hipEvent_t e0 = d->query_events[0], e1 = d->query_events[1];
+    check_hip(hipEventRecord(e0, d->stream), "hipEventRecord");
+    check_hip(launch_query(s->gpu, static_cast<const float*>(d_rays), uint32_t(n), tmin, tmax, static_cast<mrg_hit_raw*>(d_hits), d_bary,
+                           d_occ, d->query_work.as<uint32_t>(), d->pool, st.as<uint64_t>(), any, stats != nullptr, d->shape, d->stream),
+              "k_query");
+    check_hip(hipEventRecord(e1, d->stream), "hipEventRecord");
+    uint64_t h[2] = {0, 0};
+    if (stats) check_hip(hipMemcpyAsync(h, st.ptr, 16, hipMemcpyDeviceToHost, d->stream), "read stats");
+    check_hip(hipStreamSynchronize(d->stream), "hipStreamSynchronize");
+    if (kernel_ms) check_hip(hipEventElapsedTime(kernel_ms, e0, e1), "hipEventElapsedTime");
+    if (stats) {
+        stats->bounds_tests   = h[0];
+        stats->triangle_tests = h[1];
+    }
+}
+
+int mrg_intersect(mrg_scene* s, const float* rays, uint64_t n, float tmin, float tmax, mrg_hit* hits, float* bary, mrg_trace_stats* stats) {
+    return guarded([&] {
+        if (!s || (n && (!rays || !hits))) throw std::invalid_argument("null argument");
+        if (n == 0) return;
+        check_hip(hipSetDevice(s->device->ordinal), "hipSetDevice");
+        DevBuf dr(n * 24), dh(n * 8), db(bary ? n * 12 : 0);
+        check_hip(hipMemcpy(dr.ptr, rays, n * 24, hipMemcpyHostToDevice), "hipMemcpy");
+        run_query(s, dr.ptr, n, tmin, tmax, dh.ptr, bary ? db.as<float>() : nullptr, nullptr, false, stats, nullptr);
+        check_hip(hipMemcpy(hits, dh.ptr, n * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+        if (bary) check_hip(hipMemcpy(bary, db.ptr, n * 12, hipMemcpyDeviceToHost), "hipMemcpy");
+    });
+}
+
+int mrg_occluded(mrg_scene* s, const float* rays, uint64_t n, float tmin, float tmax, uint8_t* occ, mrg_trace_stats* stats) {
+    return guarded([&] {
+        if (!s || (n && (!rays || !occ))) throw std::invalid_argument("null argument");
+        if (n == 0) return;
+        check_hip(hipSetDevice(s->device->ordinal), "hipSetDevice");
+        DevBuf dr(n * 24), dout(n);
+        check_hip(hipMemcpy(dr.ptr, rays, n * 24, hipMemcpyHostToDevice), "hipMemcpy");
+        run_query(s, dr.ptr, n, tmin, tmax, nullptr, nullptr, dout.as<uint8_t>(), true, stats, nullptr);
+        check_hip(hipMemcpy(occ, dout.ptr, n, hipMemcpyDeviceToHost), "hipMemcpy");
+    });
+}
+
+int mrg_intersect_device(mrg_scene* s, const void* rays, uint64_t n, float tmin, float tmax, void* hits, mrg_trace_stats* stats,
+                         float* kernel_ms) {
+    return guarded([&] {
+        if (!s || (n && (!rays || !hits))) throw std::invalid_argument("null argument");
+        if (n == 0) return;
+        run_query(s, rays, n, tmin, tmax, hits, nullptr, nullptr, false, stats, kernel_ms);
+    });
+}
+
+int mrg_job_submit(mrg_device* d, mrg_scene* s, const mrg_job_desc* desc, mrg_job** out) {
+    return guarded([&] {
+        if (!d || !s || !desc || !out) throw std::invalid_argument("null argument");
+        if (s->device != d) throw std::invalid_argument("scene belongs to another device");
+        validate_job(*desc);
+        auto j        = std::make_unique<mrg_job>();
+        j->device     = d;
+        j->scene      = s;
+        j->desc       = *desc;
+        j->camera     = make_camera(desc->camera);
+        j->film_bytes = film_bytes_of(*desc);
+        j->submitted  = Clock::now();
+        j->refs       = 2;  // caller + scheduler queue
+        std::lock_guard<std::mutex> lock(d->mutex);
+        if (j->film_bytes + d->pool_bytes + s->info.device_bytes > budget(d))
+            throw Error(MRG_NOT_ADMITTED, "job exceeds the device HBM budget");
+        s->refs++;
+        j->id = d->next_job_id++;
+        d->jobs.push_back(j.get());
+        *out = j.release();
+        d->cv.notify_all();
+    });
+}
+
+int mrg_job_pause(mrg_job* j, int* accepted) {
+    return guarded([&] {
+        if (!j) throw std::invalid_argument("job is null");
+        std::lock_guard<std::mutex> lock(j->device->mutex);
+        const bool ok = j->state == MRG_STATE_RUNNING;
+        if (ok) {
+            j->pause_requested = true;
+            j->state           = j->in_flight ? MRG_STATE_PAUSING : MRG_STATE_PAUSED;
+            j->device->cv.notify_all();
+        }
+        if (accepted) *accepted = ok;
+    });
+}
+
+int mrg_job_resume(mrg_job* j, int* accepted) {
+    return guarded([&] {
+        if (!j) throw std::invalid_argument("job is null");
+        std::lock_guard<std::mutex> lock(j->device->mutex);
+        const bool ok = j->state == MRG_STATE_PAUSED || j->state == MRG_STATE_PAUSING;
+        if (ok) {
+            j->pause_requested = false;
+            j->state           = MRG_STATE_RUNNING;
+            j->device->cv.notify_all();
+        }
+        if (accepted) *accepted = ok;
+    });
+}
+
+int mrg_job_cancel(mrg_job* j) {
+    return guarded([&] {
+        if (!j) throw std::invalid_argument("job is null");
+        std::lock_guard<std::mutex> lock(j->device->mutex);
+        if (terminal(j->state)) return;
+        j->cancel_requested = true;
+        j->pause_requested  = false;
+        j->state            = MRG_STATE_CANCELLING;
+        j->device->cv.notify_all();
+    });
+}
+
+int mrg_job_wait(mrg_job* j, int timeout_ms) {
+    int rc = MRG_OK;
+    const int g = guarded([&] {
+        if (!j) throw std::invalid_argument("job is null");
+        std::unique_lock<std::mutex> lock(j->device->mutex);
+        auto done = [&] { return terminal(j->state); };
+        if (timeout_ms < 0) {
+            j->device->cv.wait(lock, done);
+        } else if (!j->device->cv.wait_for(lock, std::chrono::milliseconds(timeout_ms), done)) {
+            rc = MRG_TIMEOUT;
+        }
+    });
+    return g != MRG_OK ? g : rc;
+}
+
+int mrg_job_progress(mrg_job* j, mrg_progress* p) {
+    return guarded([&] {
+        if (!j || !p) throw std::invalid_argument("null argument");
+        std::lock_guard<std::mutex> lock(j->device->mutex);
+        std::memset(p, 0, sizeof *p);
+        p->state            = j->state;
+        p->status           = j->status;
+        p->job_id           = j->id;
+        p->recorded_samples = j->recorded;
+        p->total_samples    = uint64_t(j->desc.width) * j->desc.height * j->desc.samples_per_pixel;
+        p->completed_pixels = j->completed_pixels;
+        const auto end      = j->finished ? j->finished_at : Clock::now();
+        p->elapsed_seconds  = std::chrono::duration<double>(end - j->submitted).count();
+        p->queued_seconds   = j->admitted_at.time_since_epoch().count() ? std::chrono::duration<double>(j->admitted_at - j->submitted).count() : 0;
+        p->first_pass_seconds = j->first_pass_done ? std::chrono::duration<double>(j->first_pass_at - j->submitted).count() : 0;
+        p->completion_seconds = j->finished ? std::chrono::duration<double>(j->finished_at - j->submitted).count() : 0;
+        p->paths                 = j->paths;
+        p->extension_rays        = j->ext_rays;
+        p->shadow_rays           = j->shadow_rays;
+        p->scattering_events     = j->scatter;
+        p->roulette_terminations = j->rr_term;
+        std::snprintf(p->error, sizeof p->error, "%s", j->error.c_str());
+    });
+}
+
+static void read_film_raw(mrg_job* j, std::vector<double>& sums, std::vector<uint32_t>& counts) {
+    const uint64_t px = uint64_t(j->desc.width) * j->desc.height;
+    sums.assign(px * 3, 0.0);
+    counts.assign(px, 0);
+    std::lock_guard<std::mutex> fl(j->film_mutex);
+    if (!j->film_sums.ptr) return;  // not admitted yet
+    check_hip(hipSetDevice(j->device->ordinal), "hipSetDevice");
+    check_hip(hipMemcpyAsync(sums.data(), j->film_sums.ptr, px * 24, hipMemcpyDeviceToHost, j->device->copy_stream), "read film");
+    check_hip(hipMemcpyAsync(counts.data(), j->film_counts.ptr, px * 4, hipMemcpyDeviceToHost, j->device->copy_stream), "read film");
+    check_hip(hipStreamSynchronize(j->device->copy_stream), "hipStreamSynchronize");
+}
+
+int mrg_job_read_film(mrg_job* j, double* sums, uint32_t* counts) {
+    return guarded([&] {
+        if (!j) throw std::invalid_argument("job is null");
+        std::vector<double>   s;
+        std::vector<uint32_t> c;
+        read_film_raw(j, s, c);
+        if (sums) std::memcpy(sums, s.data(), s.size() * 8);
+        if (counts) std::memcpy(counts, c.data(), c.size() * 4);
+    });
+}
+
+int mrg_job_read_image(mrg_job* j, float* rgb) {
+    return guarded([&] {
+        if (!j || !rgb) throw std::invalid_argument("null argument");
+        std::vector<double>   s;
+        std::vector<uint32_t> c;
+        read_film_raw(j, s, c);
+        for (size_t p = 0; p < c.size(); ++p)
+            for (int k = 0; k < 3; ++k) rgb[p * 3 + k] = c[p] ? float(s[p * 3 + k] / double(c[p])) : 0.0f;
+    });
+}
+
+int mrg_job_preview(mrg_job* j, uint32_t max_dim, float* rgb, uint32_t* width, uint32_t* height, uint64_t* recorded) {
+    return guarded([&] {
+        if (!j || !width || !height) throw std::invalid_argument("null argument");
+        const uint32_t W = j->desc.width, H = j->desc.height;
+        const double scale = max_dim ? std::max(1.0, double(std::max(W, H)) / double(max_dim)) : 1.0;
+        const uint32_t w = std::max<uint32_t>(1, uint32_t(double(W) / scale));
+        const uint32_t h = std::max<uint32_t>(1, uint32_t(double(H) / scale));
+        *width  = w;
+        *height = h;
+        if (!rgb) return;
+        uint64_t rec;
+        {
+            std::lock_guard<std::mutex> lock(j->device->mutex);
+            rec = j->recorded;
+        }
+        std::vector<double>   s;
+        std::vector<uint32_t> c;
+        read_film_raw(j, s, c);
+        for (uint32_t y = 0; y < h; ++y) {
+            const uint32_t sy = std::min<uint32_t>(H - 1, uint32_t((uint64_t(y) * H + H / 2) / h));
+            for (uint32_t x = 0; x < w; ++x) {
+                const uint32_t sx = std::min<uint32_t>(W - 1, uint32_t((uint64_t(x) * W + W / 2) / w));
+                const uint64_t p  = uint64_t(sy) * W + sx;
+                for (int k = 0; k < 3; ++k) rgb[(uint64_t(y) * w + x) * 3 + k] = c[p] ? float(s[p * 3 + k] / double(c[p])) : 0.0f;
+            }
+        }
+        if (recorded) *recorded = rec;
+    });
+}
+
+int mrg_job_release(mrg_job* j) {
+    return guarded([&] {
+        if (!j) return;
+        mrg_device* d = j->device;
+        std::lock_guard<std::mutex> lock(d->mutex);
+        if (!terminal(j->state)) {
+            j->cancel_requested = true;
+            j->state            = MRG_STATE_CANCELLING;
+            d->cv.notify_all();
+        }
+        release_job_locked(j);
+    });
+}
+
+int mrg_render(mrg_device* d, mrg_scene* s, const mrg_job_desc* desc, float* rgb, mrg_progress* progress) {
+    mrg_job* j = nullptr;
+    int rc = mrg_job_submit(d, s, desc, &j);
+    if (rc) return rc;
+    rc = mrg_job_wait(j, -1);
+    mrg_progress p{};
+    if (!rc) rc = mrg_job_progress(j, &p);
+    if (!rc && p.state != MRG_STATE_COMPLETED) {
+        g_error = p.error[0] ? p.error : "render did not complete";
+        rc      = p.status ? p.status : MRG_DEVICE_ERROR;
+    }
+    if (!rc && rgb) rc = mrg_job_read_image(j, rgb);
+    if (progress) *progress = p;
+    const std::string keep = g_error;
+    mrg_job_release(j);
+    g_error = keep;
+    return rc;
+}
+
+int mrg_bvh_build_host(const float* positions, uint64_t n, uint64_t* out) {
+    return guarded([&] {
+        if (!out || (n && !positions)) throw std::invalid_argument("null argument");
+        const BvhBuildResult b = build_bvh(positions, n, 0);
+        // structural check: every primitive exactly once, leaves <= kMaxLeaf, child boxes inside parents
+        std::vector<uint32_t> seen(n, 0);
+        uint64_t max_leaf = 0, leaf_prims = 0, bad = 0;
+        for (const GpuNode& g : b.nodes) {
+            const uint32_t refs[2] = {g.ref0, g.ref1};
+            for (uint32_t r : refs) {
+                if (r == kEmptyRef) continue;
+                if (is_leaf(r)) {
+                    const uint32_t f = leaf_first(r), c = leaf_count(r);
+                    max_leaf = std::max<uint64_t>(max_leaf, c);
+                    for (uint32_t k = 0; k < c; ++k) {
+                        if (f + k >= n) { ++bad; continue; }
+                        seen[b.order[f + k]]++;
+                        ++leaf_prims;
+                    }
+                } else if (r >= b.nodes.size()) {
+                    ++bad;
+                }
+            }
+        }
+        for (uint64_t i = 0; i < n; ++i) bad += seen[i] != 1;
+        out[0] = b.nodes.size();
+        out[1] = b.leaves;
+        out[2] = b.max_depth;
+        out[3] = max_leaf;
+        out[4] = leaf_prims;
+        out[5] = bad;
+    });
+}
+
+int mrg_device_kernel_times(mrg_device* d, mrg_kernel_times* out, int reset) {
+    return guarded([&] {
+        if (!d || !out) throw std::invalid_argument("null argument");
+        std::lock_guard<std::mutex> lock(d->mutex);
+        *out = d->times;
+        if (reset) d->times = mrg_kernel_times{};
+    });
+}
+
+}  // extern "C"

@@ -97,7 +97,7 @@ struct Builder {
             cb.grow(prims[order[i]].c);
         }
         const uint32_t count = end - begin;
-        if (count <= 4 && depth > 0) return make_leaf(bounds, begin, end, depth);
+        if (count == 1 || (count <= 4 && depth > 0)) return make_leaf(bounds, begin, end, depth);
         const bool must_balance = depth + ceil_log2((count + 3) / 4) >= kMaxDepth;
         int        best_axis = -1;
         int        best_split = 0;

@@ -1,0 +1,82 @@
+// Kernel-facing declarations shared by mrg_kernels.hip and the host library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mrg_layout.h"
+
+namespace mrg {
+
+struct mrg_hit_raw {
+    float   t;
+    int32_t primitive;
+};
+
+// Path-state flags kept in prev.w: bit 0 = previous event was discrete (delta); bits 16.. = job slot.
+constexpr uint32_t kPrevDelta = 1u;
+constexpr uint32_t kMaxBounces = 130;  // depth <= 128 (IntegratorSettings::validate) plus one
+
+// Counter words (uint32): extension-queue counts per bounce, shadow counts per bounce, and the
+// persistent-kernel work counters.
+constexpr uint32_t kCtrExt        = 0;
+constexpr uint32_t kCtrShadow     = kCtrExt + kMaxBounces + 1;
+constexpr uint32_t kCtrWorkExt    = kCtrShadow + kMaxBounces + 1;
+constexpr uint32_t kCtrWorkShadow = kCtrWorkExt + kMaxBounces + 1;
+constexpr uint32_t kCounterWords  = kCtrWorkShadow + kMaxBounces + 1;
+
+// Batch header words (device memory, written by the host before each batch) so a captured graph
+// needs no per-batch kernel arguments.
+constexpr uint32_t kHdrChunks = 0;
+constexpr uint32_t kHdrSlots  = 1;
+constexpr uint32_t kHdrPixels = 2;
+constexpr uint32_t kHdrWords  = 4;
+
+// Batch statistics (uint64).
+constexpr uint32_t kStatShadowRays = 0;
+constexpr uint32_t kStatScatter    = 1;
+constexpr uint32_t kStatRoulette   = 2;
+constexpr uint32_t kStatWords      = 8;
+
+// Path pool: structure of arrays over `capacity` path slots.
+struct Pool {
+    float4*   ray_o;     // origin, tmin
+    float4*   ray_d;     // direction, tmax
+    float4*   hit;       // t, barycentrics
+    uint32_t* hit_slot;  // leaf-order slot or 0xffffffff
+    float4*   beta;      // throughput, previous pdf
+    float4*   rad;       // radiance, eta scale
+    float4*   prev;      // previous reference point, flags
+    uint4*    rng;       // PCG state, increment
+    uint32_t* q_ext0;
+    uint32_t* q_ext1;
+    uint32_t* q_sh;
+    float4*   sh_o;      // shadow origin, limit
+    float4*   sh_d;
+    float4*   sh_c;      // contribution
+    uint32_t* counters;
+    uint64_t* stats;
+    const uint32_t* header;  // kHdr* words of the current batch
+    uint32_t* spill_ref;
+    float*    spill_t;
+    uint32_t  capacity;
+    uint32_t  spill_lanes;
+};
+
+struct LaunchShape {
+    uint32_t compute_units;
+    uint32_t traversal_blocks;  // persistent grid of the traversal kernels (256-thread blocks)
+    uint32_t shade_blocks;
+};
+
+hipError_t launch_query(const GpuScene& sc, const float* rays, uint32_t n, float tmin, float tmax, mrg_hit_raw* hits, float* bary,
+                        uint8_t* occluded, uint32_t* work, const Pool& P, uint64_t* stats, bool any, bool with_stats,
+                        const LaunchShape& shape, hipStream_t stream);
+hipError_t launch_reset(const Pool& P, hipStream_t stream);
+hipError_t launch_generate(const Pool& P, const GpuJob* jobs, const GpuChunk* chunks, const LaunchShape& shape, hipStream_t stream);
+hipError_t launch_extend(const Pool& P, const GpuJob* jobs, uint32_t iter, const LaunchShape& shape, hipStream_t stream);
+hipError_t launch_shade(const Pool& P, const GpuJob* jobs, uint32_t iter, const LaunchShape& shape, hipStream_t stream);
+hipError_t launch_shadow(const Pool& P, const GpuJob* jobs, uint32_t iter, const LaunchShape& shape, hipStream_t stream);
+hipError_t launch_accumulate(const Pool& P, const GpuJob* jobs, const GpuChunk* chunks, const LaunchShape& shape, hipStream_t stream);
+
+}  // namespace mrg

@@ -57,6 +57,22 @@ __device__ __forceinline__ StackView make_stack(uint32_t* lds_ref, float* lds_t,
     return s;
 }
 
+// Reported distance of the winning hit: the ray/plane solve of that one triangle in FP64
+// (t = n.(v0 - o) / n.d with n = (v1 - v0) x (v2 - v0)), rounded to float. The hit decision stays
+// with the FP32 watertight test; this only removes the FP32 cancellation error of t for hits close
+// to the ray origin. Falls back to the FP32 t for a ray parallel to the plane.
+__device__ __forceinline__ float refine_distance(const GpuScene& sc, uint32_t slot, const RayPre& r, float t32) {
+    const float4* tp = reinterpret_cast<const float4*>(sc.tris + slot);
+    const float4  a = tp[0], b = tp[1], c = tp[2];
+    const double  e1x = double(b.x) - a.x, e1y = double(b.y) - a.y, e1z = double(b.z) - a.z;
+    const double  e2x = double(c.x) - a.x, e2y = double(c.y) - a.y, e2z = double(c.z) - a.z;
+    const double  nx = e1y * e2z - e2y * e1z, ny = e1z * e2x - e2z * e1x, nz = e1x * e2y - e2x * e1y;
+    const double  den = (nx * r.d.x + ny * r.d.y) + nz * r.d.z;
+    if (den == 0.0) return t32;
+    const double num = (nx * (double(a.x) - r.o.x) + ny * (double(a.y) - r.o.y)) + nz * (double(a.z) - r.o.z);
+    return static_cast<float>(num / den);
+}
+
 // ---------------------------------------------------------------------------------------------
 // Intersection microbenchmark / SceneBvh::intersect and ::occluded over a packed ray array.
 template <bool ANY, bool STATS>
@@ -82,7 +98,7 @@ __global__ void __launch_bounds__(256) k_query(GpuScene sc, const float* __restr
                 occluded[i] = res.slot != 0xffffffffu ? 1 : 0;
             } else {
                 mrg_hit_raw h;
-                h.t         = res.slot != 0xffffffffu ? res.t : __builtin_inff();
+                h.t         = res.slot != 0xffffffffu ? refine_distance(sc, res.slot, rp, res.t) : __builtin_inff();
                 h.primitive = res.slot != 0xffffffffu ? static_cast<int32_t>(res.prim) : -1;
                 hits[i]     = h;
                 if (bary) {
@@ -101,10 +117,10 @@ __global__ void __launch_bounds__(256) k_query(GpuScene sc, const float* __restr
 
 // ---------------------------------------------------------------------------------------------
 // Batch start: clear counters (a fill kernel, so a captured graph holds no memset node).
-__global__ void k_reset(uint32_t* counters, uint32_t words, uint64_t* stats, uint32_t stat_words, uint32_t total_paths) {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < words; i += gridDim.x * blockDim.x) counters[i] = 0;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < stat_words; i += gridDim.x * blockDim.x) stats[i] = 0;
-    if (blockIdx.x == 0 && threadIdx.x == 0) counters[kCtrExt] = total_paths;
+__global__ void k_reset(Pool P) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < kCounterWords; i += gridDim.x * blockDim.x)
+        P.counters[i] = i == kCtrExt ? P.header[kHdrSlots] : 0u;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < kStatWords; i += gridDim.x * blockDim.x) P.stats[i] = 0;
 }
 
 __device__ __forceinline__ uint32_t find_chunk_by_slot(const GpuChunk* chunks, uint32_t count, uint32_t slot) {
@@ -126,8 +142,9 @@ __device__ __forceinline__ uint32_t find_chunk_by_pixel(const GpuChunk* chunks, 
 }
 
 // Camera rays (PathTracer::sample_pixel, path_tracer.cpp:368-386; Camera::generate_lens_ray).
-__global__ void __launch_bounds__(256) k_generate(Pool P, const GpuJob* __restrict__ jobs, const GpuChunk* __restrict__ chunks,
-                                                  uint32_t chunk_count, uint32_t total) {
+__global__ void __launch_bounds__(256) k_generate(Pool P, const GpuJob* __restrict__ jobs, const GpuChunk* __restrict__ chunks) {
+    const uint32_t chunk_count = P.header[kHdrChunks];
+    const uint32_t total       = P.header[kHdrSlots];
     for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < total; slot += gridDim.x * blockDim.x) {
         const uint32_t  ci = find_chunk_by_slot(chunks, chunk_count, slot);
         const GpuChunk  ch = chunks[ci];
@@ -431,8 +448,9 @@ __global__ void __launch_bounds__(256) k_shadow(Pool P, const GpuJob* __restrict
 }
 
 // Film commit: per pixel, samples in index order, FP64 sums (render_session.cpp:39-74).
-__global__ void __launch_bounds__(256) k_accumulate(Pool P, const GpuJob* __restrict__ jobs, const GpuChunk* __restrict__ chunks,
-                                                    uint32_t chunk_count, uint32_t total_pixels) {
+__global__ void __launch_bounds__(256) k_accumulate(Pool P, const GpuJob* __restrict__ jobs, const GpuChunk* __restrict__ chunks) {
+    const uint32_t chunk_count  = P.header[kHdrChunks];
+    const uint32_t total_pixels = P.header[kHdrPixels];
     for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total_pixels; g += gridDim.x * blockDim.x) {
         const uint32_t ci = find_chunk_by_pixel(chunks, chunk_count, g);
         const GpuChunk ch = chunks[ci];
@@ -481,14 +499,13 @@ hipError_t launch_query(const GpuScene& sc, const float* rays, uint32_t n, float
     return hipGetLastError();
 }
 
-hipError_t launch_reset(const Pool& P, uint32_t total_paths, hipStream_t stream) {
-    hipLaunchKernelGGL(k_reset, dim3(4), dim3(256), 0, stream, P.counters, kCounterWords, P.stats, kStatWords, total_paths);
+hipError_t launch_reset(const Pool& P, hipStream_t stream) {
+    hipLaunchKernelGGL(k_reset, dim3(4), dim3(256), 0, stream, P);
     return hipGetLastError();
 }
 
-hipError_t launch_generate(const Pool& P, const GpuJob* jobs, const GpuChunk* chunks, uint32_t chunk_count, uint32_t total,
-                           const LaunchShape& shape, hipStream_t stream) {
-    hipLaunchKernelGGL(k_generate, dim3(grid_for(shape, 8)), dim3(256), 0, stream, P, jobs, chunks, chunk_count, total);
+hipError_t launch_generate(const Pool& P, const GpuJob* jobs, const GpuChunk* chunks, const LaunchShape& shape, hipStream_t stream) {
+    hipLaunchKernelGGL(k_generate, dim3(grid_for(shape, 8)), dim3(256), 0, stream, P, jobs, chunks);
     return hipGetLastError();
 }
 
@@ -507,9 +524,8 @@ hipError_t launch_shadow(const Pool& P, const GpuJob* jobs, uint32_t iter, const
     return hipGetLastError();
 }
 
-hipError_t launch_accumulate(const Pool& P, const GpuJob* jobs, const GpuChunk* chunks, uint32_t chunk_count, uint32_t total_pixels,
-                             const LaunchShape& shape, hipStream_t stream) {
-    hipLaunchKernelGGL(k_accumulate, dim3(grid_for(shape, 4)), dim3(256), 0, stream, P, jobs, chunks, chunk_count, total_pixels);
+hipError_t launch_accumulate(const Pool& P, const GpuJob* jobs, const GpuChunk* chunks, const LaunchShape& shape, hipStream_t stream) {
+    hipLaunchKernelGGL(k_accumulate, dim3(grid_for(shape, 4)), dim3(256), 0, stream, P, jobs, chunks);
     return hipGetLastError();
 }
 

@@ -138,8 +138,9 @@ struct GpuChunk {
     uint32_t pixel_count;
     uint32_t sample_begin;
     uint32_t sample_count;
-    uint32_t slot_begin;  // first path slot of this chunk in the pool
-    uint32_t pad0, pad1;
+    uint32_t slot_begin;    // first path slot of this chunk in the pool
+    uint32_t pixel_prefix;  // pixels of all earlier chunks of the batch
+    uint32_t pad;
 };
 
 }  // namespace mrg

@@ -225,7 +225,7 @@ __device__ __forceinline__ uint32_t select_light(const GpuScene& sc, float u, fl
         const uint32_t mid = (lo + hi) >> 1;
         if (u < sc.cdf[mid]) hi = mid; else lo = mid + 1;
     }
-    const uint32_t index = lo - 1;
+    const uint32_t index = lo == 0 ? 0u : (lo > sc.choice_count ? sc.choice_count - 1u : lo - 1u);
     mass                 = sc.cdf[index + 1] - sc.cdf[index];
     return index;
 }

@@ -131,6 +131,7 @@ struct StackView {
 };
 
 __device__ __forceinline__ void stack_push(const StackView& s, uint32_t& sp, uint32_t ref, float t) {
+    if (sp >= kStackTotal) return;  // unreachable: the builder bounds the depth by kMaxDepth < kStackTotal
     if (sp < kStackLds) {
         s.lds_ref[sp * s.block + s.lane] = ref;
         s.lds_t[sp * s.block + s.lane]   = t;

@@ -80,8 +80,10 @@ def cornell_box(lamp_radiance=(17.0, 12.0, 4.0)) -> Scene:
 
 
 def _hash3(ix, iy, iz, seed):
-    h = (ix.astype(np.uint64) * np.uint64(0x8DA6B343) ^ iy.astype(np.uint64) * np.uint64(0xD8163841)
-         ^ iz.astype(np.uint64) * np.uint64(0xCB1AB31F) ^ np.uint64(seed) * np.uint64(0x9E3779B97F4A7C15))
+    salt = np.uint64((int(seed) * 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF)
+    with np.errstate(over="ignore"):
+        h = (ix.astype(np.uint64) * np.uint64(0x8DA6B343) ^ iy.astype(np.uint64) * np.uint64(0xD8163841)
+             ^ iz.astype(np.uint64) * np.uint64(0xCB1AB31F) ^ salt)
     return (rng.mix64(h) >> np.uint64(11)).astype(np.float64) / float(1 << 53)
 
 

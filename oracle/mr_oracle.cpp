@@ -1188,6 +1188,20 @@ template <class F> static void parallel_for(uint64_t n, int threads, F&& body) {
         if (!e.empty()) throw std::runtime_error(e);
 }
 
+// The FP32 twin's reported distance of a winning hit: the FP64 ray/plane solve of that triangle
+// (n = (v1 - v0) x (v2 - v0), t = n.(v0 - o) / n.d), rounded to float; the FP32 t if n.d == 0.
+// The hit decision itself stays with the FP32 watertight test.
+static double refine_distance(const std::array<V3<float>, 3>& v, const Ray<float>& ray, float t32) {
+    const double e1x = double(v[1].x) - v[0].x, e1y = double(v[1].y) - v[0].y, e1z = double(v[1].z) - v[0].z;
+    const double e2x = double(v[2].x) - v[0].x, e2y = double(v[2].y) - v[0].y, e2z = double(v[2].z) - v[0].z;
+    const double nx = e1y * e2z - e2y * e1z, ny = e1z * e2x - e2z * e1x, nz = e1x * e2y - e2x * e1y;
+    const double den = (nx * ray.d.x + ny * ray.d.y) + nz * ray.d.z;
+    if (den == 0.0) return t32;
+    const double num = (nx * (double(v[0].x) - ray.o.x) + ny * (double(v[0].y) - ray.o.y)) + nz * (double(v[0].z) - ray.o.z);
+    return double(static_cast<float>(num / den));
+}
+static double refine_distance(const std::array<V3<float>, 3>&, const Ray<double>&, double t) { return t; }
+
 template <class R>
 static void intersect_impl(const Scene& s, const double* rays, uint64_t n, double tmin, double tmax, int threads,
                            int32_t* prim, double* t, double* t2, int32_t* p2, double* bary, OrcStats* stats) {
@@ -1199,7 +1213,11 @@ static void intersect_impl(const Scene& s, const double* rays, uint64_t n, doubl
             std::optional<PrimHit<R>> second;
             const auto hit = s.trace<R>(ray, R(tmin), R(tmax), false, &local[w], (t2 || p2) ? &second : nullptr);
             prim[i] = hit ? static_cast<int32_t>(hit->prim) : -1;
-            if (t) t[i] = hit ? double(hit->hit.t) : std::numeric_limits<double>::infinity();
+            if (t) {
+                double reported = hit ? double(hit->hit.t) : std::numeric_limits<double>::infinity();
+                if (hit && std::is_same_v<R, float>) reported = refine_distance(s.pos[hit->prim], ray, hit->hit.t);
+                t[i] = reported;
+            }
             if (t2) t2[i] = second ? double(second->hit.t) : std::numeric_limits<double>::infinity();
             if (p2) p2[i] = second ? static_cast<int32_t>(second->prim) : -1;
             if (bary) {

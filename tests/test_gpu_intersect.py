@@ -1,0 +1,80 @@
+"""GPU closest-hit / any-hit parity against the oracle (FP32 twin: bit-exact; FP64 reference:
+hit ids equal outside near-ties, t within 1e-5 relative)."""
+import numpy as np
+import pytest
+
+from minirender_amd import scenegen
+
+pytestmark = pytest.mark.gpu
+
+NEAR_TIE = 1e-6   # |t2 - t1| <= 1e-6 * t1 counts as a near-tie
+T_REL = 1e-5
+
+
+def _compare(device, oracle, scene, rays, tmin=0.0, tmax=np.inf):
+    print(f"[intersect parity] scene={scene.name} tris={scene.triangle_count} rays={len(rays)}", flush=True)
+    bvh = device.upload(scene)
+    gpu = bvh.intersect(rays, tmin, tmax, barycentric=True)
+    o = oracle.OracleScene(scene)
+    r64 = rays.astype(np.float64)
+    twin = o.intersect(r64, tmin, tmax, precision=32)
+    ref = o.intersect(r64, tmin, tmax, precision=64, second=True)
+    # FP32 twin: identical ids, t and barycentrics
+    np.testing.assert_array_equal(gpu["primitive"], twin["prim"])
+    hit = twin["prim"] >= 0
+    np.testing.assert_array_equal(gpu["t"][hit], twin["t"][hit].astype(np.float32))
+    np.testing.assert_array_equal(gpu["barycentric"][hit], twin["bary"][hit].astype(np.float32))
+    # FP64 reference intersector
+    t1, t2 = ref["t"], ref["t2"]
+    near_tie = np.isfinite(t2) & (np.abs(t2 - t1) <= NEAR_TIE * np.abs(t1))
+    mismatch = gpu["primitive"] != ref["prim"]
+    bad = mismatch & ~near_tie
+    print(f"  hits={hit.sum()} near_ties={near_tie.sum()} id_mismatch={mismatch.sum()} outside_near_ties={bad.sum()}")
+    assert bad.sum() == 0
+    both = (gpu["primitive"] == ref["prim"]) & (ref["prim"] >= 0)
+    rel = np.abs(gpu["t"][both] - t1[both]) / np.abs(t1[both])
+    assert rel.max(initial=0) <= T_REL
+    # any-hit agrees with the twin
+    occ = bvh.occluded(rays, tmin, tmax)
+    occ_twin, _ = o.occluded(r64, tmin, tmax, precision=32)
+    np.testing.assert_array_equal(occ, occ_twin)
+    return gpu
+
+
+def test_icosphere_random_rays(device, oracle):
+    scene = scenegen.icosphere(level=5)
+    rays = scenegen.intersection_rays(scene, 1 << 17)
+    gpu = _compare(device, oracle, scene, rays)
+    assert (gpu["primitive"] >= 0).mean() > 0.3
+
+
+def test_cornell_single_sided_rays(device, oracle):
+    scene = scenegen.cornell_box()
+    rays = scenegen.intersection_rays(scene, 1 << 15, seed=7, scale=0.9)
+    _compare(device, oracle, scene, rays)
+    _compare(device, oracle, scene, rays, tmin=1e-3, tmax=1.5)
+
+
+def test_empty_and_single_triangle(device, oracle):
+    one = scenegen.floor_scene()
+    from minirender_amd.scene import concat, material
+    tri = concat([(one.positions[:1], one.normals[:1])], [material()])
+    rays = np.array([[0.5, 1, 1.5, 0, -1, 0], [3, 1, -3, 0, -1, 0], [0.5, 1, 1.5, 0, 1, 0]], np.float32)
+    gpu = device.upload(tri).intersect(rays)
+    o = oracle.OracleScene(tri).intersect(rays.astype(np.float64), precision=32)
+    np.testing.assert_array_equal(gpu["primitive"], o["prim"])
+    empty = concat([], [])
+    e = device.upload(empty).intersect(rays)
+    assert (e["primitive"] == -1).all()
+
+
+def test_coincident_triangles_tie_break(device, oracle):
+    """256 coincident triangles: the lowest scene index wins (bvh.cpp:273-276)."""
+    from minirender_amd.scene import concat, material
+    t = np.array([[[0, 0, 0], [1, 0, 0], [0, 1, 0]]], np.float32)
+    n = np.tile(np.array([0, 0, 1], np.float32), (1, 3, 1))
+    scene = concat([(t, n)] * 256, [material()] * 256)
+    rays = np.array([[0.25, 0.25, 2, 0, 0, -1]], np.float32)
+    gpu = device.upload(scene).intersect(rays)
+    assert gpu["primitive"][0] == 0
+    assert gpu["t"][0] == np.float32(2)
