@@ -1,0 +1,215 @@
+"""Benchmark: Msamples/s of the plain GPU path tracer on the mixed-material 1080p / 256-spp workload.
+
+One step = one complete render job of the workload (every pixel, every sample) through libmrg's
+plain (one job at a time) schedule. Inputs (the scene) are resident in HBM before the timed region;
+the timed region is submit -> job complete on the device.
+
+Prints one JSON line with the throughput, a roofline entry for the dominant kernel (k_extend,
+closest-hit traversal; algorithmic bytes per ray from the FP64 oracle's traversal counters on the
+reference BVH topology) and a CPU baseline (the FP64 oracle on the host cores, bounded sample).
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload mixed|cornell|intersect]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+B_BOX, B_TRI, B_RAY_IN, B_HIT_OUT, B_FILM = 24, 36, 24, 8, 12
+
+WORKLOADS = {
+    # 2M-triangle fBm terrain + 64 spheres, sun + sky, thin lens f/2.8
+    "mixed": dict(width=1920, height=1080, spp=256, max_depth=16, seed=0, cpu_rows=(516, 564)),
+    # Cornell box 512^2, 64 spp, depth 8
+    "cornell": dict(width=512, height=512, spp=64, max_depth=8, seed=0, cpu_rows=(192, 320)),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_scene(name):
+    from minirender_amd import scenegen
+    if name == "mixed":
+        return scenegen.mixed_terrain()
+    if name == "cornell":
+        return scenegen.cornell_box()
+    if name == "intersect":
+        return scenegen.icosphere(level=8)
+    raise SystemExit(f"unknown workload {name}")
+
+
+def read_traffic(kernel, workload):
+    """HBM bytes per launch measured by rocprofv3 --pmc (see profiles/), or None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch"):
+            return float(line.split("=")[1])
+    return None
+
+
+def cpu_baseline(scene, w, threads):
+    """FP64 oracle (restatement of the reference CPU path tracer) on a bounded crop at full spp."""
+    from oracle import oracle as O
+    o = O.OracleScene(scene)
+    y0, y1 = w["cpu_rows"]
+    t = time.perf_counter()
+    _, st = o.render(scene.camera, w["width"], w["height"], w["spp"], seed=w["seed"], max_depth=w["max_depth"],
+                     precision=64, threads=threads, crop=(0, y0, w["width"], y1))
+    dt = time.perf_counter() - t
+    return st, dt, (y0, y1)
+
+
+def bench_render(args):
+    from minirender_amd import api
+    w = WORKLOADS[args.workload]
+    log(f"[bench] generating workload {args.workload}")
+    scene = make_scene(args.workload)
+    dev = api.Device(0, schedule="plain", persistent_blocks_per_cu=args.blocks_per_cu)
+    info = dev.query()
+    t0 = time.perf_counter()
+    bvh = dev.upload(scene)
+    log(f"[bench] device {info['name']} CUs={info['compute_units']}; scene {scene.triangle_count} tris uploaded in "
+        f"{time.perf_counter() - t0:.2f}s; {bvh.info()['nodes']} nodes")
+    rs = api.RenderSettings(w["width"], w["height"], w["spp"], w["seed"])
+    ins = api.IntegratorSettings("mis", w["max_depth"], True, 3)
+
+    def run_job():
+        job = api.Job(bvh, scene.camera, rs, ins)
+        job.wait(-1)
+        p = job.progress()
+        job.release()
+        if p.state != "completed":
+            raise SystemExit(f"render ended {p.state}: {p.error}")
+        return p
+
+    for i in range(args.warmup):
+        t = time.perf_counter()
+        run_job()
+        log(f"[bench] warmup {i} {time.perf_counter() - t:.3f}s")
+    dev.kernel_times(reset=True)
+    times, progs = [], []
+    for i in range(args.steps):
+        t = time.perf_counter()
+        progs.append(run_job())
+        times.append(time.perf_counter() - t)
+        log(f"[bench] step {i} {times[-1]:.3f}s")
+    kt = dev.kernel_times()
+    samples = w["width"] * w["height"] * w["spp"]
+    total = sum(times)
+    value = samples * args.steps / total / 1e6
+    rays = kt["extension_rays"] + kt["shadow_rays"]
+
+    result = {
+        "metric": "Msamples/s at 1080p 256spp; p50/p99 job latency, served vs plain, 1 GPU",
+        "value": round(value, 3),
+        "unit": "Msamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(total / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload} {w['width']}x{w['height']} {w['spp']}spp depth {w['max_depth']} "
+                               f"MIS RR3 ({scene.triangle_count} triangles, plain one-job schedule)",
+                   "width": w["width"], "height": w["height"], "spp": w["spp"], "max_depth": w["max_depth"],
+                   "triangles": scene.triangle_count},
+        "mrays_per_s": round(rays / total / 1e6, 1),
+        "kernel_ms_per_step": {k: round(kt[k] / args.steps, 2) for k in
+                               ("generate_ms", "extend_ms", "shade_ms", "shadow_ms", "accumulate_ms")},
+    }
+
+    # CPU baseline + per-ray algorithmic bytes from the FP64 oracle counters
+    threads = min(16, os.cpu_count() or 1)
+    cpu = None
+    if not args.no_cpu:
+        log(f"[bench] cpu baseline: FP64 oracle, {threads} threads")
+        st, dt, rows = cpu_baseline(scene, w, threads)
+        cpu = {"value": round(st["paths"] / dt / 1e6, 4), "unit": "Msamples/s", "cores": threads, "kind": "port",
+               "sample": f"rows {rows[0]}-{rows[1] - 1} ({(rows[1] - rows[0]) * w['width']} px) at {w['spp']} spp, "
+                         f"FP64 oracle, {dt:.1f}s",
+               "gpu_over_cpu": round(value / (st["paths"] / dt / 1e6), 1)}
+        ext_bounds = (st["bounds_tests"] - st["shadow_bounds_tests"]) / st["extension_rays"]
+        ext_tris = (st["triangle_tests"] - st["shadow_triangle_tests"]) / st["extension_rays"]
+        b_ray = B_RAY_IN + B_HIT_OUT + B_BOX * ext_bounds + B_TRI * ext_tris
+        launches = max(kt["extend_launches"], 1)
+        per_launch_bytes = kt["extension_rays"] / launches * b_ray
+        avg_s = kt["extend_ms"] / launches / 1e3
+        achieved = per_launch_bytes / avg_s / 1e9 if avg_s > 0 else None
+        traffic = read_traffic("k_extend", args.workload)
+        result["roofline"] = {"kernel": "k_extend", "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                              "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                              "bytes_per_ray": round(b_ray, 1), "bounds_tests_per_ray": round(ext_bounds, 2),
+                              "triangle_tests_per_ray": round(ext_tris, 2),
+                              "avg_launch_ms": round(avg_s * 1e3, 4),
+                              "rays_per_launch": round(kt["extension_rays"] / launches, 1)}
+    result["cpu_baseline"] = cpu
+    bvh.release()
+    dev.close()
+    return result
+
+
+def bench_intersect(args):
+    """16M seeded rays against the 1.31M-triangle displaced icosphere, closest hit."""
+    import ctypes
+
+    from minirender_amd import _native as N, api, scenegen
+    scene = make_scene("intersect")
+    n = 1 << 24
+    rays = scenegen.intersection_rays(scene, n)
+    dev = api.Device(0)
+    bvh = dev.upload(scene)
+    hip = ctypes.CDLL("libamdhip64.so")
+    d_rays, d_hits = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(d_rays), ctypes.c_size_t(rays.nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(d_hits), ctypes.c_size_t(n * 8)) == 0
+    assert hip.hipMemcpy(d_rays, rays.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(rays.nbytes), 1) == 0
+    for _ in range(args.warmup):
+        bvh.intersect_device(d_rays.value, n, d_hits.value)
+    ms = [bvh.intersect_device(d_rays.value, n, d_hits.value) for _ in range(args.steps)]
+    hip.hipFree(d_rays)
+    hip.hipFree(d_hits)
+    avg = sum(ms) / len(ms)
+    return {"metric": "Mrays/s closest hit (intersection microbench)", "value": round(n / avg / 1e3, 1),
+            "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(avg, 3),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": f"icosphere level 8 ({scene.triangle_count} tris), {n} rays"}}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="mixed", choices=["mixed", "cornell", "intersect"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--blocks-per-cu", type=int, default=0)
+    ap.add_argument("--spp", type=int, default=0, help="override samples per pixel (calibration runs only)")
+    args = ap.parse_args()
+    if args.spp:
+        WORKLOADS[args.workload]["spp"] = args.spp
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build runs the path on a single MI355X (see DESIGN.md for how "
+              "it would shard); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    res = bench_intersect(args) if args.workload == "intersect" else bench_render(args)
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

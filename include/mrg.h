@@ -240,6 +240,8 @@ typedef struct mrg_kernel_times {
     uint64_t batches;
     uint64_t extension_rays;
     uint64_t shadow_rays;
+    uint64_t extend_launches;
+    uint64_t shadow_launches;
 } mrg_kernel_times;
 int mrg_device_kernel_times(mrg_device* device, mrg_kernel_times* out, int reset);
 

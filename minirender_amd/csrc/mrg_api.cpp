@@ -707,6 +707,8 @@ void run_batch(mrg_device* d, BatchPlan& b) {
     d->times.shadow_ms += acc[3];
     d->times.accumulate_ms += acc[4];
     d->times.batches++;
+    d->times.extend_launches += iters;
+    d->times.shadow_launches += iters;
     d->times.extension_rays += ext;
     d->times.shadow_rays += sh;
     const bool shared = b.jobs.size() > 1;

@@ -176,13 +176,13 @@ def _fbm_height(x, z, seed, octaves=6):
 
 
 def terrain(cells=1024, extent=40.0, height=4.0, seed=3):
-    """fBm heightfield over [-extent, extent]^2, (cells+1)^2 vertices, 2*cells^2 triangles."""
+    """fBm heightfield over [-extent, extent]^2, (cells+1)^2 vertices, 2*cells^2 triangles; per-vertex
+    normals from central differences of the height grid."""
     g = np.linspace(-extent, extent, cells + 1)
     x, z = np.meshgrid(g, g, indexing="xy")
     y = height * _fbm_height(x, z, seed)
-    eps = 2 * extent / cells
-    dx = height * (_fbm_height(x + eps, z, seed) - _fbm_height(x - eps, z, seed)) / (2 * eps)
-    dz = height * (_fbm_height(x, z + eps, seed) - _fbm_height(x, z - eps, seed)) / (2 * eps)
+    step = 2 * extent / cells
+    dz, dx = np.gradient(y, step, step)
     n = _unit(np.stack([-dx, np.ones_like(dx), -dz], -1))
     v = np.stack([x, y, z], -1)
     a, b, c, d = v[:-1, :-1], v[1:, :-1], v[1:, 1:], v[:-1, 1:]

@@ -939,6 +939,7 @@ struct Settings {
 struct PathCounters {
     uint64_t paths = 0, shadow = 0, scatter = 0, rr_term = 0, extension = 0;
     Counters geom;
+    Counters shadow_geom;
 };
 
 template <class R> void check_radiance(V3<R> v) {
@@ -958,7 +959,7 @@ V3<R> shadow_transmittance(const Scene& s, Ray<R> ray, const LightSample<R>& lig
         ray.d = diff / limit;
         limit = std::nextafter(limit, R(0));
     }
-    if (s.trace<R>(ray, R(0), limit, true, st ? &st->geom : nullptr)) return {0, 0, 0};
+    if (s.trace<R>(ray, R(0), limit, true, st ? &st->shadow_geom : nullptr)) return {0, 0, 0};
     return {1, 1, 1};
 }
 
@@ -1159,9 +1160,11 @@ static void add_stats(OrcStats* out, const PathCounters& c) {
     out->shadow_rays += c.shadow;
     out->scattering_events += c.scatter;
     out->roulette_terminations += c.rr_term;
-    out->bounds_tests += c.geom.bounds;
-    out->triangle_tests += c.geom.triangles;
+    out->bounds_tests += c.geom.bounds + c.shadow_geom.bounds;
+    out->triangle_tests += c.geom.triangles + c.shadow_geom.triangles;
     out->extension_rays += c.extension;
+    out->shadow_bounds_tests += c.shadow_geom.bounds;
+    out->shadow_triangle_tests += c.shadow_geom.triangles;
 }
 
 template <class F> static void parallel_for(uint64_t n, int threads, F&& body) {

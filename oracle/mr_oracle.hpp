@@ -89,6 +89,8 @@ struct OrcStats {
     uint64_t bounds_tests;
     uint64_t triangle_tests;
     uint64_t extension_rays;
+    uint64_t shadow_bounds_tests;    // the part of bounds_tests spent on shadow (any-hit) rays
+    uint64_t shadow_triangle_tests;  // the part of triangle_tests spent on shadow rays
 };
 
 const char* orc_last_error(void);

@@ -36,7 +36,8 @@ class _SceneDesc(C.Structure):
 
 class Stats(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("paths", "shadow_rays", "scattering_events", "roulette_terminations",
-                                          "bounds_tests", "triangle_tests", "extension_rays")]
+                                          "bounds_tests", "triangle_tests", "extension_rays",
+                                          "shadow_bounds_tests", "shadow_triangle_tests")]
 
     def as_dict(self):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}
