@@ -245,6 +245,10 @@ typedef struct mrg_kernel_times {
 } mrg_kernel_times;
 int mrg_device_kernel_times(mrg_device* device, mrg_kernel_times* out, int reset);
 
+/* HBM accounting of the admission policy: budget, bytes held by resident scenes + admitted films,
+ * and the fixed path-pool / traversal-stack bytes. */
+int mrg_device_memory(mrg_device* device, uint64_t* budget_bytes, uint64_t* resident_bytes, uint64_t* pool_bytes);
+
 /*
  * Host-only BVH build of [n][3][3] FP32 triangles (no device needed): out[0] nodes, out[1] leaves,
  * out[2] maximum depth, out[3] largest leaf, out[4] primitives referenced by leaves, out[5] structural

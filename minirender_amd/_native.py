@@ -118,6 +118,7 @@ SIGNATURES = {
     "mrg_render": (C.c_int, [P, P, C.POINTER(JobDesc), P, C.POINTER(Progress)]),
     "mrg_device_kernel_times": (C.c_int, [P, C.POINTER(KernelTimes), C.c_int]),
     "mrg_bvh_build_host": (C.c_int, [P, C.c_uint64, P]),
+    "mrg_device_memory": (C.c_int, [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
 }
 
 _lib = None

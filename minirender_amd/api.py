@@ -108,6 +108,11 @@ class Device:
         N.check(N.lib().mrg_device_query(self.handle, name, 256, C.byref(cus), C.byref(hbm), C.byref(l2)))
         return dict(name=name.value.decode(), compute_units=cus.value, hbm_bytes=hbm.value, l2_bytes=l2.value)
 
+    def memory(self):
+        b, r, p = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        N.check(N.lib().mrg_device_memory(self.handle, C.byref(b), C.byref(r), C.byref(p)))
+        return dict(budget_bytes=b.value, resident_bytes=r.value, pool_bytes=p.value)
+
     def upload(self, scene: Scene) -> "SceneBvh":
         return SceneBvh(self, scene)
 
@@ -303,10 +308,22 @@ class RenderSession:
         integrator.validate()
         if self._job is not None and self.progress().state not in N.TERMINAL:
             raise RuntimeError("Render session is still active")
+        if camera is None:
+            scene.camera.validate()
         if self._job is not None:
             self._job.release()
-        self._bvh = self.device.upload(scene)
+            self._job = None
         self._result = None
+        self._failure = None
+        self._jobs_started = getattr(self, "_jobs_started", 0) + 1
+        try:
+            # the reference builds the BVH in the controller thread, so scene-content errors surface
+            # as an asynchronous `failed` state rather than from start() (render_session.cpp:494-505)
+            self._bvh = self.device.upload(scene)
+        except N.MrgError as e:
+            self._failure = Progress("failed", 0, 0, render.width * render.height * render.samples_per_pixel, 0,
+                                     0.0, 0.0, 0.0, 0.0, {}, str(e))
+            return
         self._job = Job(self._bvh, camera or scene.camera, render, integrator, pass_samples)
 
     def pause(self) -> bool:
@@ -327,6 +344,8 @@ class RenderSession:
         return self._job.wait(milliseconds) if self._job else True
 
     def progress(self) -> Progress:
+        if getattr(self, "_failure", None) is not None:
+            return self._failure
         if self._job is None:
             return Progress("idle", 0, 0, 0, 0, 0.0, 0.0, 0.0, 0.0, {}, "")
         return self._job.progress()

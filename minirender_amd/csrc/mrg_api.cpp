@@ -175,11 +175,15 @@ struct mrg_device {
     mrg_device_options opt{};
     hipStream_t        stream = nullptr;
     hipStream_t        copy_stream = nullptr;
+    hipStream_t        query_stream = nullptr;  // ray queries: own stream and stack-spill area
+    Pool               query_pool{};
     hipDeviceProp_t    props{};
     LaunchShape        shape{};
     // path pool
     Pool               pool{};
     DevBuf             pool_mem, spill_ref, spill_t, counters, stats, header, jobs_buf, chunks_buf, query_work;
+    DevBuf             query_spill_ref, query_spill_t;
+    std::mutex         query_mutex;
     uint32_t           max_chunks = 0, max_jobs = 0;
     uint64_t           pool_bytes = 0;
     // scene cache + scheduler
@@ -245,7 +249,12 @@ void alloc_pool(mrg_device* d) {
     d->jobs_buf.alloc(sizeof(GpuJob) * d->max_jobs);
     d->chunks_buf.alloc(sizeof(GpuChunk) * d->max_chunks);
     d->query_work.alloc(64);
-    d->pool_bytes = d->pool_mem.bytes + d->spill_ref.bytes + d->spill_t.bytes;
+    d->query_spill_ref.alloc(lanes * (kStackTotal - kStackLds) * 4);
+    d->query_spill_t.alloc(lanes * (kStackTotal - kStackLds) * 4);
+    d->query_pool           = pl;
+    d->query_pool.spill_ref = d->query_spill_ref.as<uint32_t>();
+    d->query_pool.spill_t   = d->query_spill_t.as<float>();
+    d->pool_bytes = d->pool_mem.bytes + d->spill_ref.bytes + d->spill_t.bytes + d->query_spill_ref.bytes + d->query_spill_t.bytes;
 }
 
 // Camera constants: CameraRays<float> of the oracle (scene.cpp:171-199 in FP32; tan in FP64).
@@ -584,15 +593,18 @@ BatchPlan plan_batch_locked(mrg_device* d) {
     const uint32_t cap      = d->pool.capacity;
     const bool     batched  = d->opt.schedule == MRG_SCHEDULE_BATCHED;
     const uint32_t max_jobs = batched ? d->max_jobs : 1u;
-    for (mrg_job* j : d->jobs) {
-        if (b.jobs.size() >= max_jobs || b.slots >= cap) break;
-        if (!runnable(j)) continue;
+    // Candidates in arrival order, jobs still waiting for their first pass ahead of the others.
+    std::vector<mrg_job*> cand;
+    for (int pass = 0; pass < 2; ++pass)
+        for (mrg_job* j : d->jobs)
+            if (runnable(j) && (j->first_pass_done == (pass == 1)) && cand.size() < max_jobs) cand.push_back(j);
+    // Each co-scheduled job gets at most a fair share of the path pool in this batch.
+    const uint32_t share = cand.empty() ? cap : std::max<uint32_t>(cap / uint32_t(cand.size()), 1u);
+    for (mrg_job* j : cand) {
+        if (b.slots >= cap) break;
         const uint32_t slot = uint32_t(b.jobs.size());
         // one chunk per job per batch: a pixel never appears twice in one accumulate launch
-        if (!plan_chunk(j, slot, cap - b.slots, !batched, b)) {
-            if (b.jobs.empty()) continue;
-            break;
-        }
+        if (!plan_chunk(j, slot, std::min(cap - b.slots, share), !batched, b)) continue;
         b.jobs.push_back(j);
         b.gjobs.push_back(gpu_job(j));
         b.max_depth = std::max(b.max_depth, j->desc.maximum_depth);
@@ -651,6 +663,7 @@ void launch_iteration_kernels(mrg_device* d, uint32_t iters, bool timed, std::ve
         check_hip(launch_shadow(d->pool, jobs, it, d->shape, d->stream), "k_shadow");
         rec(3);
     }
+    check_hip(launch_validate(d->pool, jobs, chunks, d->shape, d->stream), "k_validate");
     check_hip(launch_accumulate(d->pool, jobs, chunks, d->shape, d->stream), "k_accumulate");
     rec(4);
 }
@@ -717,8 +730,10 @@ void run_batch(mrg_device* d, BatchPlan& b) {
         uint64_t samples = 0;
         for (const auto& c : b.chunks)
             if (c.job == i) samples += uint64_t(c.pixel_count) * c.sample_count;
-        j->recorded += samples;
-        j->paths += samples;
+        if (flags[i * 4] == 0) {
+            j->recorded += samples;
+            j->paths += samples;
+        }
         if (!shared) {
             j->ext_rays += ext;
             j->shadow_rays += st[kStatShadowRays];
@@ -854,6 +869,7 @@ int mrg_device_open(int ordinal, const mrg_device_options* options, mrg_device**
         check_hip(hipGetDeviceProperties(&d->props, ordinal), "hipGetDeviceProperties");
         check_hip(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking), "hipStreamCreate");
         check_hip(hipStreamCreateWithFlags(&d->copy_stream, hipStreamNonBlocking), "hipStreamCreate");
+        check_hip(hipStreamCreateWithFlags(&d->query_stream, hipStreamNonBlocking), "hipStreamCreate");
         const uint32_t cus = uint32_t(std::max(1, d->props.multiProcessorCount));
         const uint32_t per = d->opt.persistent_blocks_per_cu ? d->opt.persistent_blocks_per_cu : 4u;
         d->shape           = LaunchShape{cus, cus * per, cus * 8};
@@ -889,6 +905,7 @@ int mrg_device_close(mrg_device* d) {
         for (auto& e : d->query_events) (void)hipEventDestroy(e);
         (void)hipStreamDestroy(d->stream);
         (void)hipStreamDestroy(d->copy_stream);
+        (void)hipStreamDestroy(d->query_stream);
         delete d;  // remaining scenes/jobs owned by callers are leaked intentionally only if unreleased
     });
 }
@@ -973,20 +990,20 @@ static void run_query(mrg_scene* s, const void* d_rays, uint64_t n, float tmin, 
         throw std::invalid_argument("Ray interval must satisfy 0 <= minimum <= maximum");
     if (n >= (uint64_t(1) << 32) - 64) throw std::invalid_argument("too many rays for one call");
     mrg_device* d = s->device;
-    std::lock_guard<std::mutex> lock(d->mutex);  // the query buffers and spill area are shared
+    std::lock_guard<std::mutex> lock(d->query_mutex);  // one query at a time per device (shared spill area)
     check_hip(hipSetDevice(d->ordinal), "hipSetDevice");
     DevBuf st(16);
-    check_hip(hipMemsetAsync(st.ptr, 0, 16, d->stream), "hipMemsetAsync");
-    check_hip(hipMemsetAsync(d->query_work.ptr, 0, 64, d->stream), "hipMemsetAsync");
+    check_hip(hipMemsetAsync(st.ptr, 0, 16, d->query_stream), "hipMemsetAsync");
+    check_hip(hipMemsetAsync(d->query_work.ptr, 0, 64, d->query_stream), "hipMemsetAsync");
     hipEvent_t e0 = d->query_events[0], e1 = d->query_events[1];
-    check_hip(hipEventRecord(e0, d->stream), "hipEventRecord");
+    check_hip(hipEventRecord(e0, d->query_stream), "hipEventRecord");
     check_hip(launch_query(s->gpu, static_cast<const float*>(d_rays), uint32_t(n), tmin, tmax, static_cast<mrg_hit_raw*>(d_hits), d_bary,
-                           d_occ, d->query_work.as<uint32_t>(), d->pool, st.as<uint64_t>(), any, stats != nullptr, d->shape, d->stream),
+                           d_occ, d->query_work.as<uint32_t>(), d->query_pool, st.as<uint64_t>(), any, stats != nullptr, d->shape, d->query_stream),
               "k_query");
-    check_hip(hipEventRecord(e1, d->stream), "hipEventRecord");
+    check_hip(hipEventRecord(e1, d->query_stream), "hipEventRecord");
     uint64_t h[2] = {0, 0};
-    if (stats) check_hip(hipMemcpyAsync(h, st.ptr, 16, hipMemcpyDeviceToHost, d->stream), "read stats");
-    check_hip(hipStreamSynchronize(d->stream), "hipStreamSynchronize");
+    if (stats) check_hip(hipMemcpyAsync(h, st.ptr, 16, hipMemcpyDeviceToHost, d->query_stream), "read stats");
+    check_hip(hipStreamSynchronize(d->query_stream), "hipStreamSynchronize");
     if (kernel_ms) check_hip(hipEventElapsedTime(kernel_ms, e0, e1), "hipEventElapsedTime");
     if (stats) {
         stats->bounds_tests   = h[0];
@@ -1260,6 +1277,16 @@ int mrg_bvh_build_host(const float* positions, uint64_t n, uint64_t* out) {
         out[3] = max_leaf;
         out[4] = leaf_prims;
         out[5] = bad;
+    });
+}
+
+int mrg_device_memory(mrg_device* d, uint64_t* budget_bytes, uint64_t* resident_bytes, uint64_t* pool_bytes) {
+    return guarded([&] {
+        if (!d) throw std::invalid_argument("device is null");
+        std::lock_guard<std::mutex> lock(d->mutex);
+        if (budget_bytes) *budget_bytes = budget(d);
+        if (resident_bytes) *resident_bytes = d->resident_bytes;
+        if (pool_bytes) *pool_bytes = d->pool_bytes;
     });
 }
 

@@ -447,8 +447,10 @@ __global__ void __launch_bounds__(256) k_shadow(Pool P, const GpuJob* __restrict
     }
 }
 
-// Film commit: per pixel, samples in index order, FP64 sums (render_session.cpp:39-74).
-__global__ void __launch_bounds__(256) k_accumulate(Pool P, const GpuJob* __restrict__ jobs, const GpuChunk* __restrict__ chunks) {
+// Film validation of a batch (Film::commit checks the whole batch before publishing any of it,
+// render_session.cpp:39-51): nonfinite or negative sample radiance, or a pixel mean above FLT_MAX,
+// flags the job; k_accumulate then leaves that job's film untouched.
+__global__ void __launch_bounds__(256) k_validate(Pool P, const GpuJob* __restrict__ jobs, const GpuChunk* __restrict__ chunks) {
     const uint32_t chunk_count  = P.header[kHdrChunks];
     const uint32_t total_pixels = P.header[kHdrPixels];
     for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total_pixels; g += gridDim.x * blockDim.x) {
@@ -460,23 +462,46 @@ __global__ void __launch_bounds__(256) k_accumulate(Pool P, const GpuJob* __rest
         double sx = job.film_sums[static_cast<size_t>(pixel) * 3 + 0];
         double sy = job.film_sums[static_cast<size_t>(pixel) * 3 + 1];
         double sz = job.film_sums[static_cast<size_t>(pixel) * 3 + 2];
-        bool   bad = false;
+        uint32_t bad = 0;
         for (uint32_t s = 0; s < ch.sample_count; ++s) {
             const float4 v = P.rad[ch.slot_begin + s * ch.pixel_count + p];
             if (!(__builtin_isfinite(v.x) && __builtin_isfinite(v.y) && __builtin_isfinite(v.z)) || v.x < 0.0f || v.y < 0.0f || v.z < 0.0f)
-                bad = true;
+                bad |= 1u;
             sx = sx + static_cast<double>(v.x);
             sy = sy + static_cast<double>(v.y);
             sz = sz + static_cast<double>(v.z);
         }
-        const uint32_t count = job.film_counts[pixel] + ch.sample_count;
-        const double   fm    = 3.4028234663852886e38;
-        if (!(sx / count <= fm && sy / count <= fm && sz / count <= fm)) bad = true;
+        const double count = static_cast<double>(job.film_counts[pixel] + ch.sample_count);
+        const double fm    = 3.4028234663852886e38;
+        if (!(sx / count <= fm && sy / count <= fm && sz / count <= fm)) bad |= 2u;
+        if (bad) atomicOr(job.error_flag, bad);
+    }
+}
+
+// Film commit: per pixel, samples in index order, FP64 sums (render_session.cpp:39-74).
+__global__ void __launch_bounds__(256) k_accumulate(Pool P, const GpuJob* __restrict__ jobs, const GpuChunk* __restrict__ chunks) {
+    const uint32_t chunk_count  = P.header[kHdrChunks];
+    const uint32_t total_pixels = P.header[kHdrPixels];
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total_pixels; g += gridDim.x * blockDim.x) {
+        const uint32_t ci = find_chunk_by_pixel(chunks, chunk_count, g);
+        const GpuChunk ch = chunks[ci];
+        const uint32_t p  = g - ch.pixel_prefix;
+        const GpuJob&  job = jobs[ch.job];
+        if (*job.error_flag != 0u) continue;  // a failed batch publishes nothing
+        const uint32_t pixel = ch.pixel_begin + p;
+        double sx = job.film_sums[static_cast<size_t>(pixel) * 3 + 0];
+        double sy = job.film_sums[static_cast<size_t>(pixel) * 3 + 1];
+        double sz = job.film_sums[static_cast<size_t>(pixel) * 3 + 2];
+        for (uint32_t s = 0; s < ch.sample_count; ++s) {
+            const float4 v = P.rad[ch.slot_begin + s * ch.pixel_count + p];
+            sx = sx + static_cast<double>(v.x);
+            sy = sy + static_cast<double>(v.y);
+            sz = sz + static_cast<double>(v.z);
+        }
         job.film_sums[static_cast<size_t>(pixel) * 3 + 0] = sx;
         job.film_sums[static_cast<size_t>(pixel) * 3 + 1] = sy;
         job.film_sums[static_cast<size_t>(pixel) * 3 + 2] = sz;
-        job.film_counts[pixel] = count;
-        if (bad) atomicOr(job.error_flag, 2u);
+        job.film_counts[pixel] = job.film_counts[pixel] + ch.sample_count;
     }
 }
 
@@ -521,6 +546,11 @@ hipError_t launch_shade(const Pool& P, const GpuJob* jobs, uint32_t iter, const 
 
 hipError_t launch_shadow(const Pool& P, const GpuJob* jobs, uint32_t iter, const LaunchShape& shape, hipStream_t stream) {
     hipLaunchKernelGGL(k_shadow, dim3(shape.traversal_blocks), dim3(256), 0, stream, P, jobs, iter);
+    return hipGetLastError();
+}
+
+hipError_t launch_validate(const Pool& P, const GpuJob* jobs, const GpuChunk* chunks, const LaunchShape& shape, hipStream_t stream) {
+    hipLaunchKernelGGL(k_validate, dim3(grid_for(shape, 4)), dim3(256), 0, stream, P, jobs, chunks);
     return hipGetLastError();
 }
 

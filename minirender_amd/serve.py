@@ -1,0 +1,130 @@
+"""Render-request serving loop over one device: Poisson arrivals of uneven jobs against a set of
+resident (deduplicated) scenes, HBM-budget admission and progressive passes inside libmrg, random
+cancellation, and per-job latency (time to first pass, time to completion).
+
+The same request stream can be replayed on a device with the plain schedule (one job at a time)
+or the batched schedule (up to `max_jobs_in_flight` jobs share every wavefront batch); both see
+identical submission times, so throughput and latency compare directly.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+import threading
+import time
+
+import numpy as np
+
+from . import api, scenegen
+
+
+def scene_pool(seed=4):
+    """Eight distinct resident scenes from the three generators, about 1k to 500k triangles."""
+    return [
+        scenegen.mixed_terrain(cells=24, spheres=4, sphere_level=1, seed=seed),          # ~1.5k
+        scenegen.cornell_box(),                                                         # 36
+        scenegen.icosphere(level=4, seed=seed),                                         # 5k
+        scenegen.mixed_terrain(cells=96, spheres=8, sphere_level=2, seed=seed + 1),     # ~20k
+        scenegen.icosphere(level=6, seed=seed + 1),                                     # 82k
+        scenegen.mixed_terrain(cells=256, spheres=32, sphere_level=2, seed=seed + 2),   # ~140k
+        scenegen.icosphere(level=7, seed=seed + 2),                                     # 328k
+        scenegen.mixed_terrain(cells=440, spheres=16, sphere_level=3, seed=seed + 3),   # ~410k
+    ]
+
+
+@dataclasses.dataclass
+class Request:
+    index: int
+    scene: int
+    width: int
+    height: int
+    spp: int
+    max_depth: int
+    seed: int
+    arrival: float          # seconds after the stream starts
+    cancel_after: float     # seconds after submission, or -1
+
+
+def request_stream(count=64, rate=40.0, seed=5, cancel_fraction=0.05, scenes=8, side=(128, 1024), spp=(16, 1024),
+                   depth=(8, 16)):
+    """Poisson arrivals at `rate` jobs/s; sizes and sample counts log-uniform; seeded."""
+    r = np.random.default_rng(seed)
+    t = 0.0
+    reqs = []
+    for i in range(count):
+        t += r.exponential(1.0 / rate)
+        s = int(round(math.exp(r.uniform(math.log(side[0]), math.log(side[1])))))
+        n = int(round(math.exp(r.uniform(math.log(spp[0]), math.log(spp[1])))))
+        cancel = float(r.uniform(0.0, 0.5)) if r.random() < cancel_fraction else -1.0
+        reqs.append(Request(i, int(r.integers(0, scenes)), s, s, n, int(r.integers(depth[0], depth[1] + 1)),
+                            int(r.integers(0, 2**31)), t, cancel))
+    return reqs
+
+
+def _percentile(v, q):
+    return float(np.percentile(np.asarray(v, np.float64), q)) if len(v) else None
+
+
+def run_stream(device: api.Device, bvhs, scenes, requests, pass_samples=16, poll=0.002, keep_films=False):
+    """Submits each request at its arrival time, cancels the selected ones, waits for all of them and
+    returns per-job records plus aggregate throughput and latency percentiles."""
+    jobs = [None] * len(requests)
+    cancels = []
+    start = time.perf_counter()
+    for req in requests:
+        delay = req.arrival - (time.perf_counter() - start)
+        if delay > 0:
+            time.sleep(delay)
+        sc = scenes[req.scene]
+        jobs[req.index] = api.Job(bvhs[req.scene], sc.camera, api.RenderSettings(req.width, req.height, req.spp, req.seed),
+                                  api.IntegratorSettings("mis", req.max_depth, True, 3), pass_samples)
+        if req.cancel_after >= 0:
+            t = threading.Timer(req.cancel_after, jobs[req.index].cancel)
+            t.start()
+            cancels.append(t)
+    for j in jobs:
+        j.wait(-1)
+    end = time.perf_counter()
+    for t in cancels:
+        t.join()
+    records, films = [], {}
+    for req, j in zip(requests, jobs):
+        p = j.progress()
+        records.append(dict(index=req.index, state=p.state, samples=p.recorded_samples, total=p.total_samples,
+                            first_pass=p.first_pass_seconds, completion=p.completion_seconds, queued=p.queued_seconds))
+        if keep_films and p.state == "completed":
+            films[req.index] = j.film()
+        j.release()
+    done = [r for r in records if r["state"] == "completed"]
+    samples = sum(r["samples"] for r in records)
+    makespan = end - start
+    summary = dict(
+        jobs=len(records), completed=len(done), cancelled=sum(r["state"] == "cancelled" for r in records),
+        failed=sum(r["state"] == "failed" for r in records), samples=samples, makespan_s=makespan,
+        msamples_per_s=samples / makespan / 1e6,
+        first_pass_p50_s=_percentile([r["first_pass"] for r in records if r["first_pass"] > 0], 50),
+        first_pass_p99_s=_percentile([r["first_pass"] for r in records if r["first_pass"] > 0], 99),
+        completion_p50_s=_percentile([r["completion"] for r in done], 50),
+        completion_p99_s=_percentile([r["completion"] for r in done], 99),
+    )
+    return summary, records, films
+
+
+def compare(requests=None, scenes=None, hbm_budget=64 << 30, max_jobs=64, use_graph=True, keep_films=False):
+    """Replays one request stream on the plain and on the batched schedule."""
+    scenes = scenes or scene_pool()
+    requests = requests or request_stream()
+    out = {}
+    films = {}
+    for mode in ("plain", "batched"):
+        dev = api.Device(0, schedule=mode, hbm_budget_bytes=hbm_budget, max_jobs_in_flight=max_jobs,
+                         use_graph=use_graph)
+        bvhs = [dev.upload(s) for s in scenes]
+        summary, records, f = run_stream(dev, bvhs, scenes, requests, keep_films=keep_films)
+        out[mode] = summary
+        films[mode] = f
+        for b in bvhs:
+            b.release()
+        dev.close()
+    out["speedup"] = out["batched"]["msamples_per_s"] / out["plain"]["msamples_per_s"]
+    return out, films

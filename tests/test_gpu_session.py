@@ -1,0 +1,171 @@
+"""RenderSession contract on the GPU backend (tests/session_tests.cpp): bitwise equality with the
+serial render across pass sizes and schedules, pause/resume storms, cancellation that publishes only
+recorded samples, failure without partial publication, recovery, HBM-budget admission, and each job's
+result identical alone and inside a batched mix."""
+import time
+
+import numpy as np
+import pytest
+
+from minirender_amd import _native as N, api, scenegen
+from minirender_amd.scene import concat
+
+pytestmark = pytest.mark.gpu
+
+
+def sky():
+    return concat([], [], environment=(0.2, 0.4, 0.6))
+
+
+def wait_until(pred, timeout=20.0):
+    end = time.time() + timeout
+    while not pred():
+        assert time.time() < end, "timed out waiting for a render state"
+        time.sleep(0.001)
+
+
+def test_session_equals_serial_across_pass_sizes(device):
+    scene = scenegen.cornell_box()
+    rs, ins = api.RenderSettings(17, 13, 9, 413), api.IntegratorSettings("mis", 4, False, 3)
+    ref, _ = api.render(device, scene, rs, ins)
+    s = api.RenderSession(device)
+    for pass_samples in (1, 4, 8):
+        print(f"[session] pass_samples={pass_samples}", flush=True)
+        s.start(scene, rs, ins, pass_samples=pass_samples)
+        assert s.wait_for(20000)
+        p = s.progress()
+        assert p.state == "completed", p.error
+        assert p.recorded_samples == 17 * 13 * 9 and p.completed_pixels == 17 * 13
+        assert p.statistics["paths"] == p.recorded_samples
+        np.testing.assert_array_equal(s.result(), ref)
+        full, rec = s.preview(0)
+        assert rec == p.total_samples
+        np.testing.assert_array_equal(full, ref)
+        small, _ = s.preview(8)
+        assert small.shape[0] <= 8 and small.shape[1] <= 8
+
+
+def test_pause_resume_storm_and_cancel(device):
+    s = api.RenderSession(device)
+    s.start(sky(), api.RenderSettings(64, 40, 1000000, 31), api.IntegratorSettings(), pass_samples=4)
+    wait_until(lambda: s.progress().recorded_samples > 0)
+    with pytest.raises(RuntimeError):
+        s.start(sky(), api.RenderSettings(8, 8, 1, 1))
+    assert s.pause()
+    wait_until(lambda: s.progress().state == "paused")
+    count = s.progress().recorded_samples
+    time.sleep(0.05)
+    assert s.progress().recorded_samples == count
+    img, rec = s.preview(32)
+    assert rec == count
+    vals = {tuple(v) for v in img.reshape(-1, 3)}
+    assert vals <= {(0.0, 0.0, 0.0), tuple(np.float32([0.2, 0.4, 0.6]))}
+    assert any(v[0] > 0 for v in vals)
+    for _ in range(12):
+        assert s.resume()
+        assert s.pause()
+        wait_until(lambda: s.progress().state == "paused")
+    assert s.resume()
+    wait_until(lambda: s.progress().recorded_samples > count)
+    s.cancel()
+    assert s.wait_for(10000)
+    p = s.progress()
+    assert p.state == "cancelled"
+    assert count <= p.recorded_samples < p.total_samples
+    assert s.result() is None
+    sums, counts = s.film()
+    assert int(counts.sum()) == p.recorded_samples
+
+
+def test_cancel_restart_cycles_and_job_ids(device):
+    s = api.RenderSession(device)
+    first = None
+    for _ in range(8):
+        s.start(sky(), api.RenderSettings(64, 64, 1000000, 1))
+        first = first or s.progress().job_id
+        s.cancel()
+        assert s.wait_for(10000)
+        assert s.progress().state == "cancelled"
+    s.start(sky(), api.RenderSettings(8, 6, 5, 11))
+    assert s.wait_for(10000)
+    p = s.progress()
+    assert p.state == "completed" and p.recorded_samples == 240 and p.job_id == first + 8
+
+
+def test_overflow_fails_without_publishing(device):
+    """session_tests.cpp:203-242: a sample above FLT_MAX fails the job and publishes nothing."""
+    from minirender_amd.scene import Camera, material
+    q = np.array([[(-1, 0, -1), (-1, 0, 1), (1, 0, 1)], [(-1, 0, -1), (1, 0, 1), (1, 0, -1)]], np.float64)
+    scene = concat([(q, np.tile([0, 1.0, 0], (2, 3, 1)))], [material("dielectric", (1, 1, 1))],
+                   environment=(2e38, 2e38, 2e38),
+                   camera=Camera(position=(0, -1, 0), target=(0, 0, 0), up=(0, 0, 1)))
+    s = api.RenderSession(device)
+    s.start(scene, api.RenderSettings(4, 4, 64, 0), api.IntegratorSettings("mis", 2, False, 3))
+    assert s.wait_for(10000)
+    p = s.progress()
+    assert p.state == "failed" and p.error and p.recorded_samples == 0
+    img, _ = s.preview(0)
+    assert img is None or (img == 0).all()
+    # recovery
+    s.start(sky(), api.RenderSettings(8, 8, 1, 1))
+    assert s.wait_for(10000) and s.progress().state == "completed"
+
+
+def test_collapsed_scene_fails_then_recovers(device):
+    scene = scenegen.floor_scene()
+    scene.positions[:] = np.float32(1e10) + scene.positions * np.float32(1e-4)
+    s = api.RenderSession(device)
+    s.start(scene, api.RenderSettings(8, 8, 1, 1))
+    assert s.wait_for(1000)
+    p = s.progress()
+    assert p.state == "failed" and p.error
+    assert s.result() is None
+    s.start(sky(), api.RenderSettings(8, 8, 1, 1))
+    assert s.wait_for(10000) and s.progress().state == "completed" and not s.progress().error
+
+
+def test_hbm_budget_admission():
+    """A job that does not fit the budget is refused; two that do not fit together run one after the other."""
+    scene = scenegen.cornell_box()
+    film = 256 * 256 * 28
+    with api.Device(0, path_capacity=1 << 18) as probe:
+        b0 = probe.upload(scene)
+        m = probe.memory()
+        b0.release()
+    budget = m["pool_bytes"] + m["resident_bytes"] + 3 * film // 2
+    with api.Device(0, schedule="batched", path_capacity=1 << 18, hbm_budget_bytes=budget) as d:
+        bvh = d.upload(scene)
+        with pytest.raises(N.NotAdmitted):
+            api.Job(bvh, scene.camera, api.RenderSettings(1024, 1024, 1, 0), api.IntegratorSettings())
+        a = api.Job(bvh, scene.camera, api.RenderSettings(256, 256, 64, 0), api.IntegratorSettings())
+        b = api.Job(bvh, scene.camera, api.RenderSettings(256, 256, 64, 1), api.IntegratorSettings())
+        a.wait()
+        b.wait()
+        pa, pb = a.progress(), b.progress()
+        assert pa.state == pb.state == "completed"
+        assert pb.queued_seconds >= pa.queued_seconds
+        a.release()
+        b.release()
+
+
+def test_results_identical_alone_and_in_the_mix():
+    """Each job's film is bit-identical when rendered alone (plain) and co-scheduled (batched)."""
+    scenes = [scenegen.cornell_box(), scenegen.mixed_terrain(cells=48, spheres=8, sphere_level=2),
+              scenegen.icosphere(level=5)]
+    specs = [(0, 96, 64, 24, 8), (1, 128, 72, 16, 16), (2, 64, 64, 40, 10), (1, 33, 17, 7, 12), (0, 200, 150, 5, 8)]
+    films = {}
+    for mode in ("plain", "batched"):
+        print(f"[mix] {mode}", flush=True)
+        with api.Device(0, schedule=mode, path_capacity=1 << 17, use_graph=(mode == "batched")) as d:
+            bvhs = [d.upload(s) for s in scenes]
+            jobs = [api.Job(bvhs[k], scenes[k].camera, api.RenderSettings(w, h, n, 7 + i), api.IntegratorSettings("mis", dep),
+                            pass_samples=4) for i, (k, w, h, n, dep) in enumerate(specs)]
+            for j in jobs:
+                j.wait()
+                assert j.progress().state == "completed"
+            films[mode] = [j.film() for j in jobs]
+            for j in jobs:
+                j.release()
+    for (s0, c0), (s1, c1) in zip(films["plain"], films["batched"]):
+        np.testing.assert_array_equal(c0, c1)
+        np.testing.assert_array_equal(s0, s1)
