@@ -77,7 +77,7 @@ def bench_render(args):
     w = WORKLOADS[args.workload]
     log(f"[bench] generating workload {args.workload}")
     scene = make_scene(args.workload)
-    dev = api.Device(0, schedule="plain", persistent_blocks_per_cu=args.blocks_per_cu)
+    dev = api.Device(0, schedule="plain_fill", persistent_blocks_per_cu=args.blocks_per_cu)
     info = dev.query()
     t0 = time.perf_counter()
     bvh = dev.upload(scene)
@@ -126,13 +126,18 @@ def bench_render(args):
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"{args.workload} {w['width']}x{w['height']} {w['spp']}spp depth {w['max_depth']} "
-                               f"MIS RR3 ({scene.triangle_count} triangles, plain one-job schedule)",
+                               f"MIS RR3 ({scene.triangle_count} triangles, plain one-job schedule, pool-filling batches)",
                    "width": w["width"], "height": w["height"], "spp": w["spp"], "max_depth": w["max_depth"],
                    "triangles": scene.triangle_count},
         "mrays_per_s": round(rays / total / 1e6, 1),
         "kernel_ms_per_step": {k: round(kt[k] / args.steps, 2) for k in
                                ("generate_ms", "extend_ms", "shade_ms", "shadow_ms", "accumulate_ms")},
     }
+
+    bvh.release()
+    dev.close()
+    if not args.no_serving:
+        result["serving"] = bench_serving()
 
     # CPU baseline + per-ray algorithmic bytes from the FP64 oracle counters
     threads = min(16, os.cpu_count() or 1)
@@ -159,9 +164,26 @@ def bench_render(args):
                               "avg_launch_ms": round(avg_s * 1e3, 4),
                               "rays_per_launch": round(kt["extension_rays"] / launches, 1)}
     result["cpu_baseline"] = cpu
-    bvh.release()
-    dev.close()
     return result
+
+
+def bench_serving():
+    """Poisson request stream (40 jobs/s, 64 uneven jobs over 8 resident scenes, 64 GB HBM
+    budget, 16-spp progressive passes, 5% cancelled) replayed on the plain and the batched schedule."""
+    from minirender_amd import serve
+    log("[bench] serving: building the 8 resident scenes and the request stream")
+    scenes = serve.scene_pool()
+    reqs = serve.request_stream()
+    out, _ = serve.compare(reqs, scenes)
+    keep = ("completed", "cancelled", "failed", "samples", "makespan_s", "msamples_per_s", "first_pass_p50_s",
+            "first_pass_p99_s", "completion_p50_s", "completion_p99_s")
+    res = {m: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in out[m].items() if k in keep}
+           for m in ("plain", "plain_fill", "batched")}
+    res["speedup"] = round(out["speedup"], 3)
+    res["speedup_vs_plain_fill"] = round(out["speedup_vs_plain_fill"], 3)
+    res["stream"] = "64 jobs, Poisson 40/s, side 128-1024, spp 16-1024, depth 8-16, 8 scenes (36-408k tris), 5% cancel"
+    log(f"[bench] serving: plain {res['plain']['msamples_per_s']:.1f} Ms/s, batched {res['batched']['msamples_per_s']:.1f} Ms/s")
+    return res
 
 
 def bench_intersect(args):
@@ -198,6 +220,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="mixed", choices=["mixed", "cornell", "intersect"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--spp", type=int, default=0, help="override samples per pixel (calibration runs only)")
     args = ap.parse_args()

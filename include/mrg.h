@@ -51,8 +51,15 @@ enum { MRG_DIFFUSE = 0, MRG_CONDUCTOR = 1, MRG_MIRROR = 2, MRG_DIELECTRIC = 3 };
 /* SamplingMode (path_tracer.hpp:13). */
 enum { MRG_MODE_BSDF = 0, MRG_MODE_LIGHT = 1, MRG_MODE_MIS = 2 };
 
-/* Scheduling policy of a device's job loop. */
-enum { MRG_SCHEDULE_PLAIN = 0, MRG_SCHEDULE_BATCHED = 1 };
+/*
+ * Scheduling policy of a device's job loop.
+ *   PLAIN       one job at a time in submission order, one progressive pass (pass_samples spp, or as
+ *               many pixels of it as fit the path pool) per batch: the plain drop-in path.
+ *   BATCHED     up to max_jobs_in_flight jobs share every batch (fair share of the path pool, jobs
+ *               still waiting for their first pass first, leftover capacity handed back out).
+ *   PLAIN_FILL  one job at a time, each batch filled with as many samples of that job as fit.
+ */
+enum { MRG_SCHEDULE_PLAIN = 0, MRG_SCHEDULE_BATCHED = 1, MRG_SCHEDULE_PLAIN_FILL = 2 };
 
 typedef struct mrg_device mrg_device;
 typedef struct mrg_scene  mrg_scene;

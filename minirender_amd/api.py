@@ -82,7 +82,7 @@ class Device:
         o = N.DeviceOptions()
         o.hbm_budget_bytes = hbm_budget_bytes
         o.path_capacity = path_capacity
-        o.schedule = {"plain": 0, "batched": 1}[schedule]
+        o.schedule = {"plain": 0, "batched": 1, "plain_fill": 2}[schedule]
         o.max_jobs_in_flight = max_jobs_in_flight
         o.use_graph = int(bool(use_graph))
         o.persistent_blocks_per_cu = persistent_blocks_per_cu

@@ -592,46 +592,65 @@ BatchPlan plan_batch_locked(mrg_device* d) {
     BatchPlan b;
     const uint32_t cap      = d->pool.capacity;
     const bool     batched  = d->opt.schedule == MRG_SCHEDULE_BATCHED;
+    const bool     fill     = d->opt.schedule == MRG_SCHEDULE_PLAIN_FILL;
     const uint32_t max_jobs = batched ? d->max_jobs : 1u;
-    // Candidates in arrival order, jobs still waiting for their first pass ahead of the others.
+    // Plain schedules: the oldest runnable job only (FIFO, one job at a time). Batched: candidates in
+    // arrival order, jobs still waiting for their first pass ahead of the others.
     std::vector<mrg_job*> cand;
-    for (int pass = 0; pass < 2; ++pass)
+    for (int pass = 0; pass < (batched ? 2 : 1); ++pass)
         for (mrg_job* j : d->jobs)
-            if (runnable(j) && (j->first_pass_done == (pass == 1)) && cand.size() < max_jobs) cand.push_back(j);
+            if (runnable(j) && (!batched || j->first_pass_done == (pass == 1)) && cand.size() < max_jobs) cand.push_back(j);
     // Each co-scheduled job gets at most a fair share of the path pool in this batch.
     const uint32_t share = cand.empty() ? cap : std::max<uint32_t>(cap / uint32_t(cand.size()), 1u);
     for (mrg_job* j : cand) {
         if (b.slots >= cap) break;
         const uint32_t slot = uint32_t(b.jobs.size());
         // one chunk per job per batch: a pixel never appears twice in one accumulate launch
-        if (!plan_chunk(j, slot, std::min(cap - b.slots, share), !batched, b)) continue;
+        if (!plan_chunk(j, slot, std::min(cap - b.slots, share), fill, b)) continue;
         b.jobs.push_back(j);
         b.gjobs.push_back(gpu_job(j));
         b.max_depth = std::max(b.max_depth, j->desc.maximum_depth);
     }
     if (batched && !b.jobs.empty() && b.slots < cap) {
-        // spread the remaining capacity over whole-image chunks by raising their sample counts
+        // Hand the capacity left by the fair shares back out: whole-image chunks take more samples,
+        // pixel-split chunks take more pixels of their current sample range.
         bool grew = true;
         while (grew && b.slots < cap) {
             grew = false;
-            for (size_t i = 0; i < b.chunks.size(); ++i) {
+            for (size_t i = 0; i < b.chunks.size() && b.slots < cap; ++i) {
                 GpuChunk& c = b.chunks[i];
                 mrg_job*  j = b.jobs[c.job];
                 const uint32_t pixels = j->desc.width * j->desc.height;
-                if (c.pixel_count != pixels || j->next_sample >= j->desc.samples_per_pixel) continue;
-                const uint32_t add = std::min({pass_of(j), j->desc.samples_per_pixel - j->next_sample, (cap - b.slots) / pixels});
-                if (add == 0) continue;
-                c.sample_count += add;
-                j->next_sample += add;
-                b.slots += add * pixels;
-                grew = true;
+                if (c.pixel_count == pixels) {
+                    if (j->next_sample >= j->desc.samples_per_pixel) continue;
+                    const uint32_t add = std::min({pass_of(j), j->desc.samples_per_pixel - j->next_sample, (cap - b.slots) / pixels});
+                    if (add == 0) continue;
+                    c.sample_count += add;
+                    j->next_sample += add;
+                    b.slots += add * pixels;
+                    grew = true;
+                } else if (j->next_pixel == c.pixel_begin + c.pixel_count && j->next_pixel != 0) {
+                    const uint32_t add = std::min(pixels - j->next_pixel, (cap - b.slots) / c.sample_count);
+                    if (add == 0) continue;
+                    c.pixel_count += add;
+                    j->next_pixel += add;
+                    b.slots += add * c.sample_count;
+                    b.pixels += add;
+                    if (j->next_pixel == pixels) {
+                        j->next_pixel = 0;
+                        j->next_sample += c.sample_count;
+                    }
+                    grew = true;
+                }
             }
         }
-        // recompute slot offsets
-        uint32_t s = 0;
+        // recompute slot and pixel offsets
+        uint32_t s = 0, px = 0;
         for (auto& c : b.chunks) {
-            c.slot_begin = s;
+            c.slot_begin   = s;
+            c.pixel_prefix = px;
             s += c.pixel_count * c.sample_count;
+            px += c.pixel_count;
         }
     }
     for (mrg_job* j : b.jobs) {

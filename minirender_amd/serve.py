@@ -116,7 +116,7 @@ def compare(requests=None, scenes=None, hbm_budget=64 << 30, max_jobs=64, use_gr
     requests = requests or request_stream()
     out = {}
     films = {}
-    for mode in ("plain", "batched"):
+    for mode in ("plain", "plain_fill", "batched"):
         dev = api.Device(0, schedule=mode, hbm_budget_bytes=hbm_budget, max_jobs_in_flight=max_jobs,
                          use_graph=use_graph)
         bvhs = [dev.upload(s) for s in scenes]
@@ -127,4 +127,5 @@ def compare(requests=None, scenes=None, hbm_budget=64 << 30, max_jobs=64, use_gr
             b.release()
         dev.close()
     out["speedup"] = out["batched"]["msamples_per_s"] / out["plain"]["msamples_per_s"]
+    out["speedup_vs_plain_fill"] = out["batched"]["msamples_per_s"] / out["plain_fill"]["msamples_per_s"]
     return out, films
