@@ -187,14 +187,17 @@ def bench_serving():
 
 
 def bench_intersect(args):
-    """16M seeded rays against the 1.31M-triangle displaced icosphere, closest hit."""
+    """16,777,216 seeded rays against the 1,310,720-triangle displaced icosphere, closest
+    hit, rays and hits resident in HBM. Parity and per-ray algorithmic bytes come from the FP64 oracle
+    on every 256th ray of the same set."""
     import ctypes
 
-    from minirender_amd import _native as N, api, scenegen
+    from minirender_amd import api, scenegen
     scene = make_scene("intersect")
     n = 1 << 24
+    log("[bench] intersect: generating 16M rays")
     rays = scenegen.intersection_rays(scene, n)
-    dev = api.Device(0)
+    dev = api.Device(0, persistent_blocks_per_cu=args.blocks_per_cu)
     bvh = dev.upload(scene)
     hip = ctypes.CDLL("libamdhip64.so")
     d_rays, d_hits = ctypes.c_void_p(), ctypes.c_void_p()
@@ -204,13 +207,87 @@ def bench_intersect(args):
     for _ in range(args.warmup):
         bvh.intersect_device(d_rays.value, n, d_hits.value)
     ms = [bvh.intersect_device(d_rays.value, n, d_hits.value) for _ in range(args.steps)]
+    hits = np.empty(n, np.dtype([("t", "<f4"), ("primitive", "<i4")]))
+    assert hip.hipMemcpy(hits.ctypes.data_as(ctypes.c_void_p), d_hits, ctypes.c_size_t(n * 8), 2) == 0
     hip.hipFree(d_rays)
     hip.hipFree(d_hits)
     avg = sum(ms) / len(ms)
-    return {"metric": "Mrays/s closest hit (intersection microbench)", "value": round(n / avg / 1e3, 1),
-            "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(avg, 3),
-            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-            "config": {"workload": f"icosphere level 8 ({scene.triangle_count} tris), {n} rays"}}
+    value = n / avg / 1e3
+    result = {"metric": "Mrays/s closest hit (intersection microbench)", "value": round(value, 1), "unit": "Mrays/s",
+              "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(avg, 3),
+              "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+              "config": {"workload": f"icosphere level 8 ({scene.triangle_count} tris, 5% value-noise displacement), "
+                                     f"{n} rays, closest hit", "triangles": scene.triangle_count, "rays": n}}
+    if not args.no_cpu:
+        from oracle import oracle as O
+        threads = min(16, os.cpu_count() or 1)
+        o = O.OracleScene(scene)
+        idx = np.arange(0, n, 256)
+        sample = rays[idx].astype(np.float64)
+        t = time.perf_counter()
+        ref = o.intersect(sample, precision=64, second=True, threads=threads)
+        dt = time.perf_counter() - t
+        b = ref["stats"]["bounds_tests"] / len(idx)
+        tt = ref["stats"]["triangle_tests"] / len(idx)
+        b_ray = B_RAY_IN + B_HIT_OUT + B_BOX * b + B_TRI * tt
+        achieved = n * b_ray / (avg / 1e3) / 1e9
+        with np.errstate(invalid="ignore"):
+            near = np.abs(ref["t2"] - ref["t"]) <= 1e-6 * np.abs(ref["t"])
+        mism = hits["primitive"][idx] != ref["prim"]
+        gp, gt = hits["primitive"][idx], hits["t"][idx]
+        edge = np.zeros(len(idx), bool)
+        for i in np.where(mism & (gp >= 0) & (ref["prim"] >= 0))[0]:
+            if abs(float(gt[i]) - ref["t"][i]) <= 1e-6 * ref["t"][i]:
+                va = {tuple(v) for v in scene.positions[gp[i]].tolist()}
+                edge[i] = bool(va & {tuple(v) for v in scene.positions[ref["prim"][i]].tolist()})
+        both = (~mism) & (ref["prim"] >= 0)
+        rel = np.abs(hits["t"][idx][both] - ref["t"][both]) / ref["t"][both]
+        result["parity"] = {"sample_rays": int(len(idx)), "id_mismatch": int(mism.sum()),
+                            "near_ties": int(near.sum()), "shared_edge_ties": int(edge.sum()),
+                            "id_mismatch_outside_near_ties": int((mism & ~near & ~edge).sum()),
+                            "max_t_rel": float(rel.max(initial=0))}
+        result["roofline"] = {"kernel": "k_query", "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                              "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                              "bytes_per_ray": round(b_ray, 1), "bounds_tests_per_ray": round(b, 2),
+                              "triangle_tests_per_ray": round(tt, 2), "avg_launch_ms": round(avg, 3)}
+        result["cpu_baseline"] = {"value": round(len(idx) / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads,
+                                  "kind": "port", "sample": f"every 256th ray ({len(idx)} rays), FP64 oracle closest "
+                                  f"+ second hit, {dt:.1f}s"}
+    bvh.release()
+    dev.close()
+    return result
+
+
+def bench_convergence(args):
+    """RMSE of 64-spp Cornell renders (GPU FP32, and the FP64 oracle) against an FP64 render at
+    65,536 spp of a 32x32 crop (configs[0] camera, depth 8, RR 3, seed 0)."""
+    from minirender_amd import api, scenegen
+    from oracle import oracle as O
+    w = WORKLOADS["cornell"]
+    scene = scenegen.cornell_box()
+    crop = (240, 200, 272, 232)
+    threads = min(16, os.cpu_count() or 1)
+    o = O.OracleScene(scene)
+    t = time.perf_counter()
+    ref, _ = o.render(scene.camera, w["width"], w["height"], 65536, seed=99, max_depth=8, precision=64,
+                      threads=threads, crop=crop)
+    t_ref = time.perf_counter() - t
+    cpu64, _ = o.render(scene.camera, w["width"], w["height"], w["spp"], seed=0, max_depth=8, precision=64,
+                        threads=threads, crop=crop)
+    with api.Device(0) as dev:
+        res = {}
+        for spp in (w["spp"], 1024, 16384):
+            img, _ = api.render(dev, scene, api.RenderSettings(w["width"], w["height"], spp, 0),
+                                api.IntegratorSettings("mis", 8, True, 3))
+            res[spp] = img[crop[1]:crop[3], crop[0]:crop[2]]
+    rmse = lambda a: float(np.sqrt(np.mean((a.astype(np.float64) - ref.astype(np.float64)) ** 2)))
+    out = {"metric": "RMSE vs FP64 65536-spp reference (linear RGB)", "crop": crop,
+           "reference_seconds": round(t_ref, 1), "reference_threads": threads,
+           "rmse_gpu_fp32_64spp": rmse(res[w["spp"]]), "rmse_cpu_fp64_64spp": rmse(cpu64),
+           "rmse_gpu_fp32_1024spp": rmse(res[1024]), "rmse_gpu_fp32_16384spp": rmse(res[16384]),
+           "mean_reference": float(ref.mean())}
+    out["gpu_over_cpu_rmse_64spp"] = out["rmse_gpu_fp32_64spp"] / out["rmse_cpu_fp64_64spp"]
+    return out
 
 
 def main():
@@ -218,7 +295,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="mixed", choices=["mixed", "cornell", "intersect"])
+    ap.add_argument("--workload", default="mixed", choices=["mixed", "cornell", "intersect", "convergence"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--blocks-per-cu", type=int, default=0)
@@ -230,7 +307,12 @@ def main():
         print("bench.py measures one GPU only: this build runs the path on a single MI355X (see DESIGN.md for how "
               "it would shard); use --gpus 1", file=sys.stderr)
         sys.exit(2)
-    res = bench_intersect(args) if args.workload == "intersect" else bench_render(args)
+    if args.workload == "intersect":
+        res = bench_intersect(args)
+    elif args.workload == "convergence":
+        res = bench_convergence(args)
+    else:
+        res = bench_render(args)
     print(json.dumps(res), flush=True)
 
 

@@ -143,7 +143,7 @@ def icosphere(level=8, amplitude=0.05, seed=2, radius=1.0) -> Scene:
     return concat([(pos, nrm)], mats, environment=(0.5, 0.5, 0.5), name=f"icosphere{level}")
 
 
-def intersection_rays(scene: Scene, count, seed=2, start=0, scale=1.2):
+def intersection_rays(scene: Scene, count, seed=2, start=0, scale=1.2, indices=None):
     """Seeded ray set: ray i uses Sampler::for_sample(seed, i, 0); origins uniform in the `scale`x
     AABB about its centre (3 dims), directions uniform on the sphere (2 dims). FP32 rays, as
     consumed by the GPU; the FP64 oracle is given the same float values widened."""
@@ -151,7 +151,7 @@ def intersection_rays(scene: Scene, count, seed=2, start=0, scale=1.2):
     lo, hi = p.min(0), p.max(0)
     c, h = (lo + hi) / 2, (hi - lo) / 2 * scale
     lo, hi = c - h, c + h
-    idx = np.arange(start, start + count, dtype=np.uint64)
+    idx = np.arange(start, start + count, dtype=np.uint64) if indices is None else np.asarray(indices, np.uint64)
     s = rng.Streams.for_sample(np.uint64(seed), idx, np.uint64(0))
     u = np.stack([s.next_1d() for _ in range(5)], axis=1)
     origin = lo + (hi - lo) * u[:, :3]
