@@ -257,10 +257,11 @@ int mrg_device_kernel_times(mrg_device* device, mrg_kernel_times* out, int reset
 int mrg_device_memory(mrg_device* device, uint64_t* budget_bytes, uint64_t* resident_bytes, uint64_t* pool_bytes);
 
 /*
- * Host-only BVH build of [n][3][3] FP32 triangles (no device needed): out[0] nodes, out[1] leaves,
- * out[2] maximum depth, out[3] largest leaf, out[4] primitives referenced by leaves, out[5] structural
- * errors (a primitive not referenced exactly once, or a reference out of range). Diagnostic of the
- * builder behind mrg_scene_upload (the BvhBuildStatistics of bvh.hpp:29-34).
+ * Host-only BVH build of [n][3][3] FP32 triangles (no device needed): out[0] four-wide nodes,
+ * out[1] leaves, out[2] maximum binary depth, out[3] largest leaf, out[4] primitives referenced by
+ * leaves, out[5] structural errors (a primitive not referenced exactly once, a reference out of range,
+ * or a worst-case stack above the traversal bound), out[6] worst-case traversal stack entries.
+ * Diagnostic of the builder behind mrg_scene_upload (the BvhBuildStatistics of bvh.hpp:29-34).
  */
 int mrg_bvh_build_host(const float* positions, uint64_t n, uint64_t* out);
 

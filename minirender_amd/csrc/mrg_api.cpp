@@ -321,7 +321,8 @@ mrg_scene* build_scene(mrg_device* d, const mrg_scene_desc& s, const std::string
     sc->device = d;
     sc->hash   = hash;
     const BvhBuildResult bvh = build_bvh(s.positions, n, 0);
-    if (bvh.max_depth > kMaxDepth) throw Error(MRG_DEVICE_ERROR, "BVH depth exceeds the traversal stack bound");
+    if (bvh.max_depth > kMaxDepth || bvh.max_stack > kStackTotal)
+        throw Error(MRG_DEVICE_ERROR, "BVH exceeds the traversal stack bound");
     // leaf-order triangle records
     std::vector<GpuTri>   tris(n), nrms(n);
     std::vector<uint32_t> slot_instance(n);
@@ -443,7 +444,7 @@ mrg_scene* build_scene(mrg_device* d, const mrg_scene_desc& s, const std::string
     upload(sc->areas, areas);
     upload(sc->directional, dirs);
     GpuScene& g     = sc->gpu;
-    g.nodes         = sc->nodes.as<GpuNode>();
+    g.nodes         = sc->nodes.as<GpuNode4>();
     g.tris          = sc->tris.as<GpuTri>();
     g.normals       = sc->normals.as<GpuTri>();
     g.slot_instance = sc->slot_instance.as<uint32_t>();
@@ -1272,9 +1273,8 @@ int mrg_bvh_build_host(const float* positions, uint64_t n, uint64_t* out) {
         // structural check: every primitive exactly once, leaves <= kMaxLeaf, child boxes inside parents
         std::vector<uint32_t> seen(n, 0);
         uint64_t max_leaf = 0, leaf_prims = 0, bad = 0;
-        for (const GpuNode& g : b.nodes) {
-            const uint32_t refs[2] = {g.ref0, g.ref1};
-            for (uint32_t r : refs) {
+        for (const GpuNode4& g : b.nodes) {
+            for (uint32_t r : g.ref) {
                 if (r == kEmptyRef) continue;
                 if (is_leaf(r)) {
                     const uint32_t f = leaf_first(r), c = leaf_count(r);
@@ -1295,7 +1295,8 @@ int mrg_bvh_build_host(const float* positions, uint64_t n, uint64_t* out) {
         out[2] = b.max_depth;
         out[3] = max_leaf;
         out[4] = leaf_prims;
-        out[5] = bad;
+        out[5] = bad + (b.max_stack > kStackTotal ? 1 : 0);
+        out[6] = b.max_stack;
     });
 }
 

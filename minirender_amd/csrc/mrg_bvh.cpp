@@ -17,6 +17,7 @@
 #include <numeric>
 #include <stdexcept>
 #include <thread>
+#include <unordered_map>
 
 namespace mrg {
 namespace {
@@ -185,42 +186,80 @@ struct Builder {
     }
 };
 
-void set_child(GpuNode& g, int which, const Box& b, uint32_t ref) {
-    if (which == 0) {
-        g.lo0[0] = b.lo[0];
-        g.lo0[1] = b.lo[1];
-        g.lo0[2] = b.lo[2];
-        g.hi0x   = b.hi[0];
-        g.hi0yz[0] = b.hi[1];
-        g.hi0yz[1] = b.hi[2];
-        g.ref0   = ref;
-    } else {
-        g.lo1xy[0] = b.lo[0];
-        g.lo1xy[1] = b.lo[1];
-        g.lo1z     = b.lo[2];
-        g.hi1[0]   = b.hi[0];
-        g.hi1[1]   = b.hi[1];
-        g.hi1[2]   = b.hi[2];
-        g.ref1     = ref;
-    }
-}
+struct Collapser {
+    std::vector<GpuNode4>& out;
+    uint64_t               max_stack = 0;
+    std::unordered_map<const TNode*, uint32_t> h;
 
-// Depth-first flattening: an inner node's left subtree follows it directly.
-uint32_t flatten(const TNode* n, std::vector<GpuNode>& out) {
-    const uint32_t index = uint32_t(out.size());
-    out.emplace_back();
-    uint32_t refs[2];
-    const TNode* kids[2] = {n->l.get(), n->r.get()};
-    for (int k = 0; k < 2; ++k) {
-        const TNode* c = kids[k];
-        refs[k]        = c->l ? flatten(c, out) : leaf_ref(c->first, c->count);
+    uint32_t hgt(const TNode* n) {
+        if (!n->l) return 0;
+        auto it = h.find(n);
+        if (it != h.end()) return it->second;
+        const uint32_t v = 1 + std::max(hgt(n->l.get()), hgt(n->r.get()));
+        h[n] = v;
+        return v;
     }
-    GpuNode& g = out[index];
-    g          = GpuNode{};
-    set_child(g, 0, kids[0]->box, refs[0]);
-    set_child(g, 1, kids[1]->box, refs[1]);
-    return index;
-}
+
+    // Builds the four-wide node for binary node n. `acc` = stack entries that may be pending when
+    // this node is visited. Children are gathered by repeatedly opening the largest-area inner child,
+    // as long as every child can still be traversed within the stack budget (binary below it in the
+    // worst case, one push per inner level).
+    uint32_t emit(const TNode* n, uint64_t acc) {
+        std::vector<const TNode*> kids = {n->l.get(), n->r.get()};
+        const uint64_t budget = kStackTotal - 2;
+        for (;;) {
+            if (kids.size() == 4) break;
+            int    best = -1;
+            double area = -1;
+            for (int k = 0; k < int(kids.size()); ++k) {
+                if (!kids[k]->l) continue;
+                const double a = kids[k]->box.area();
+                if (a > area) {
+                    area = a;
+                    best = k;
+                }
+            }
+            if (best < 0) break;
+            std::vector<const TNode*> next;
+            for (int k = 0; k < int(kids.size()); ++k) {
+                if (k == best) {
+                    next.push_back(kids[k]->l.get());
+                    next.push_back(kids[k]->r.get());
+                } else {
+                    next.push_back(kids[k]);
+                }
+            }
+            const uint64_t child_acc = acc + next.size() - 1;
+            bool ok = true;
+            for (const TNode* c : next) ok = ok && child_acc + hgt(c) <= budget;
+            if (!ok) break;
+            kids.swap(next);
+        }
+        const uint32_t index = uint32_t(out.size());
+        out.emplace_back();
+        const uint64_t child_acc = acc + kids.size() - 1;
+        max_stack = std::max(max_stack, child_acc);
+        uint32_t refs[4] = {kEmptyRef, kEmptyRef, kEmptyRef, kEmptyRef};
+        for (size_t k = 0; k < kids.size(); ++k)
+            refs[k] = kids[k]->l ? emit(kids[k], child_acc) : leaf_ref(kids[k]->first, kids[k]->count);
+        GpuNode4& g = out[index];
+        for (int k = 0; k < 4; ++k) {
+            Box b;
+            if (k < int(kids.size())) b = kids[k]->box;
+            g.lox[k] = b.lo[0];
+            g.loy[k] = b.lo[1];
+            g.loz[k] = b.lo[2];
+            g.hix[k] = b.hi[0];
+            g.hiy[k] = b.hi[1];
+            g.hiz[k] = b.hi[2];
+            g.ref[k] = refs[k];
+            g.pad[k] = 0;
+        }
+        return index;
+    }
+};
+
+uint64_t count_binary(const TNode* n) { return n->l ? 1 + count_binary(n->l.get()) + count_binary(n->r.get()) : 0; }
 
 }  // namespace
 
@@ -243,16 +282,24 @@ BvhBuildResult build_bvh(const float* positions, uint64_t count, unsigned thread
     if (threads == 0) threads = std::max(1u, std::thread::hardware_concurrency());
     Builder b(prims, out.order, int(threads));
     std::unique_ptr<TNode> root = b.build(0, uint32_t(count), 0);
-    out.nodes.reserve(count);
+    out.nodes.reserve(count / 2 + 1);
     if (!root->l) {
-        // A single triangle: a root with one leaf child and one empty child.
-        GpuNode g{};
-        set_child(g, 0, root->box, leaf_ref(root->first, root->count));
-        Box empty;
-        set_child(g, 1, empty, kEmptyRef);
+        // A single triangle: a root with one leaf child and three empty slots.
+        GpuNode4 g{};
+        for (int k = 0; k < 4; ++k) {
+            Box b;
+            if (k == 0) b = root->box;
+            g.lox[k] = b.lo[0]; g.loy[k] = b.lo[1]; g.loz[k] = b.lo[2];
+            g.hix[k] = b.hi[0]; g.hiy[k] = b.hi[1]; g.hiz[k] = b.hi[2];
+            g.ref[k] = k == 0 ? leaf_ref(root->first, root->count) : kEmptyRef;
+        }
         out.nodes.push_back(g);
+        out.max_stack = 0;
     } else {
-        flatten(root.get(), out.nodes);
+        Collapser c{out.nodes};
+        c.emit(root.get(), 0);
+        out.max_stack    = c.max_stack;
+        out.binary_nodes = count_binary(root.get());
     }
     out.leaves    = b.leaves.load();
     out.max_depth = b.max_depth.load();

@@ -9,14 +9,17 @@
 namespace mrg {
 
 struct BvhBuildResult {
-    std::vector<GpuNode>  nodes;
+    std::vector<GpuNode4> nodes;
     std::vector<uint32_t> order;      // leaf order -> scene-order primitive
     uint64_t              leaves = 0;
-    uint64_t              max_depth = 0;
+    uint64_t              max_depth = 0;     // binary-tree depth
+    uint64_t              max_stack = 0;     // worst-case traversal stack entries of the 4-wide tree
+    uint64_t              binary_nodes = 0;
 };
 
-// Binned SAH build over world-space FP32 triangles ([n][3][3]). Leaves hold <= kMaxLeaf triangles
-// and the depth never exceeds kMaxDepth (median splits take over where SAH would go deeper).
+// Binned SAH build over world-space FP32 triangles ([n][3][3]) into a binary tree (leaves <= kMaxLeaf
+// triangles, depth <= kMaxDepth: median splits take over where SAH would go deeper), collapsed into
+// four-wide nodes whose worst-case traversal stack stays within kStackTotal.
 BvhBuildResult build_bvh(const float* positions, uint64_t count, unsigned threads);
 
 }  // namespace mrg

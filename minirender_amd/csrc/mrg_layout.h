@@ -1,9 +1,9 @@
 // Device data layout shared by the host builder/scheduler (C++) and the HIP kernels.
 //
 // HBM layout of one resident scene:
-//   nodes      [node_count]    GpuNode, 64 B each (two per 128 B cache line): both child boxes of an
-//                              inner node plus the two child references, so one node fetch gives the
-//                              whole two-box test. Depth-first order, root = 0.
+//   nodes      [node_count]    GpuNode4, 128 B each (one cache line): the four child boxes of an inner
+//                              node plus the four child references, so one node fetch gives the whole
+//                              four-box test. Depth-first order, root = 0.
 //   tris       [n]             GpuTri, 48 B: three float4 vertices in leaf order; v0.w carries the
 //                              scene-order primitive index (the tie-break key of bvh.cpp:273-276).
 //   normals    [n]             GpuTri layout, per-corner world normals in leaf order.
@@ -27,9 +27,9 @@ namespace mrg {
 constexpr uint32_t kLeafBit    = 0x80000000u;
 constexpr uint32_t kEmptyRef   = 0xffffffffu;
 constexpr uint32_t kMaxLeaf    = 16;   // triangles per leaf (count-1 packed in 4 bits)
-constexpr uint32_t kMaxDepth   = 30;   // builder guarantee: traversal stack never exceeds 32
+constexpr uint32_t kMaxDepth   = 30;   // binary-tree depth bound of the builder
 constexpr uint32_t kStackLds   = 16;   // stack entries per lane kept in LDS
-constexpr uint32_t kStackTotal = 32;   // LDS + global spill entries per lane
+constexpr uint32_t kStackTotal = 64;   // LDS + global spill entries per lane (builder-checked bound)
 
 MRG_HD inline uint32_t leaf_ref(uint32_t first, uint32_t count) { return kLeafBit | ((count - 1u) << 27) | first; }
 MRG_HD inline bool     is_leaf(uint32_t ref) { return (ref & kLeafBit) != 0u; }
@@ -43,6 +43,15 @@ struct alignas(16) GpuNode {
     uint32_t ref0, ref1, pad0, pad1;
 };
 static_assert(sizeof(GpuNode) == 64, "GpuNode must be 64 bytes");
+
+// Four-wide node, one 128 B cache line: child boxes in SoA order (lo.x[4], lo.y[4], lo.z[4],
+// hi.x[4], hi.y[4], hi.z[4]) then the four child references (kEmptyRef for unused slots).
+struct alignas(128) GpuNode4 {
+    float    lox[4], loy[4], loz[4], hix[4], hiy[4], hiz[4];
+    uint32_t ref[4];
+    uint32_t pad[4];
+};
+static_assert(sizeof(GpuNode4) == 128, "GpuNode4 must be 128 bytes");
 
 struct alignas(16) GpuTri {
     float v0[3];
@@ -86,7 +95,7 @@ struct GpuDirectional {
 
 // Per-scene pointers and constants handed to the kernels by value.
 struct GpuScene {
-    const GpuNode*        nodes;
+    const GpuNode4*       nodes;
     const GpuTri*         tris;
     const GpuTri*         normals;
     const uint32_t*       slot_instance;

@@ -1,6 +1,7 @@
-// BVH traversal for one ray per lane: two-box nodes from HBM, the current node in VGPRs, a stack of
-// (child reference, entry distance) pairs in LDS (kStackLds per lane) that spills to a per-lane
-// global region for deeper entries (the builder bounds the depth, so 32 entries always suffice).
+// BVH traversal for one ray per lane: four-wide 128 B nodes from HBM, the current node in VGPRs, a
+// stack of (child reference, entry distance) pairs in LDS (kStackLds per lane) that spills to a
+// per-lane global region for deeper entries (the builder bounds the worst-case stack by kStackTotal).
+// Children of a node are visited nearest first (a five-exchange sorting network on entry distance).
 //
 // Box test: conservative slab test (each slab bound computed as (bound - origin) * (1/d), then the
 // entry/exit distances widened by 2^-20 relative), so a box is never culled when the watertight
@@ -131,7 +132,7 @@ struct StackView {
 };
 
 __device__ __forceinline__ void stack_push(const StackView& s, uint32_t& sp, uint32_t ref, float t) {
-    if (sp >= kStackTotal) return;  // unreachable: the builder bounds the depth by kMaxDepth < kStackTotal
+    if (sp >= kStackTotal) return;  // unreachable: the builder bounds the worst-case stack by kStackTotal
     if (sp < kStackLds) {
         s.lds_ref[sp * s.block + s.lane] = ref;
         s.lds_t[sp * s.block + s.lane]   = t;
@@ -155,6 +156,16 @@ __device__ __forceinline__ void stack_pop(const StackView& s, uint32_t& sp, uint
     }
 }
 
+__device__ __forceinline__ void sort2(float& ta, uint32_t& ra, float& tb, uint32_t& rb) {
+    const bool     swap = tb < ta;
+    const float    t    = swap ? tb : ta;
+    const uint32_t q    = swap ? rb : ra;
+    tb                  = swap ? ta : tb;
+    rb                  = swap ? ra : rb;
+    ta                  = t;
+    ra                  = q;
+}
+
 struct TraceResult {
     float    t, b0, b1, b2;
     uint32_t slot;  // leaf-order triangle slot, 0xffffffff on a miss
@@ -176,28 +187,28 @@ __device__ __forceinline__ TraceResult trace_ray(const GpuScene& sc, const RayPr
     for (;;) {
         if (!is_leaf(ref)) {
             const float4* np = reinterpret_cast<const float4*>(sc.nodes + ref);
-            const float4  na = np[0];
-            const float4  nb = np[1];
-            const float4  nc = np[2];
-            const uint4   nd = reinterpret_cast<const uint4*>(np)[3];
-            const float   tl = box_enter(r, na.x, na.y, na.z, na.w, nb.x, nb.y, tmin, res.t);
-            const float   tr = nd.y != kEmptyRef ? box_enter(r, nb.z, nb.w, nc.x, nc.y, nc.z, nc.w, tmin, res.t)
-                                                 : __builtin_inff();
-            if (STATS) *bounds_tests += 2;
-            const bool hl = tl != __builtin_inff();
-            const bool hr = tr != __builtin_inff();
-            if (hl && hr) {
-                const bool ln = tl <= tr;
-                stack_push(stk, sp, ln ? nd.y : nd.x, ln ? tr : tl);
-                ref = ln ? nd.x : nd.y;
-                continue;
-            }
-            if (hl) {
-                ref = nd.x;
-                continue;
-            }
-            if (hr) {
-                ref = nd.y;
+            const float4  lx = np[0], ly = np[1], lz = np[2], hx = np[3], hy = np[4], hz = np[5];
+            const uint4   rf = reinterpret_cast<const uint4*>(np)[6];
+            float    t0 = box_enter(r, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, tmin, res.t);
+            float    t1 = box_enter(r, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, tmin, res.t);
+            float    t2 = box_enter(r, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, tmin, res.t);
+            float    t3 = box_enter(r, lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, tmin, res.t);
+            uint32_t r0 = rf.x, r1 = rf.y, r2 = rf.z, r3 = rf.w;
+            if (r1 == kEmptyRef) t1 = __builtin_inff();
+            if (r2 == kEmptyRef) t2 = __builtin_inff();
+            if (r3 == kEmptyRef) t3 = __builtin_inff();
+            if (STATS) *bounds_tests += 4;
+            // sorting network on (t, ref): nearest first, misses (+inf) last
+            sort2(t0, r0, t1, r1);
+            sort2(t2, r2, t3, r3);
+            sort2(t0, r0, t2, r2);
+            sort2(t1, r1, t3, r3);
+            sort2(t1, r1, t2, r2);
+            if (t0 != __builtin_inff()) {
+                if (t3 != __builtin_inff()) stack_push(stk, sp, r3, t3);
+                if (t2 != __builtin_inff()) stack_push(stk, sp, r2, t2);
+                if (t1 != __builtin_inff()) stack_push(stk, sp, r1, t1);
+                ref = r0;
                 continue;
             }
         } else {
