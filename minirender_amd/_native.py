@@ -5,7 +5,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libmrg.so")
+LIB_PATH = os.environ.get("MRG_LIBRARY") or os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libmrg.so")
 
 MRG_OK, MRG_INVALID_ARGUMENT, MRG_CANCELLED, MRG_NOT_ADMITTED, MRG_NONFINITE, MRG_DEVICE_ERROR, MRG_TIMEOUT = range(7)
 

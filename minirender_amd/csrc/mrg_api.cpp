@@ -4,7 +4,7 @@
 // Reference interfaces replaced (see include/mrg.h): SceneBvh (src/render/bvh.cpp:31-99, 372-384),
 // LightSampler construction (src/render/light.cpp:197-264), PathTracer::render / sample_pixel
 // (src/render/path_tracer.cpp:368-405) and RenderSession (src/render/render_session.cpp:200-656).
-#include "../../include/mrg.h"
+#include "mrg.h"
 
 #include <hip/hip_runtime.h>
 

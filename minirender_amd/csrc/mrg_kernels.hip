@@ -57,6 +57,40 @@ __device__ __forceinline__ StackView make_stack(uint32_t* lds_ref, float* lds_t,
     return s;
 }
 
+// Persistent traversal with dynamic refill (used for the incoherent ray-query batches): every lane
+// keeps its own ray; whenever at least kRefillLanes lanes of the wave are idle, they reserve the next
+// entries with one atomic per wave and start new rays, so a wave does not wait for its slowest ray.
+// (The path tracer's extension/shadow kernels keep the tight per-ray loop: measured faster there.)
+// `start(i, st, scene)` initialises item i; `finish(i, st)` consumes its result.
+constexpr int kRefillLanes = 32;
+
+template <bool ANY, bool STATS, class Start, class Finish>
+__device__ __forceinline__ void persistent_trace(const StackView& stk, uint32_t n, uint32_t* work, Start start, Finish finish,
+                                                 uint64_t& nb, uint64_t& nt) {
+    TraceState st;
+    TraceScene ts{nullptr, nullptr, 0};
+    bool       active  = false;
+    bool       drained = false;  // wave-uniform: the queue has no entries left
+    uint32_t   item    = 0;
+    for (;;) {
+        const uint64_t idle = __ballot(!active);
+        if (!drained && __builtin_popcountll(idle) >= kRefillLanes) {
+            const uint32_t slot = wave_reserve(work, idle);
+            drained             = __ballot(!active && slot >= n) != 0;
+            if (!active && slot < n) {
+                item   = slot;
+                ts     = start(item, st);
+                active = true;
+            }
+        }
+        if (__ballot(active) == 0) break;
+        if (active && trace_step<ANY, STATS>(ts, st, stk, nb, nt)) {
+            finish(item, st);
+            active = false;
+        }
+    }
+}
+
 // Reported distance of the winning hit: the ray/plane solve of that one triangle in FP64
 // (t = n.(v0 - o) / n.d with n = (v1 - v0) x (v2 - v0)), rounded to float. The hit decision stays
 // with the FP32 watertight test; this only removes the FP32 cancellation error of t for hits close
@@ -84,21 +118,21 @@ __global__ void __launch_bounds__(256) k_query(GpuScene sc, const float* __restr
     __shared__ float    s_t[kStackLds * 256];
     const StackView stk = make_stack(s_ref, s_t, P);
     uint64_t        nb = 0, nt = 0;
-    for (;;) {
-        uint32_t base = 0;
-        if (lane_id() == 0) base = atomicAdd(work, 64u);
-        base = __shfl(base, 0, 64);
-        if (base >= n) break;
-        const uint32_t i = base + lane_id();
-        if (i < n) {
-            const float*  r  = rays + static_cast<size_t>(i) * 6;
-            const RayPre  rp = ray_setup(mk(r[0], r[1], r[2]), mk(r[3], r[4], r[5]));
-            const TraceResult res = trace_ray<ANY, STATS>(sc, rp, tmin, tmax, stk, &nb, &nt);
+    const TraceScene ts = trace_scene(sc);
+    persistent_trace<ANY, STATS>(
+        stk, n, work,
+        [&](uint32_t i, TraceState& st) {
+            const float* r = rays + static_cast<size_t>(i) * 6;
+            trace_init(st, mk(r[0], r[1], r[2]), mk(r[3], r[4], r[5]), tmin, tmax);
+            return ts;
+        },
+        [&](uint32_t i, const TraceState& st) {
+            const TraceResult& res = st.res;
             if (ANY) {
                 occluded[i] = res.slot != 0xffffffffu ? 1 : 0;
             } else {
                 mrg_hit_raw h;
-                h.t         = res.slot != 0xffffffffu ? refine_distance(sc, res.slot, rp, res.t) : __builtin_inff();
+                h.t         = res.slot != 0xffffffffu ? refine_distance(sc, res.slot, st.r, res.t) : __builtin_inff();
                 h.primitive = res.slot != 0xffffffffu ? static_cast<int32_t>(res.prim) : -1;
                 hits[i]     = h;
                 if (bary) {
@@ -107,8 +141,8 @@ __global__ void __launch_bounds__(256) k_query(GpuScene sc, const float* __restr
                     bary[static_cast<size_t>(i) * 3 + 2] = res.b2;
                 }
             }
-        }
-    }
+        },
+        nb, nt);
     if (STATS) {
         atomicAdd(reinterpret_cast<unsigned long long*>(stats), static_cast<unsigned long long>(nb));
         atomicAdd(reinterpret_cast<unsigned long long*>(stats + 1), static_cast<unsigned long long>(nt));

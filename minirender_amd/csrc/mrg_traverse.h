@@ -172,6 +172,110 @@ struct TraceResult {
     uint32_t prim;  // scene-order primitive index
 };
 
+// The pointers traversal needs from a scene.
+struct TraceScene {
+    const GpuNode4* nodes;
+    const GpuTri*   tris;
+    uint32_t        node_count;
+};
+
+__device__ __forceinline__ TraceScene trace_scene(const GpuScene& sc) { return TraceScene{sc.nodes, sc.tris, sc.node_count}; }
+
+// Per-lane traversal state: the ray, the best hit so far, the current node and the stack depth
+// (the stack itself lives in LDS / the spill area of this lane).
+struct TraceState {
+    RayPre      r;
+    float       tmin;
+    TraceResult res;
+    uint32_t    ref;
+    uint32_t    sp;
+};
+
+__device__ __forceinline__ void trace_init(TraceState& st, F3 o, F3 d, float tmin, float tmax) {
+    st.r        = ray_setup(o, d);
+    st.tmin     = tmin;
+    st.res.t    = tmax;
+    st.res.slot = 0xffffffffu;
+    st.res.prim = 0xffffffffu;
+    st.res.b0 = st.res.b1 = st.res.b2 = 0.0f;
+    st.ref = 0;
+    st.sp  = 0;
+}
+
+// One traversal step: an inner node (its four boxes) or a leaf (its triangles), then a pop if the
+// lane has no child to descend into. Returns true once the ray is finished: closest hit (ANY =
+// false) or any hit (ANY = true) in [tmin, tmax], result in st.res.
+template <bool ANY, bool STATS>
+__device__ __forceinline__ bool trace_step(const TraceScene& sc, TraceState& st, const StackView& stk, uint64_t& bounds_tests,
+                                           uint64_t& tri_tests) {
+    if (sc.node_count == 0) return true;
+    const RayPre& r = st.r;
+    if (!is_leaf(st.ref)) {
+        const float4* np = reinterpret_cast<const float4*>(sc.nodes + st.ref);
+        const float4  lx = np[0], ly = np[1], lz = np[2], hx = np[3], hy = np[4], hz = np[5];
+        const uint4   rf = reinterpret_cast<const uint4*>(np)[6];
+        float    t0 = box_enter(r, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, st.tmin, st.res.t);
+        float    t1 = box_enter(r, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, st.tmin, st.res.t);
+        float    t2 = box_enter(r, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, st.tmin, st.res.t);
+        float    t3 = box_enter(r, lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, st.tmin, st.res.t);
+        uint32_t r0 = rf.x, r1 = rf.y, r2 = rf.z, r3 = rf.w;
+        if (r1 == kEmptyRef) t1 = __builtin_inff();
+        if (r2 == kEmptyRef) t2 = __builtin_inff();
+        if (r3 == kEmptyRef) t3 = __builtin_inff();
+        if (STATS) bounds_tests += 4;
+        // sorting network on (t, ref): nearest first, misses (+inf) last
+        sort2(t0, r0, t1, r1);
+        sort2(t2, r2, t3, r3);
+        sort2(t0, r0, t2, r2);
+        sort2(t1, r1, t3, r3);
+        sort2(t1, r1, t2, r2);
+        if (t0 != __builtin_inff()) {
+            if (t3 != __builtin_inff()) stack_push(stk, st.sp, r3, t3);
+            if (t2 != __builtin_inff()) stack_push(stk, st.sp, r2, t2);
+            if (t1 != __builtin_inff()) stack_push(stk, st.sp, r1, t1);
+            st.ref = r0;
+            return false;
+        }
+    } else {
+        const uint32_t first = leaf_first(st.ref), count = leaf_count(st.ref);
+        for (uint32_t k = 0; k < count; ++k) {
+            const uint32_t slot = first + k;
+            const float4*  tp   = reinterpret_cast<const float4*>(sc.tris + slot);
+            const float4   a    = tp[0];
+            const float4   b    = tp[1];
+            const float4   c    = tp[2];
+            if (STATS) tri_tests += 1;
+            TriHit h;
+            const F3 v0 = mk(a.x, a.y, a.z), v1 = mk(b.x, b.y, b.z), v2 = mk(c.x, c.y, c.z);
+            if (!tri_test(r, v0, v1, v2, st.tmin, st.res.t, h)) continue;
+            const uint32_t flags = __float_as_uint(b.w);
+            if (flags & kTriSingleSided) {
+                const F3 e1      = v1 - v0;
+                const F3 e2      = v2 - v0;
+                F3       outward = cross(e1, e2) * ((flags & kTriMirrored) ? -1.0f : 1.0f);
+                if (dot(r.d, outward) >= 0.0f) continue;
+            }
+            const uint32_t prim = __float_as_uint(a.w);
+            if (st.res.slot == 0xffffffffu || h.t < st.res.t || (h.t == st.res.t && prim < st.res.prim)) {
+                st.res.t    = h.t;
+                st.res.b0   = h.b0;
+                st.res.b1   = h.b1;
+                st.res.b2   = h.b2;
+                st.res.slot = slot;
+                st.res.prim = prim;
+                if (ANY) return true;
+            }
+        }
+    }
+    // pop, skipping entries that start beyond the current closest hit
+    for (;;) {
+        if (st.sp == 0) return true;
+        float t;
+        stack_pop(stk, st.sp, st.ref, t);
+        if (!(t > st.res.t)) return false;
+    }
+}
+
 // Closest hit (ANY = false) or any hit (ANY = true) in [tmin, tmax].
 template <bool ANY, bool STATS>
 __device__ __forceinline__ TraceResult trace_ray(const GpuScene& sc, const RayPre& r, float tmin, float tmax,
