@@ -27,9 +27,9 @@ HBM_PEAK_GBS = 8000.0
 B_BOX, B_TRI, B_RAY_IN, B_HIT_OUT, B_FILM = 24, 36, 24, 8, 12
 
 WORKLOADS = {
-    # configs[2]: 2M-triangle fBm terrain + 64 spheres, sun + sky, thin lens f/2.8
+    # 2M-triangle fBm terrain + 64 spheres, sun + sky, thin lens f/2.8
     "mixed": dict(width=1920, height=1080, spp=256, max_depth=16, seed=0, cpu_rows=(516, 564)),
-    # configs[0]: Cornell box 512^2, 64 spp, depth 8
+    # Cornell box 512^2, 64 spp, depth 8
     "cornell": dict(width=512, height=512, spp=64, max_depth=8, seed=0, cpu_rows=(192, 320)),
 }
 
@@ -168,7 +168,7 @@ def bench_render(args):
 
 
 def bench_serving():
-    """Poisson request stream (40 jobs/s, 64 uneven jobs over 8 resident scenes, 64 GB HBM
+    """Served request stream: Poisson arrivals (40 jobs/s, 64 uneven jobs over 8 resident scenes, 64 GB HBM
     budget, 16-spp progressive passes, 5% cancelled) replayed on the plain and the batched schedule."""
     from minirender_amd import serve
     log("[bench] serving: building the 8 resident scenes and the request stream")
@@ -187,7 +187,7 @@ def bench_serving():
 
 
 def bench_intersect(args):
-    """16,777,216 seeded rays against the 1,310,720-triangle displaced icosphere, closest
+    """Intersection microbench: 16,777,216 seeded rays against the 1,310,720-triangle displaced icosphere, closest
     hit, rays and hits resident in HBM. Parity and per-ray algorithmic bytes come from the FP64 oracle
     on every 256th ray of the same set."""
     import ctypes
@@ -260,7 +260,7 @@ def bench_intersect(args):
 
 def bench_convergence(args):
     """RMSE of 64-spp Cornell renders (GPU FP32, and the FP64 oracle) against an FP64 render at
-    65,536 spp of a 32x32 crop (configs[0] camera, depth 8, RR 3, seed 0)."""
+    65,536 spp of a 32x32 crop (Cornell 512^2 camera, depth 8, RR 3, seed 0)."""
     from minirender_amd import api, scenegen
     from oracle import oracle as O
     w = WORKLOADS["cornell"]
@@ -290,12 +290,29 @@ def bench_convergence(args):
     return out
 
 
+def bench_copy(args):
+    """Plain 1 GiB device-to-device copy: the known-size kernel that scales rocprofv3's FETCH_SIZE and
+    WRITE_SIZE readings (KiB per dispatch) into bytes for the traffic figures of the path kernels."""
+    import torch
+    n = 1 << 28
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(max(args.steps, 1)):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / max(args.steps, 1)
+    return {"metric": "device copy GB/s (read + write)", "value": round(2 * a.numel() * 4 / dt / 1e9, 1), "unit": "GB/s",
+            "bytes_read_per_dispatch": a.numel() * 4, "bytes_written_per_dispatch": a.numel() * 4}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="mixed", choices=["mixed", "cornell", "intersect", "convergence"])
+    ap.add_argument("--workload", default="mixed", choices=["mixed", "cornell", "intersect", "convergence", "copy"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--blocks-per-cu", type=int, default=0)
@@ -304,11 +321,13 @@ def main():
     if args.spp:
         WORKLOADS[args.workload]["spp"] = args.spp
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build runs the path on a single MI355X (see DESIGN.md for how "
-              "it would shard); use --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only: this build runs the path on a single MI355X; samples "
+              "shard by pass across GPUs with one film reduce, which this build does not run; use --gpus 1", file=sys.stderr)
         sys.exit(2)
     if args.workload == "intersect":
         res = bench_intersect(args)
+    elif args.workload == "copy":
+        res = bench_copy(args)
     elif args.workload == "convergence":
         res = bench_convergence(args)
     else:

@@ -97,7 +97,7 @@ __device__ __forceinline__ void persistent_trace(const StackView& stk, uint32_t 
 // to the ray origin. Falls back to the FP32 t for a ray parallel to the plane.
 __device__ __forceinline__ float refine_distance(const GpuScene& sc, uint32_t slot, const RayPre& r, float t32) {
     const float4* tp = reinterpret_cast<const float4*>(sc.tris + slot);
-    const float4  a = tp[0], b = tp[1], c = tp[2];
+    const float4  a = gld(tp + 0), b = gld(tp + 1), c = gld(tp + 2);
     const double  e1x = double(b.x) - a.x, e1y = double(b.y) - a.y, e1z = double(b.z) - a.z;
     const double  e2x = double(c.x) - a.x, e2y = double(c.y) - a.y, e2z = double(c.z) - a.z;
     const double  nx = e1y * e2z - e2y * e1z, ny = e1z * e2x - e2z * e1x, nz = e1x * e2y - e2x * e1y;
@@ -303,12 +303,12 @@ __global__ void __launch_bounds__(256) k_shade(Pool P, const GpuJob* __restrict_
                 }
                 // surface_hit (bvh.cpp:302-370)
                 const float4* tp = reinterpret_cast<const float4*>(sc.tris + slot);
-                const float4  ta = tp[0], tb = tp[1], tc = tp[2];
+                const float4  ta = gld(tp + 0), tb = gld(tp + 1), tc = gld(tp + 2);
                 const F3      v0 = mk(ta.x, ta.y, ta.z), v1 = mk(tb.x, tb.y, tb.z), v2 = mk(tc.x, tc.y, tc.z);
                 const uint32_t tflags = __float_as_uint(tb.w);
                 const float4* np = reinterpret_cast<const float4*>(sc.normals + slot);
-                const float4  na = np[0], nb = np[1], nc = np[2];
-                const uint32_t inst = sc.slot_instance[slot];
+                const float4  na = gld(np + 0), nb = gld(np + 1), nc = gld(np + 2);
+                const uint32_t inst = gld(sc.slot_instance + slot);
                 const GpuMaterial& mat = sc.materials[inst];
                 F3 geometric = normalize(cross(v1 - v0, v2 - v0));
                 if (tflags & kTriMirrored) geometric = -geometric;

@@ -44,6 +44,26 @@ __device__ __forceinline__ bool finite3(F3 v) {
     return __builtin_isfinite(v.x) && __builtin_isfinite(v.y) && __builtin_isfinite(v.z);
 }
 __device__ __forceinline__ F3 ld3(const float* p) { return F3{p[0], p[1], p[2]}; }
+
+// Load through the global address space. Scene pointers reach the kernels inside structures held in
+// device memory, so the compiler cannot infer their address space and would emit flat loads, which
+// also count against the LDS wait counter and serialise with the LDS traversal stack.
+// (The vector forms load a clang ext-vector: a HIP vector class would be copied through a generic
+// reference, which gives the flat load back.)
+typedef float    gf4 __attribute__((ext_vector_type(4)));
+typedef uint32_t gu4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 gld(const float4* p) {
+    const gf4 v = *(const __attribute__((address_space(1))) gf4*)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ uint4 gld(const uint4* p) {
+    const gu4 v = *(const __attribute__((address_space(1))) gu4*)p;
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ float gld(const float* p) { return *(const __attribute__((address_space(1))) float*)p; }
+__device__ __forceinline__ uint32_t gld(const uint32_t* p) { return *(const __attribute__((address_space(1))) uint32_t*)p; }
+__device__ __forceinline__ void gst(float* p, float v) { *(__attribute__((address_space(1))) float*)p = v; }
+__device__ __forceinline__ void gst(uint32_t* p, uint32_t v) { *(__attribute__((address_space(1))) uint32_t*)p = v; }
 __device__ __forceinline__ float comp(F3 v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
 
 constexpr float kPiF     = 3.14159274101257324219f;  // float(pi)

@@ -59,9 +59,9 @@ __device__ __forceinline__ float box_enter(const RayPre& r, float lx, float ly, 
                                __builtin_fmaxf(__builtin_fminf(az, bz), tmin));
     float tf = __builtin_fminf(__builtin_fminf(__builtin_fmaxf(ax, bx), __builtin_fmaxf(ay, by)),
                                __builtin_fminf(__builtin_fmaxf(az, bz), tmax));
-    constexpr float lo = 1.0f - 0x1p-20f, hi = 1.0f + 0x1p-20f;
-    tn *= tn > 0.0f ? lo : hi;
-    tf *= tf > 0.0f ? hi : lo;
+    // widen by 2^-20 of the magnitude: one FMA with an |x| source modifier per side
+    tn = __builtin_fmaf(-__builtin_fabsf(tn), 0x1p-20f, tn);
+    tf = __builtin_fmaf(__builtin_fabsf(tf), 0x1p-20f, tf);
     return tn <= tf ? tn : __builtin_inff();
 }
 
@@ -71,8 +71,10 @@ __device__ __forceinline__ float dop(float a, float b, float c, float d) {
     return __builtin_fmaf(a, b, -cd) + corr;
 }
 
+// A candidate hit: distance and the three edge functions with their sum (the barycentrics are
+// a / det, b / det, c / det; the divisions wait until the ray's closest hit is known).
 struct TriHit {
-    float t, b0, b1, b2;
+    float t, a, b, c, det;
 };
 
 __device__ __forceinline__ void permute(const RayPre& r, F3 v, float& px, float& py, float& pz) {
@@ -109,10 +111,11 @@ __device__ __forceinline__ bool tri_test(const RayPre& r, F3 v0, F3 v1, F3 v2, f
     const float edge = max3(__builtin_fabsf(a), __builtin_fabsf(b), __builtin_fabsf(c));
     const float et   = 3.0f * ((g3 * edge * z + ee * z) + g3 * z * edge) / __builtin_fabsf(det);
     if (t <= et) return false;
-    out.t  = t;
-    out.b0 = a / det;
-    out.b1 = b / det;
-    out.b2 = c / det;
+    out.t   = t;
+    out.a   = a;
+    out.b   = b;
+    out.c   = c;
+    out.det = det;
     return true;
 }
 
@@ -138,8 +141,8 @@ __device__ __forceinline__ void stack_push(const StackView& s, uint32_t& sp, uin
         s.lds_t[sp * s.block + s.lane]   = t;
     } else {
         const uint32_t k = (sp - kStackLds) * s.lanes + s.glane;
-        s.spill_ref[k]   = ref;
-        s.spill_t[k]     = t;
+        gst(s.spill_ref + k, ref);
+        gst(s.spill_t + k, t);
     }
     ++sp;
 }
@@ -151,8 +154,8 @@ __device__ __forceinline__ void stack_pop(const StackView& s, uint32_t& sp, uint
         t   = s.lds_t[sp * s.block + s.lane];
     } else {
         const uint32_t k = (sp - kStackLds) * s.lanes + s.glane;
-        ref              = s.spill_ref[k];
-        t                = s.spill_t[k];
+        ref              = gld(s.spill_ref + k);
+        t                = gld(s.spill_t + k);
     }
 }
 
@@ -167,10 +170,18 @@ __device__ __forceinline__ void sort2(float& ta, uint32_t& ra, float& tb, uint32
 }
 
 struct TraceResult {
-    float    t, b0, b1, b2;
-    uint32_t slot;  // leaf-order triangle slot, 0xffffffff on a miss
-    uint32_t prim;  // scene-order primitive index
+    float    t, b0, b1, b2;  // during traversal b0..b2 hold the edge functions of the best hit
+    float    det;            // their sum; finish_barycentrics divides once at the end
+    uint32_t slot;           // leaf-order triangle slot, 0xffffffff on a miss
+    uint32_t prim;           // scene-order primitive index
 };
+
+__device__ __forceinline__ void finish_barycentrics(TraceResult& res) {
+    if (res.slot == 0xffffffffu) return;
+    res.b0 = res.b0 / res.det;
+    res.b1 = res.b1 / res.det;
+    res.b2 = res.b2 / res.det;
+}
 
 // The pointers traversal needs from a scene.
 struct TraceScene {
@@ -198,6 +209,7 @@ __device__ __forceinline__ void trace_init(TraceState& st, F3 o, F3 d, float tmi
     st.res.slot = 0xffffffffu;
     st.res.prim = 0xffffffffu;
     st.res.b0 = st.res.b1 = st.res.b2 = 0.0f;
+    st.res.det = 1.0f;
     st.ref = 0;
     st.sp  = 0;
 }
@@ -212,8 +224,8 @@ __device__ __forceinline__ bool trace_step(const TraceScene& sc, TraceState& st,
     const RayPre& r = st.r;
     if (!is_leaf(st.ref)) {
         const float4* np = reinterpret_cast<const float4*>(sc.nodes + st.ref);
-        const float4  lx = np[0], ly = np[1], lz = np[2], hx = np[3], hy = np[4], hz = np[5];
-        const uint4   rf = reinterpret_cast<const uint4*>(np)[6];
+        const float4  lx = gld(np + 0), ly = gld(np + 1), lz = gld(np + 2), hx = gld(np + 3), hy = gld(np + 4), hz = gld(np + 5);
+        const uint4   rf = gld(reinterpret_cast<const uint4*>(np) + 6);
         float    t0 = box_enter(r, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, st.tmin, st.res.t);
         float    t1 = box_enter(r, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, st.tmin, st.res.t);
         float    t2 = box_enter(r, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, st.tmin, st.res.t);
@@ -241,9 +253,9 @@ __device__ __forceinline__ bool trace_step(const TraceScene& sc, TraceState& st,
         for (uint32_t k = 0; k < count; ++k) {
             const uint32_t slot = first + k;
             const float4*  tp   = reinterpret_cast<const float4*>(sc.tris + slot);
-            const float4   a    = tp[0];
-            const float4   b    = tp[1];
-            const float4   c    = tp[2];
+            const float4   a    = gld(tp + 0);
+            const float4   b    = gld(tp + 1);
+            const float4   c    = gld(tp + 2);
             if (STATS) tri_tests += 1;
             TriHit h;
             const F3 v0 = mk(a.x, a.y, a.z), v1 = mk(b.x, b.y, b.z), v2 = mk(c.x, c.y, c.z);
@@ -258,18 +270,25 @@ __device__ __forceinline__ bool trace_step(const TraceScene& sc, TraceState& st,
             const uint32_t prim = __float_as_uint(a.w);
             if (st.res.slot == 0xffffffffu || h.t < st.res.t || (h.t == st.res.t && prim < st.res.prim)) {
                 st.res.t    = h.t;
-                st.res.b0   = h.b0;
-                st.res.b1   = h.b1;
-                st.res.b2   = h.b2;
+                st.res.b0   = h.a;
+                st.res.b1   = h.b;
+                st.res.b2   = h.c;
+                st.res.det  = h.det;
                 st.res.slot = slot;
                 st.res.prim = prim;
-                if (ANY) return true;
+                if (ANY) {
+                    finish_barycentrics(st.res);
+                    return true;
+                }
             }
         }
     }
     // pop, skipping entries that start beyond the current closest hit
     for (;;) {
-        if (st.sp == 0) return true;
+        if (st.sp == 0) {
+            finish_barycentrics(st.res);
+            return true;
+        }
         float t;
         stack_pop(stk, st.sp, st.ref, t);
         if (!(t > st.res.t)) return false;
@@ -285,14 +304,15 @@ __device__ __forceinline__ TraceResult trace_ray(const GpuScene& sc, const RayPr
     res.slot = 0xffffffffu;
     res.prim = 0xffffffffu;
     res.b0 = res.b1 = res.b2 = 0.0f;
+    res.det = 1.0f;
     if (sc.node_count == 0) return res;
     uint32_t sp  = 0;
     uint32_t ref = 0;
     for (;;) {
         if (!is_leaf(ref)) {
             const float4* np = reinterpret_cast<const float4*>(sc.nodes + ref);
-            const float4  lx = np[0], ly = np[1], lz = np[2], hx = np[3], hy = np[4], hz = np[5];
-            const uint4   rf = reinterpret_cast<const uint4*>(np)[6];
+            const float4  lx = gld(np + 0), ly = gld(np + 1), lz = gld(np + 2), hx = gld(np + 3), hy = gld(np + 4), hz = gld(np + 5);
+            const uint4   rf = gld(reinterpret_cast<const uint4*>(np) + 6);
             float    t0 = box_enter(r, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, tmin, res.t);
             float    t1 = box_enter(r, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, tmin, res.t);
             float    t2 = box_enter(r, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, tmin, res.t);
@@ -320,9 +340,9 @@ __device__ __forceinline__ TraceResult trace_ray(const GpuScene& sc, const RayPr
             for (uint32_t k = 0; k < count; ++k) {
                 const uint32_t slot = first + k;
                 const float4*  tp   = reinterpret_cast<const float4*>(sc.tris + slot);
-                const float4   a    = tp[0];
-                const float4   b    = tp[1];
-                const float4   c    = tp[2];
+                const float4   a    = gld(tp + 0);
+                const float4   b    = gld(tp + 1);
+                const float4   c    = gld(tp + 2);
                 if (STATS) *tri_tests += 1;
                 TriHit h;
                 const F3 v0 = mk(a.x, a.y, a.z), v1 = mk(b.x, b.y, b.z), v2 = mk(c.x, c.y, c.z);
@@ -337,18 +357,25 @@ __device__ __forceinline__ TraceResult trace_ray(const GpuScene& sc, const RayPr
                 const uint32_t prim = __float_as_uint(a.w);
                 if (res.slot == 0xffffffffu || h.t < res.t || (h.t == res.t && prim < res.prim)) {
                     res.t    = h.t;
-                    res.b0   = h.b0;
-                    res.b1   = h.b1;
-                    res.b2   = h.b2;
+                    res.b0   = h.a;
+                    res.b1   = h.b;
+                    res.b2   = h.c;
+                    res.det  = h.det;
                     res.slot = slot;
                     res.prim = prim;
-                    if (ANY) return res;
+                    if (ANY) {
+                        finish_barycentrics(res);
+                        return res;
+                    }
                 }
             }
         }
         // pop, skipping entries that start beyond the current closest hit
         for (;;) {
-            if (sp == 0) return res;
+            if (sp == 0) {
+                finish_barycentrics(res);
+                return res;
+            }
             float t;
             stack_pop(stk, sp, ref, t);
             if (!(t > res.t)) break;
