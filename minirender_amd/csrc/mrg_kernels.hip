@@ -44,6 +44,31 @@ __device__ __forceinline__ void wave_add64(uint64_t* counter, uint32_t value) {
     if (lane_id() == 0 && v) atomicAdd(reinterpret_cast<unsigned long long*>(counter), static_cast<unsigned long long>(v));
 }
 
+// Work distribution of the persistent traversal kernels: wave w first takes items [64 w, 64 w + 64)
+// without any atomic, then claims further 64-item chunks from `work` (counted from the end of the
+// static range). A queue that the static range covers (the late bounces) needs no atomic at all;
+// otherwise every wave pays one atomic per chunk (and one that finds the queue empty).
+struct WaveWork {
+    uint32_t base;
+    uint32_t static_end;
+};
+
+__device__ __forceinline__ WaveWork wave_work_begin() {
+    const uint32_t wave  = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
+    return WaveWork{wave * 64u, waves * 64u};
+}
+
+__device__ __forceinline__ void wave_work_next(WaveWork& w, uint32_t* work, uint32_t n) {
+    if (w.static_end >= n) {
+        w.base = n;
+        return;
+    }
+    uint32_t claimed = 0;
+    if (lane_id() == 0) claimed = atomicAdd(work, 64u);
+    w.base = w.static_end + __shfl(claimed, 0, 64);
+}
+
 __device__ __forceinline__ StackView make_stack(uint32_t* lds_ref, float* lds_t, const Pool& P) {
     StackView s;
     s.lds_ref   = lds_ref;
@@ -231,12 +256,8 @@ __global__ void __launch_bounds__(256) k_extend(Pool P, const GpuJob* __restrict
     const uint32_t* queue = (iter & 1u) ? P.q_ext1 : P.q_ext0;
     uint32_t*       work  = P.counters + kCtrWorkExt + iter;
     uint64_t        dummy = 0;
-    for (;;) {
-        uint32_t base = 0;
-        if (lane_id() == 0) base = atomicAdd(work, 64u);
-        base = __shfl(base, 0, 64);
-        if (base >= n) break;
-        const uint32_t i = base + lane_id();
+    for (WaveWork w = wave_work_begin(); w.base < n; wave_work_next(w, work, n)) {
+        const uint32_t i = w.base + lane_id();
         if (i < n) {
             const uint32_t path = queue[i];
             const float4   ro   = P.ray_o[path];
@@ -456,12 +477,8 @@ __global__ void __launch_bounds__(256) k_shadow(Pool P, const GpuJob* __restrict
     const uint32_t  n    = P.counters[kCtrShadow + iter];
     uint32_t*       work = P.counters + kCtrWorkShadow + iter;
     uint64_t        dummy = 0;
-    for (;;) {
-        uint32_t base = 0;
-        if (lane_id() == 0) base = atomicAdd(work, 64u);
-        base = __shfl(base, 0, 64);
-        if (base >= n) break;
-        const uint32_t i = base + lane_id();
+    for (WaveWork w = wave_work_begin(); w.base < n; wave_work_next(w, work, n)) {
+        const uint32_t i = w.base + lane_id();
         if (i < n) {
             const uint32_t path = P.q_sh[i];
             const float4   o    = P.sh_o[i];
