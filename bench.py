@@ -24,7 +24,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-B_BOX, B_TRI, B_RAY_IN, B_HIT_OUT, B_FILM = 24, 36, 24, 8, 12
+B_BOX, B_TRI, B_RAY_IN, B_HIT_OUT, B_OCCLUDED_OUT, B_FILM = 24, 36, 24, 8, 4, 12
 
 WORKLOADS = {
     # 2M-triangle fBm terrain + 64 spheres, sun + sky, thin lens f/2.8
@@ -77,7 +77,8 @@ def bench_render(args):
     w = WORKLOADS[args.workload]
     log(f"[bench] generating workload {args.workload}")
     scene = make_scene(args.workload)
-    dev = api.Device(0, schedule="plain_fill", persistent_blocks_per_cu=args.blocks_per_cu)
+    dev = api.Device(0, schedule="plain_fill", persistent_blocks_per_cu=args.blocks_per_cu,
+                     path_capacity=args.path_capacity, traversal=args.traversal, use_graph=args.graph)
     info = dev.query()
     t0 = time.perf_counter()
     bvh = dev.upload(scene)
@@ -142,27 +143,40 @@ def bench_render(args):
     # CPU baseline + per-ray algorithmic bytes from the FP64 oracle counters
     threads = min(16, os.cpu_count() or 1)
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and args.graph:
+        log("[bench] --graph: kernel times are not measured, so no roofline entry; CPU baseline skipped")
+    elif not args.no_cpu:
         log(f"[bench] cpu baseline: FP64 oracle, {threads} threads")
         st, dt, rows = cpu_baseline(scene, w, threads)
         cpu = {"value": round(st["paths"] / dt / 1e6, 4), "unit": "Msamples/s", "cores": threads, "kind": "port",
                "sample": f"rows {rows[0]}-{rows[1] - 1} ({(rows[1] - rows[0]) * w['width']} px) at {w['spp']} spp, "
                          f"FP64 oracle, {dt:.1f}s",
                "gpu_over_cpu": round(value / (st["paths"] / dt / 1e6), 1)}
+        # algorithmic bytes per ray: ray in + result out + 24 B per box test + 36 B per triangle test,
+        # tests per ray from the FP64 oracle (reference BVH topology) on the CPU sample
         ext_bounds = (st["bounds_tests"] - st["shadow_bounds_tests"]) / st["extension_rays"]
         ext_tris = (st["triangle_tests"] - st["shadow_triangle_tests"]) / st["extension_rays"]
+        sh_bounds = st["shadow_bounds_tests"] / max(st["shadow_rays"], 1)
+        sh_tris = st["shadow_triangle_tests"] / max(st["shadow_rays"], 1)
         b_ray = B_RAY_IN + B_HIT_OUT + B_BOX * ext_bounds + B_TRI * ext_tris
+        b_shadow = B_RAY_IN + B_OCCLUDED_OUT + B_BOX * sh_bounds + B_TRI * sh_tris
+        fused = kt["shadow_launches"] == 0  # one traversal kernel per bounce traces both kinds of ray
+        kernel = "k_trace" if fused else "k_extend"
         launches = max(kt["extend_launches"], 1)
-        per_launch_bytes = kt["extension_rays"] / launches * b_ray
+        rays = kt["extension_rays"] + (kt["shadow_rays"] if fused else 0)
+        per_launch_bytes = (kt["extension_rays"] * b_ray + (kt["shadow_rays"] * b_shadow if fused else 0)) / launches
         avg_s = kt["extend_ms"] / launches / 1e3
         achieved = per_launch_bytes / avg_s / 1e9 if avg_s > 0 else None
-        traffic = read_traffic("k_extend", args.workload)
-        result["roofline"] = {"kernel": "k_extend", "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+        traffic = read_traffic(kernel, args.workload)
+        result["roofline"] = {"kernel": kernel, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                               "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                              "bytes_per_ray": round(b_ray, 1), "bounds_tests_per_ray": round(ext_bounds, 2),
-                              "triangle_tests_per_ray": round(ext_tris, 2),
-                              "avg_launch_ms": round(avg_s * 1e3, 4),
-                              "rays_per_launch": round(kt["extension_rays"] / launches, 1)}
+                              "bytes_per_extension_ray": round(b_ray, 1), "bytes_per_shadow_ray": round(b_shadow, 1),
+                              "bounds_tests_per_extension_ray": round(ext_bounds, 2),
+                              "triangle_tests_per_extension_ray": round(ext_tris, 2),
+                              "bounds_tests_per_shadow_ray": round(sh_bounds, 2),
+                              "triangle_tests_per_shadow_ray": round(sh_tris, 2),
+                              "avg_launch_ms": round(avg_s * 1e3, 4), "rays_per_launch": round(rays / launches, 1),
+                              "algorithmic_bytes_per_launch": round(per_launch_bytes)}
     result["cpu_baseline"] = cpu
     return result
 
@@ -316,6 +330,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--blocks-per-cu", type=int, default=0)
+    ap.add_argument("--path-capacity", type=int, default=0, help="paths in flight per batch (0 = library default)")
+    ap.add_argument("--traversal", default="auto", choices=["auto", "split", "fused"])
+    ap.add_argument("--graph", action="store_true", help="capture the bounce loop in a hipGraph (no per-kernel times)")
     ap.add_argument("--spp", type=int, default=0, help="override samples per pixel (calibration runs only)")
     args = ap.parse_args()
     if args.spp:

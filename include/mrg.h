@@ -60,6 +60,9 @@ enum { MRG_MODE_BSDF = 0, MRG_MODE_LIGHT = 1, MRG_MODE_MIS = 2 };
  *   PLAIN_FILL  one job at a time, each batch filled with as many samples of that job as fit.
  */
 enum { MRG_SCHEDULE_PLAIN = 0, MRG_SCHEDULE_BATCHED = 1, MRG_SCHEDULE_PLAIN_FILL = 2 };
+/* Traversal kernels of a bounce: split = one closest-hit and one any-hit launch per bounce; fused =
+ * one launch tracing the previous bounce's shadow rays together with this bounce's extension rays. */
+enum { MRG_TRAVERSAL_AUTO = 0, MRG_TRAVERSAL_SPLIT = 1, MRG_TRAVERSAL_FUSED = 2 };
 
 typedef struct mrg_device mrg_device;
 typedef struct mrg_scene  mrg_scene;
@@ -159,13 +162,13 @@ typedef struct mrg_progress {
 
 typedef struct mrg_device_options {
     uint64_t hbm_budget_bytes;   /* admission budget for scenes + films + path pools; 0 = 64 GiB */
-    uint64_t path_capacity;      /* paths in flight per batch; 0 = automatic */
-    uint32_t schedule;           /* MRG_SCHEDULE_PLAIN or MRG_SCHEDULE_BATCHED */
+    uint64_t path_capacity;      /* paths in flight per batch; 0 = automatic (2^26 with >= 64 GB HBM) */
+    uint32_t schedule;           /* MRG_SCHEDULE_PLAIN, _BATCHED or _PLAIN_FILL */
     uint32_t max_jobs_in_flight; /* batched: jobs co-scheduled per batch; 0 = 64 */
     uint32_t use_graph;          /* capture the bounce loop in a hipGraph */
     uint32_t block_size;         /* kernel block size (64/128/256); 0 = 256 */
     uint32_t persistent_blocks_per_cu; /* 0 = automatic */
-    uint32_t reserved;
+    uint32_t traversal;          /* MRG_TRAVERSAL_*; AUTO = fused */
 } mrg_device_options;
 
 /* Hit of the intersection entry points: closest t and scene-order primitive index (-1 = miss). */
@@ -237,7 +240,9 @@ int mrg_job_release(mrg_job* job);
 /* Synchronous render of one job (plain path): submit + wait + read image. */
 int mrg_render(mrg_device* device, mrg_scene* scene, const mrg_job_desc* desc, float* rgb, mrg_progress* progress);
 
-/* Device-wide counters of the last completed batch: kernel milliseconds by kernel class. */
+/* Device-wide kernel milliseconds by kernel class (accumulated since the last reset; measured only
+ * when the bounce loop is not graph-captured). With fused traversal, extend_ms and extend_launches
+ * cover the fused launches (shadow rays included) and shadow_ms / shadow_launches stay 0. */
 typedef struct mrg_kernel_times {
     double generate_ms;
     double extend_ms;

@@ -52,6 +52,7 @@ __device__ __forceinline__ void wave_add64(uint64_t* counter, uint32_t value) {
 // tail of the more divergent shadow rays better.
 constexpr uint32_t kExtendClaim = 128;  // items per claim, extension rays (a multiple of 64)
 constexpr uint32_t kShadowClaim = 64;   // items per claim, shadow rays
+constexpr uint32_t kTraceClaim  = 128;  // items per claim, fused traversal
 
 template <uint32_t CLAIM>
 struct WaveWork {
@@ -259,7 +260,40 @@ __global__ void __launch_bounds__(256) k_generate(Pool P, const GpuJob* __restri
     }
 }
 
-// Closest hit for every path in the extension queue of this bounce.
+// One extension ray: closest hit of path queue[i].
+__device__ __forceinline__ void extend_one(const Pool& P, const GpuJob* __restrict__ jobs, const uint32_t* queue, uint32_t i,
+                                           const StackView& stk) {
+    uint64_t       dummy = 0;
+    const uint32_t path  = queue[i];
+    const float4   ro    = P.ray_o[path];
+    const float4   rd    = P.ray_d[path];
+    const uint32_t job   = __float_as_uint(P.prev[path].w) >> 16;
+    const RayPre   rp    = ray_setup(mk(ro.x, ro.y, ro.z), mk(rd.x, rd.y, rd.z));
+    const TraceResult res = trace_ray<false, false>(jobs[job].scene, rp, ro.w, rd.w, stk, &dummy, &dummy);
+    P.hit[path]      = make_float4(res.t, res.b0, res.b1, res.b2);
+    P.hit_slot[path] = res.slot;
+}
+
+// One shadow ray (entry i of the shadow queue): any hit; unoccluded rays add their contribution.
+__device__ __forceinline__ void shadow_one(const Pool& P, const GpuJob* __restrict__ jobs, uint32_t i, const StackView& stk) {
+    uint64_t       dummy = 0;
+    const uint32_t path  = P.q_sh[i];
+    const float4   o     = P.sh_o[i];
+    const float4   d     = P.sh_d[i];
+    const uint32_t job   = __float_as_uint(P.prev[path].w) >> 16;
+    const RayPre   rp    = ray_setup(mk(o.x, o.y, o.z), mk(d.x, d.y, d.z));
+    const TraceResult res = trace_ray<true, false>(jobs[job].scene, rp, 0.0f, o.w, stk, &dummy, &dummy);
+    if (res.slot == 0xffffffffu) {
+        const float4 c = P.sh_c[i];
+        float4       l = P.rad[path];
+        l.x = l.x + c.x;
+        l.y = l.y + c.y;
+        l.z = l.z + c.z;
+        P.rad[path] = l;
+    }
+}
+
+// Closest hit for every path in the extension queue of this bounce (split traversal kernels).
 __global__ void __launch_bounds__(256) k_extend(Pool P, const GpuJob* __restrict__ jobs, uint32_t iter) {
     __shared__ uint32_t s_ref[kStackLds * 256];
     __shared__ float    s_t[kStackLds * 256];
@@ -267,19 +301,36 @@ __global__ void __launch_bounds__(256) k_extend(Pool P, const GpuJob* __restrict
     const uint32_t  n     = P.counters[kCtrExt + iter];
     const uint32_t* queue = (iter & 1u) ? P.q_ext1 : P.q_ext0;
     uint32_t*       work  = P.counters + kCtrWorkExt + iter;
-    uint64_t        dummy = 0;
     WaveWork<kExtendClaim> w;
     for (w.begin(); w.base < n; w.next(work, n)) {
         const uint32_t i = w.base + lane_id();
-        if (i < n) {
-            const uint32_t path = queue[i];
-            const float4   ro   = P.ray_o[path];
-            const float4   rd   = P.ray_d[path];
-            const uint32_t job  = __float_as_uint(P.prev[path].w) >> 16;
-            const RayPre   rp   = ray_setup(mk(ro.x, ro.y, ro.z), mk(rd.x, rd.y, rd.z));
-            const TraceResult res = trace_ray<false, false>(jobs[job].scene, rp, ro.w, rd.w, stk, &dummy, &dummy);
-            P.hit[path]      = make_float4(res.t, res.b0, res.b1, res.b2);
-            P.hit_slot[path] = res.slot;
+        if (i < n) extend_one(P, jobs, queue, i, stk);
+    }
+}
+
+// Fused traversal of one bounce: the shadow rays of bounce iter - 1 and the extension rays of
+// bounce iter (both produced by the previous shading pass) share one persistent launch. The two
+// ranges are padded to whole 64-item chunks, so every wave traces one kind of ray at a time.
+// Radiance accumulation keeps its order: a path's shadow contribution of bounce iter - 1 lands
+// before the shading of bounce iter reads it.
+__global__ void __launch_bounds__(256) k_trace(Pool P, const GpuJob* __restrict__ jobs, uint32_t iter) {
+    __shared__ uint32_t s_ref[kStackLds * 256];
+    __shared__ float    s_t[kStackLds * 256];
+    const StackView stk    = make_stack(s_ref, s_t, P);
+    const uint32_t  n_ext  = P.counters[kCtrExt + iter];
+    const uint32_t  n_sh   = iter ? P.counters[kCtrShadow + iter - 1] : 0u;
+    const uint32_t  sh_end = (n_sh + 63u) & ~63u;  // shadow rays first: the longer, more divergent work
+    const uint32_t  n      = sh_end + n_ext;
+    const uint32_t* queue  = (iter & 1u) ? P.q_ext1 : P.q_ext0;
+    uint32_t*       work   = P.counters + kCtrWorkExt + iter;
+    WaveWork<kTraceClaim> w;
+    for (w.begin(); w.base < n; w.next(work, n)) {
+        if (w.base < sh_end) {
+            const uint32_t i = w.base + lane_id();
+            if (i < n_sh) shadow_one(P, jobs, i, stk);
+        } else {
+            const uint32_t i = w.base - sh_end + lane_id();
+            if (i < n_ext) extend_one(P, jobs, queue, i, stk);
         }
     }
 }
@@ -482,33 +533,17 @@ __global__ void __launch_bounds__(256) k_shade(Pool P, const GpuJob* __restrict_
     wave_add64(P.stats + kStatRoulette, n_rr);
 }
 
-// Any-hit for every shadow ray of this bounce; unoccluded rays add their contribution.
+// Any-hit for every shadow ray of this bounce (split traversal kernels).
 __global__ void __launch_bounds__(256) k_shadow(Pool P, const GpuJob* __restrict__ jobs, uint32_t iter) {
     __shared__ uint32_t s_ref[kStackLds * 256];
     __shared__ float    s_t[kStackLds * 256];
     const StackView stk  = make_stack(s_ref, s_t, P);
     const uint32_t  n    = P.counters[kCtrShadow + iter];
     uint32_t*       work = P.counters + kCtrWorkShadow + iter;
-    uint64_t        dummy = 0;
     WaveWork<kShadowClaim> w;
     for (w.begin(); w.base < n; w.next(work, n)) {
         const uint32_t i = w.base + lane_id();
-        if (i < n) {
-            const uint32_t path = P.q_sh[i];
-            const float4   o    = P.sh_o[i];
-            const float4   d    = P.sh_d[i];
-            const uint32_t job  = __float_as_uint(P.prev[path].w) >> 16;
-            const RayPre   rp   = ray_setup(mk(o.x, o.y, o.z), mk(d.x, d.y, d.z));
-            const TraceResult res = trace_ray<true, false>(jobs[job].scene, rp, 0.0f, o.w, stk, &dummy, &dummy);
-            if (res.slot == 0xffffffffu) {
-                const float4 c = P.sh_c[i];
-                float4       l = P.rad[path];
-                l.x = l.x + c.x;
-                l.y = l.y + c.y;
-                l.z = l.z + c.z;
-                P.rad[path] = l;
-            }
-        }
+        if (i < n) shadow_one(P, jobs, i, stk);
     }
 }
 
@@ -606,6 +641,11 @@ hipError_t launch_extend(const Pool& P, const GpuJob* jobs, uint32_t iter, const
 
 hipError_t launch_shade(const Pool& P, const GpuJob* jobs, uint32_t iter, const LaunchShape& shape, hipStream_t stream) {
     hipLaunchKernelGGL(k_shade, dim3(shape.shade_blocks), dim3(256), 0, stream, P, jobs, iter);
+    return hipGetLastError();
+}
+
+hipError_t launch_trace(const Pool& P, const GpuJob* jobs, uint32_t iter, const LaunchShape& shape, hipStream_t stream) {
+    hipLaunchKernelGGL(k_trace, dim3(shape.traversal_blocks), dim3(256), 0, stream, P, jobs, iter);
     return hipGetLastError();
 }
 
