@@ -203,7 +203,7 @@ def bench_serving():
 def bench_intersect(args):
     """Intersection microbench: 16,777,216 seeded rays against the 1,310,720-triangle displaced icosphere, closest
     hit, rays and hits resident in HBM. Parity and per-ray algorithmic bytes come from the FP64 oracle
-    on every 256th ray of the same set."""
+    on every k-th ray of the same set (--cpu-stride, default every 2nd: 8M rays, about 10 s on 16 cores)."""
     import ctypes
 
     from minirender_amd import api, scenegen
@@ -236,7 +236,7 @@ def bench_intersect(args):
         from oracle import oracle as O
         threads = min(16, os.cpu_count() or 1)
         o = O.OracleScene(scene)
-        idx = np.arange(0, n, 256)
+        idx = np.arange(0, n, args.cpu_stride)
         sample = rays[idx].astype(np.float64)
         t = time.perf_counter()
         ref = o.intersect(sample, precision=64, second=True, threads=threads)
@@ -265,7 +265,7 @@ def bench_intersect(args):
                               "bytes_per_ray": round(b_ray, 1), "bounds_tests_per_ray": round(b, 2),
                               "triangle_tests_per_ray": round(tt, 2), "avg_launch_ms": round(avg, 3)}
         result["cpu_baseline"] = {"value": round(len(idx) / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads,
-                                  "kind": "port", "sample": f"every 256th ray ({len(idx)} rays), FP64 oracle closest "
+                                  "kind": "port", "sample": f"every {args.cpu_stride}th ray ({len(idx)} rays), FP64 oracle closest "
                                   f"+ second hit, {dt:.1f}s"}
     bvh.release()
     dev.close()
@@ -304,6 +304,39 @@ def bench_convergence(args):
     return out
 
 
+def bench_batch(args):
+    """Batched uneven jobs: 64 jobs (sides log-uniform 128-1024, spp log-uniform 16-1024, depth 8-16)
+    over 8 resident scenes (36 to ~410k triangles, seed 4), all submitted at once. The autotuner
+    first picks each schedule's device options on a calibration copy of the job set (spp / 8), then
+    the full set is rendered on the plain (one job at a time), plain-fill and batched schedules."""
+    from minirender_amd import autotune, serve
+    scenes = serve.scene_pool(seed=4)
+    jobs = serve.batch_jobs(seed=4)
+    calib = serve.batch_jobs(seed=4, spp_divisor=8)
+    knobs = {"path_capacity": [1 << 23, 1 << 24, 1 << 25, 1 << 26, 1 << 27], "persistent_blocks_per_cu": [3, 4, 5],
+             "traversal": ["split", "fused"], "use_graph": [False, True], "max_jobs_in_flight": [16, 64]}
+    tuned = {}
+    for mode in ("plain", "plain_fill", "batched"):
+        t = time.perf_counter()
+        tuned[mode] = autotune.tune(scenes, calib, schedule=mode, knobs=knobs, repeats=1, log=log)
+        log(f"[bench] autotune {mode}: {tuned[mode]['options']} in {time.perf_counter() - t:.0f}s")
+    out = {}
+    for mode in ("plain", "plain_fill", "batched"):
+        value, secs = autotune.run_workload(tuned[mode]["options"], mode, scenes, jobs, repeats=args.steps)
+        out[mode] = {"msamples_per_s": round(value / 1e6, 1), "seconds": [round(x, 3) for x in secs],
+                     "options": tuned[mode]["options"],
+                     "calibration_trials": len(tuned[mode]["trials"])}
+        log(f"[bench] batch {mode}: {value / 1e6:.1f} Msamples/s")
+    total = sum(r.width * r.height * r.spp for r in jobs)
+    return {"metric": "Msamples/s, 64 uneven jobs submitted at once (batched vs plain)",
+            "value": out["batched"]["msamples_per_s"], "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps,
+            "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": f"64 jobs, sides 128-1024, spp 16-1024, depth 8-16, 8 scenes, {total} samples"},
+            "schedules": out,
+            "speedup_vs_plain": round(out["batched"]["msamples_per_s"] / out["plain"]["msamples_per_s"], 3),
+            "speedup_vs_plain_fill": round(out["batched"]["msamples_per_s"] / out["plain_fill"]["msamples_per_s"], 3)}
+
+
 def bench_copy(args):
     """Plain 1 GiB device-to-device copy: the known-size kernel that scales rocprofv3's FETCH_SIZE and
     WRITE_SIZE readings (KiB per dispatch) into bytes for the traffic figures of the path kernels."""
@@ -326,12 +359,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="mixed", choices=["mixed", "cornell", "intersect", "convergence", "copy"])
+    ap.add_argument("--workload", default="mixed", choices=["mixed", "cornell", "intersect", "convergence", "copy", "batch"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--path-capacity", type=int, default=0, help="paths in flight per batch (0 = library default)")
     ap.add_argument("--traversal", default="auto", choices=["auto", "split", "fused"])
+    ap.add_argument("--cpu-stride", type=int, default=2,
+                    help="intersect workload: the FP64 oracle checks and times every k-th ray")
     ap.add_argument("--graph", action="store_true", help="capture the bounce loop in a hipGraph (no per-kernel times)")
     ap.add_argument("--spp", type=int, default=0, help="override samples per pixel (calibration runs only)")
     args = ap.parse_args()
@@ -343,6 +378,8 @@ def main():
         sys.exit(2)
     if args.workload == "intersect":
         res = bench_intersect(args)
+    elif args.workload == "batch":
+        res = bench_batch(args)
     elif args.workload == "copy":
         res = bench_copy(args)
     elif args.workload == "convergence":

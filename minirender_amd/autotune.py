@@ -1,0 +1,106 @@
+"""Autotuner for libmrg's device options.
+
+The knobs are the launch and queue choices of the wavefront integrator:
+
+* ``path_capacity``             paths in flight per batch (the extension / shadow queue capacity),
+* ``persistent_blocks_per_cu``  resident 256-thread blocks per CU of the persistent traversal grid,
+* ``traversal``                 ``split`` (closest-hit and any-hit kernels per bounce) or ``fused``
+                                (one traversal launch per bounce for both kinds of ray),
+* ``use_graph``                 the bounce loop captured once in a hipGraph and replayed,
+* ``max_jobs_in_flight``        (batched schedule) jobs co-scheduled in one batch.
+
+A calibration workload (a list of scenes and job requests, all submitted at once) is rendered with
+each candidate; the score is samples per second over the whole workload, best of ``repeats`` runs
+after one warm-up. Coordinate descent: starting from the library defaults, each knob in turn is
+swept with the others fixed and the best value kept; ``rounds`` passes over all knobs. Results are
+plain JSON (``save`` / ``load``), so a C++ caller can read the same file into ``mrg_device_options``.
+"""
+from __future__ import annotations
+
+import json
+import time
+
+from . import api
+
+KNOBS = {
+    "path_capacity": [1 << 22, 1 << 23, 1 << 24, 1 << 25, 1 << 26, 1 << 27],
+    "persistent_blocks_per_cu": [2, 3, 4, 5],
+    "traversal": ["split", "fused"],
+    "use_graph": [False, True],
+    "max_jobs_in_flight": [8, 16, 32, 64],
+}
+
+DEFAULTS = {"path_capacity": 1 << 26, "persistent_blocks_per_cu": 4, "traversal": "fused", "use_graph": False,
+            "max_jobs_in_flight": 64}
+
+
+def run_workload(options, schedule, scenes, requests, repeats=2, pass_samples=16):
+    """Renders every request (all submitted together) on a fresh device with `options`; returns
+    (best samples/s, per-run seconds). Scene uploads happen before the timed region."""
+    dev = api.Device(0, schedule=schedule, **options)
+    try:
+        bvhs = [dev.upload(s) for s in scenes]
+        runs = []
+        for rep in range(repeats + 1):
+            t = time.perf_counter()
+            jobs = [api.Job(bvhs[r.scene], scenes[r.scene].camera, api.RenderSettings(r.width, r.height, r.spp, r.seed),
+                            api.IntegratorSettings("mis", r.max_depth, True, 3), pass_samples) for r in requests]
+            samples = 0
+            for j in jobs:
+                j.wait(-1)
+                p = j.progress()
+                if p.state != "completed":
+                    raise RuntimeError(f"calibration job ended {p.state}: {p.error}")
+                samples += p.recorded_samples
+                j.release()
+            dt = time.perf_counter() - t
+            if rep > 0:  # the first run warms up (graph capture, page faults of the pool)
+                runs.append((samples / dt, dt))
+        for b in bvhs:
+            b.release()
+    finally:
+        dev.close()
+    best = max(runs)
+    return best[0], [r[1] for r in runs]
+
+
+def tune(scenes, requests, schedule="batched", knobs=None, rounds=1, repeats=2, log=None):
+    """Coordinate descent over `knobs` (name -> candidate values; default KNOBS, without
+    max_jobs_in_flight for the plain schedules). Returns the best options and every measurement."""
+    knobs = dict(knobs or KNOBS)
+    if schedule != "batched":
+        knobs.pop("max_jobs_in_flight", None)
+    best = {k: DEFAULTS[k] for k in knobs}
+    cache = {}
+    trials = []
+
+    def score(opts):
+        key = json.dumps(opts, sort_keys=True)
+        if key not in cache:
+            value, secs = run_workload(opts, schedule, scenes, requests, repeats)
+            cache[key] = value
+            trials.append({"options": dict(opts), "samples_per_s": value, "seconds": secs})
+            if log:
+                log(f"[autotune] {schedule} {opts} -> {value / 1e6:.1f} Msamples/s")
+        return cache[key]
+
+    best_value = score(best)
+    for _ in range(rounds):
+        for name, values in knobs.items():
+            for v in values:
+                cand = dict(best, **{name: v})
+                s = score(cand)
+                if s > best_value:
+                    best, best_value = cand, s
+    return {"schedule": schedule, "options": best, "samples_per_s": best_value, "trials": trials}
+
+
+def save(result, path):
+    with open(path, "w") as f:
+        json.dump(result, f, indent=1)
+
+
+def load(path):
+    """Options of a saved result, ready for ``api.Device(**options)``."""
+    with open(path) as f:
+        return dict(json.load(f)["options"])

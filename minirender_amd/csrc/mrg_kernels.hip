@@ -251,8 +251,7 @@ __global__ void __launch_bounds__(256) k_generate(Pool P, const GpuJob* __restri
         const float cosine = dot(d, fwd);
         P.ray_o[slot] = make_float4(o.x, o.y, o.z, cam.near_plane / cosine);
         P.ray_d[slot] = make_float4(d.x, d.y, d.z, cam.far_plane / cosine);
-        P.beta[slot]  = make_float4(1.0f, 1.0f, 1.0f, 0.0f);             // beta, previous pdf
-        P.rad[slot]   = make_float4(0.0f, 0.0f, 0.0f, 1.0f);             // radiance, eta scale
+        // beta = 1, pdf 0, radiance 0 and eta scale 1 are implied for bounce 0 (k_shade)
         P.prev[slot]  = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(kPrevDelta | (ch.job << 16)));
         P.rng[slot]   = make_uint4(static_cast<uint32_t>(rng.state), static_cast<uint32_t>(rng.state >> 32),
                                    static_cast<uint32_t>(rng.inc), static_cast<uint32_t>(rng.inc >> 32));
@@ -336,7 +335,9 @@ __global__ void __launch_bounds__(256) k_trace(Pool P, const GpuJob* __restrict_
 }
 
 // Shading of one bounce (PathTracer::trace loop body, path_tracer.cpp:164-363).
-__global__ void __launch_bounds__(256) k_shade(Pool P, const GpuJob* __restrict__ jobs, uint32_t iter) {
+// At most 128 VGPRs (4 waves per SIMD): the kernel is bound by the latency of its path-state
+// and scene loads, so occupancy pays more than the registers the compiler would otherwise use.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_shade(Pool P, const GpuJob* __restrict__ jobs, uint32_t iter) {
     const uint32_t  n       = P.counters[kCtrExt + iter];
     const uint32_t* queue   = (iter & 1u) ? P.q_ext1 : P.q_ext0;
     uint32_t*       next_q  = (iter & 1u) ? P.q_ext0 : P.q_ext1;
@@ -360,8 +361,10 @@ __global__ void __launch_bounds__(256) k_shade(Pool P, const GpuJob* __restrict_
             const GpuScene& sc  = job.scene;
             const float4  ro = P.ray_o[path], rd = P.ray_d[path];
             const F3      ray_d = mk(rd.x, rd.y, rd.z);
-            const float4  bt = P.beta[path];
-            const float4  lr = P.rad[path];
+            // camera paths (bounce 0) start with beta = 1, pdf 0, L = 0, eta scale 1: k_generate
+            // writes none of them and they are not read here
+            const float4  bt = iter == 0 ? make_float4(1.0f, 1.0f, 1.0f, 0.0f) : P.beta[path];
+            const float4  lr = iter == 0 ? make_float4(0.0f, 0.0f, 0.0f, 1.0f) : P.rad[path];
             F3            beta = mk(bt.x, bt.y, bt.z);
             float         prev_pdf  = bt.w;
             F3            L         = mk(lr.x, lr.y, lr.z);

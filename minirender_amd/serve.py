@@ -61,6 +61,16 @@ def request_stream(count=64, rate=40.0, seed=5, cancel_fraction=0.05, scenes=8, 
     return reqs
 
 
+def batch_jobs(count=64, seed=4, spp_divisor=1, **kw):
+    """Uneven jobs submitted together (no arrivals, no cancels): the request distribution of
+    request_stream with every arrival at 0; `spp_divisor` shrinks sample counts for calibration."""
+    reqs = request_stream(count=count, seed=seed, cancel_fraction=0.0, **kw)
+    for r in reqs:
+        r.arrival = 0.0
+        r.spp = max(2, r.spp // spp_divisor)
+    return reqs
+
+
 def _percentile(v, q):
     return float(np.percentile(np.asarray(v, np.float64), q)) if len(v) else None
 
