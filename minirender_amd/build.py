@@ -49,6 +49,23 @@ def build(verbose=False):
     return LIB
 
 
+SESSION_CHECK = os.path.join(OUT_DIR, "session_check")
+
+
+def build_session_check(verbose=False):
+    """The C++ consumer of include/mrg_session.hpp used by the tests (host code only, g++)."""
+    src = os.path.join(HERE, "..", "tests", "cpp", "session_check.cpp")
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-Werror", "-I", os.path.join(HERE, "..", "include"), src,
+           "-o", SESSION_CHECK, "-L", OUT_DIR, "-lmrg", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"session_check build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print("built", SESSION_CHECK)
+    return SESSION_CHECK
+
+
 if __name__ == "__main__":
     build(verbose=True)
+    build_session_check(verbose=True)
     sys.exit(0)
