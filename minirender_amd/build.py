@@ -65,7 +65,26 @@ def build_session_check(verbose=False):
     return SESSION_CHECK
 
 
+RENDER_CLI = os.path.join(OUT_DIR, "mrg_render")
+
+
+def build_cli(verbose=False):
+    """mrg_render: the --render command line (scene JSON + OBJ in, EXR + PNG out) over libmrg."""
+    host = os.path.join(CSRC, "host")
+    srcs = [os.path.join(host, f) for f in ("render_cli.cpp", "scene_io.cpp", "image_io.cpp")]
+    cmd = ["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-Wall", "-Wextra", "-Werror",
+           "-I", os.path.join(HERE, "..", "include"), "-I", host] + srcs + \
+          ["-o", RENDER_CLI, "-L", OUT_DIR, "-lmrg", "-lz", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"mrg_render build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print("built", RENDER_CLI)
+    return RENDER_CLI
+
+
 if __name__ == "__main__":
     build(verbose=True)
     build_session_check(verbose=True)
+    build_cli(verbose=True)
     sys.exit(0)

@@ -1,0 +1,231 @@
+// Command line of the GPU path tracer: the reference's `--render` entry point
+// (src/app/render_cli.cpp:46-181, dispatched from cli.cpp:33-35) with `--integrator gpu`:
+//
+//   mrg_render --render scene.json output-prefix [--width N --height N --spp N --depth N
+//              --mode mis|bsdf|light --seed N --exposure EV --no-rr --rr-depth N --threads N
+//              --tile N --batch N --integrator gpu]
+//
+// writes output-prefix.exr (linear float RGB) and output-prefix.png (display image), printing
+// progress lines to stderr and the job summary to stdout. --threads and --tile are validated as
+// the reference does and have no GPU meaning; --batch is the progressive pass size.
+//
+// Diagnostics used by the tests (no GPU needed):
+//   mrg_render --scene-info scene.json [flat.bin]   triangle / instance counts, optional dump
+//   mrg_render --obj mesh.obj mesh.bin              OBJ loader output
+//   mrg_render --encode image.bin output-prefix [--exposure EV]   EXR + PNG of a raw image
+#include <charconv>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <locale>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <string_view>
+#include <type_traits>
+
+#include "image_io.hpp"
+#include "mrg_session.hpp"
+#include "scene_io.hpp"
+
+namespace {
+
+namespace gpu = minirender::gpu;
+
+template <typename T>
+T number(std::string_view text) {
+    T    result{};
+    bool valid = false;
+    if constexpr (std::is_floating_point_v<T>) {
+        std::istringstream input{std::string(text)};
+        input.imbue(std::locale::classic());
+        input >> std::noskipws >> result;
+        valid = static_cast<bool>(input) && input.peek() == std::char_traits<char>::eof();
+    } else {
+        const auto parsed = std::from_chars(text.data(), text.data() + text.size(), result);
+        valid             = parsed.ec == std::errc{} && parsed.ptr == text.data() + text.size();
+    }
+    if (!valid) throw std::invalid_argument("Invalid numeric render option: " + std::string(text));
+    return result;
+}
+
+template <class T>
+void put(std::ofstream& out, const T& v) {
+    out.write(reinterpret_cast<const char*>(&v), sizeof v);
+}
+template <class T>
+void put_vec(std::ofstream& out, const std::vector<T>& v) {
+    out.write(reinterpret_cast<const char*>(v.data()), std::streamsize(v.size() * sizeof(T)));
+}
+
+// u64 n, u32 instances, u32 directional, f32 env[3], mrg_camera, then positions, normals,
+// instance, triangle, mrg_material[instances], mrg_directional_light[directional]
+void dump_flat(const mrg_host::FlatScene& f, const std::string& path) {
+    std::ofstream out(path, std::ios::binary);
+    out.exceptions(std::ios::badbit | std::ios::failbit);
+    put(out, uint64_t(f.instance.size()));
+    put(out, uint32_t(f.materials.size()));
+    put(out, uint32_t(f.lights.size()));
+    put(out, f.environment);
+    put(out, f.camera);
+    put_vec(out, f.positions);
+    put_vec(out, f.normals);
+    put_vec(out, f.instance);
+    put_vec(out, f.triangle);
+    put_vec(out, f.materials);
+    put_vec(out, f.lights);
+}
+
+int scene_info(int argc, char** argv) {
+    if (argc < 1) throw std::invalid_argument("Usage: --scene-info scene.json [flat.bin]");
+    const auto doc  = mrg_host::load_scene(std::filesystem::u8path(argv[0]));
+    const auto flat = mrg_host::flatten(doc);
+    std::cout << "triangles: " << flat.instance.size() << "\ninstances: " << flat.materials.size()
+              << "\nmeshes: " << doc.meshes.size() << "\ndirectional lights: " << flat.lights.size() << '\n';
+    if (argc >= 2) dump_flat(flat, argv[1]);
+    return 0;
+}
+
+// u32 positions, u32 triangles, f32 positions[3n], then per corner: u32 position, f32 normal[3], f32 uv[2]
+int dump_obj(int argc, char** argv) {
+    if (argc < 2) throw std::invalid_argument("Usage: --obj mesh.obj mesh.bin");
+    const auto    mesh = mrg_host::load_obj(std::filesystem::u8path(argv[0]));
+    std::ofstream out(argv[1], std::ios::binary);
+    out.exceptions(std::ios::badbit | std::ios::failbit);
+    put(out, uint32_t(mesh.positions.size()));
+    put(out, uint32_t(mesh.triangles.size()));
+    for (const auto& p : mesh.positions) put(out, p);
+    for (const auto& t : mesh.triangles)
+        for (const auto& c : t) {
+            put(out, c.position);
+            put(out, c.normal);
+            put(out, c.uv);
+        }
+    return 0;
+}
+
+void write_outputs(const mrg_host::RgbImage& image, std::filesystem::path prefix, double exposure) {
+    if (prefix.extension() == ".png" || prefix.extension() == ".exr") prefix.replace_extension();
+    if (!prefix.parent_path().empty()) std::filesystem::create_directories(prefix.parent_path());
+    auto exr = prefix, png = prefix;
+    exr += ".exr";
+    png += ".png";
+    mrg_host::write_exr(image, exr);
+    mrg_host::write_png(mrg_host::display_image(image, exposure), png);
+}
+
+int encode(int argc, char** argv) {
+    if (argc < 2) throw std::invalid_argument("Usage: --encode image.bin output-prefix [--exposure EV]");
+    double exposure = 0;
+    if (argc >= 4 && std::string_view(argv[2]) == "--exposure") exposure = number<double>(argv[3]);
+    std::ifstream in(argv[0], std::ios::binary);
+    mrg_host::RgbImage image;
+    in.read(reinterpret_cast<char*>(&image.width), 4);
+    in.read(reinterpret_cast<char*>(&image.height), 4);
+    if (!in || image.width == 0 || image.height == 0 || uint64_t(image.width) * image.height > (uint64_t(1) << 28))
+        throw std::invalid_argument("bad image file");
+    image.rgb.resize(size_t(image.width) * image.height * 3);
+    in.read(reinterpret_cast<char*>(image.rgb.data()), std::streamsize(image.rgb.size() * 4));
+    if (!in) throw std::invalid_argument("truncated image file");
+    write_outputs(image, std::filesystem::u8path(argv[1]), exposure);
+    return 0;
+}
+
+int render(int argc, char** argv) {
+    if (argc < 2)
+        throw std::invalid_argument(
+            "Usage: --render scene.json output-prefix [--width N --height N --spp N --depth N "
+            "--mode mis|bsdf|light --seed N --exposure EV --no-rr --rr-depth N --threads N --tile "
+            "N --batch N --integrator gpu]");
+    uint64_t width = 256, height = 256, seed = 1;
+    uint32_t spp = 64, depth = 8, rr_depth = 3, threads = 0, tile = 16, batch = 4, mode = MRG_MODE_MIS;
+    bool     rr       = true;
+    double   exposure = 0;
+    for (int index = 2; index < argc; ++index) {
+        const std::string_view option(argv[index]);
+        if (option == "--no-rr") {
+            rr = false;
+            continue;
+        }
+        if (index + 1 >= argc) throw std::invalid_argument("Render option is missing its value: " + std::string(option));
+        const std::string_view value(argv[++index]);
+        if (option == "--width") width = number<uint64_t>(value);
+        else if (option == "--height") height = number<uint64_t>(value);
+        else if (option == "--spp") spp = number<uint32_t>(value);
+        else if (option == "--depth") depth = number<uint32_t>(value);
+        else if (option == "--rr-depth") rr_depth = number<uint32_t>(value);
+        else if (option == "--threads") threads = number<uint32_t>(value);
+        else if (option == "--tile") tile = number<uint32_t>(value);
+        else if (option == "--batch") batch = number<uint32_t>(value);
+        else if (option == "--seed") seed = number<uint64_t>(value);
+        else if (option == "--mode") {
+            if (value == "bsdf") mode = MRG_MODE_BSDF;
+            else if (value == "light") mode = MRG_MODE_LIGHT;
+            else if (value == "mis") mode = MRG_MODE_MIS;
+            else throw std::invalid_argument("Sampling mode must be bsdf, light or mis");
+        } else if (option == "--integrator") {
+            if (value == "path" || value == "sppm")
+                throw std::invalid_argument("The CPU integrators (path, sppm) are not part of this build; use --integrator gpu");
+            if (value != "gpu") throw std::invalid_argument("Integrator must be gpu");
+        } else if (option == "--exposure") exposure = number<double>(value);
+        else throw std::invalid_argument("Unknown render option: " + std::string(option));
+    }
+    // RenderSettings / IntegratorSettings / SessionSettings::validate
+    if (width == 0 || height == 0 || width > 16384 || height > 16384 || width * height > 64u * 1024u * 1024u || spp == 0)
+        throw std::invalid_argument("Render dimensions or sample count exceed the image budget");
+    if (depth > 128 || rr_depth > 128) throw std::invalid_argument("Path and roulette depths must not exceed 128");
+    if (threads > 32 || tile == 0 || tile > 128 || batch == 0 || batch > 1024)
+        throw std::invalid_argument("Session requires <=32 threads, tile size in [1,128], batch in [1,1024]");
+    if (!std::isfinite(exposure) || std::abs(exposure) > 64) throw std::invalid_argument("Exposure must be finite and in [-64,64]");
+
+    const auto flat  = mrg_host::flatten(mrg_host::load_scene(std::filesystem::u8path(argv[0])));
+    const auto start = std::chrono::steady_clock::now();
+    gpu::Device device(0);
+    auto        scene = std::make_shared<gpu::SceneBvh>(device, flat.desc());
+    gpu::RenderSession session(device);
+    session.start(scene, flat.camera, {uint32_t(width), uint32_t(height), spp, seed}, {mode, depth, rr, rr_depth}, {batch});
+    while (!session.wait_for(std::chrono::milliseconds(200))) {
+        const auto p = session.progress();
+        std::cerr << "progress: " << gpu::render_state_name(p.state) << ' ' << p.recorded_samples << '/' << p.total_samples << '\n';
+    }
+    session.wait();
+    const auto progress = session.progress();
+    if (progress.state != gpu::RenderState::completed)
+        throw std::runtime_error(progress.error.empty() ? "Render did not complete" : progress.error);
+    const auto         result = session.result();
+    mrg_host::RgbImage image{result->width, result->height, result->rgb};
+    const auto         finished = std::chrono::steady_clock::now();
+    write_outputs(image, std::filesystem::u8path(argv[1]), exposure);
+    static const char* modes[] = {"bsdf", "light", "mis"};
+    const auto&        st      = progress.statistics;
+    std::cout << "GPU path render complete\nmode: " << modes[mode] << "\nresolution: " << width << 'x' << height
+              << "\nspp: " << spp << "\ntriangles: " << flat.instance.size() << "\npaths: " << st.paths
+              << "\nrecorded samples: " << progress.recorded_samples << "\nextension rays: " << st.extension_rays
+              << "\nshadow rays: " << st.shadow_rays << "\nscattering events: " << st.scattering_events
+              << "\nroulette terminations: " << st.roulette_terminations
+              << "\nelapsed ms: " << std::chrono::duration<double, std::milli>(finished - start).count()
+              << "\nEXR and PNG saved.\n";
+    return 0;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    try {
+        if (argc >= 2) {
+            const std::string_view mode(argv[1]);
+            if (mode == "--render") return render(argc - 2, argv + 2);
+            if (mode == "--scene-info") return scene_info(argc - 2, argv + 2);
+            if (mode == "--obj") return dump_obj(argc - 2, argv + 2);
+            if (mode == "--encode") return encode(argc - 2, argv + 2);
+        }
+        std::cerr << "Usage: mrg_render --render scene.json output-prefix [options] | --scene-info scene.json [flat.bin] | "
+                     "--obj mesh.obj mesh.bin | --encode image.bin output-prefix [--exposure EV]\n";
+        return 2;
+    } catch (const std::exception& error) {
+        std::cerr << error.what() << '\n';
+        return 1;
+    }
+}

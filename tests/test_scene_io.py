@@ -1,0 +1,296 @@
+"""Scene ingestion and image output of the `mrg_render` command line (CPU; no GPU needed).
+
+Pinned by: the reference's own golden facts (studio.json flattens to 26 triangles,
+CMakeLists.txt:205-207; OBJ quads / seams / degenerate rejection, scene_tests.cpp:76-112; the
+inverse-transpose normal test, scene_tests.cpp:60-72), the reference's tinyexr-written EXR files
+(decoded by an independent reader; our header bytes equal theirs for the same size), the PNG and
+EXR format specifications (independent readers in tests/_formats.py), and closed forms of the
+display transform. Float transform bits depend on glm 1.0.3 and the platform's cosf/sinf, which
+the reference does not ship: world positions are checked against an FP64 evaluation of
+T * Rz * Ry * Rx * S within 2e-6 relative ("parity unpinned" at the bit level)."""
+import json
+import math
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+import _formats as F
+from minirender_amd import build as B
+
+REF = "/root/reference/scenes"
+
+
+def cli(*args, check=True):
+    if not os.path.exists(B.RENDER_CLI):
+        pytest.fail(f"{B.RENDER_CLI} is not built (python -m minirender_amd.build)")
+    r = subprocess.run([B.RENDER_CLI, *map(str, args)], capture_output=True, text=True, timeout=120)
+    if check:
+        assert r.returncode == 0, r.stderr
+    return r
+
+
+# ---------------------------------------------------------------------------------------------
+# OBJ (obj_loader.cpp:14-93)
+
+CUBE = """# unit cube, quads with per-face normals
+v -1 -1 -1\nv 1 -1 -1\nv 1 1 -1\nv -1 1 -1\nv -1 -1 1\nv 1 -1 1\nv 1 1 1\nv -1 1 1
+vn 0 0 -1\nvn 0 0 1\nvn 0 -1 0\nvn 0 1 0\nvn -1 0 0\nvn 1 0 0
+f 1//1 4//1 3//1 2//1\nf 5//2 6//2 7//2 8//2\nf 1//3 2//3 6//3 5//3
+f 4//4 8//4 7//4 3//4\nf 1//5 5//5 8//5 4//5\nf 2//6 3//6 7//6 6//6
+"""
+PLANE = "v -1 0 -1\nv 1 0 -1\nv 1 0 1\nv -1 0 1\nvt 0 0\nvt 1 0\nvt 1 1\nvt 0 1\nf 1/1 4/4 3/3 2/2\n"
+
+
+def test_obj_quads_triangulate_with_unit_normals(tmp_path):
+    (tmp_path / "cube.obj").write_text(CUBE)
+    cli("--obj", tmp_path / "cube.obj", tmp_path / "cube.bin")
+    pos, tris = F.read_obj_dump(tmp_path / "cube.bin")
+    assert len(pos) == 8 and len(tris) == 12
+    assert np.allclose(np.linalg.norm(tris["normal"], axis=-1), 1.0, atol=1e-6)
+    assert (tris["position"] < 8).all()
+    (tmp_path / "plane.obj").write_text(PLANE)
+    cli("--obj", tmp_path / "plane.obj", tmp_path / "plane.bin")
+    pos, tris = F.read_obj_dump(tmp_path / "plane.bin")
+    assert len(tris) == 2
+    # no vn records: the face normal (winding 1,4,3 points up)
+    assert tris[0]["normal"][0][1] == pytest.approx(1.0)
+
+
+def test_obj_rejects_missing_and_degenerate(tmp_path):
+    assert cli("--obj", tmp_path / "missing.obj", tmp_path / "o.bin", check=False).returncode == 1
+    (tmp_path / "bad.obj").write_text("v 0 0 0\nv 1 0 0\nv 2 0 0\nf 1 2 3\n")
+    r = cli("--obj", tmp_path / "bad.obj", tmp_path / "o.bin", check=False)
+    assert r.returncode == 1 and "degenerate" in r.stderr
+    (tmp_path / "zero.obj").write_text("v 0 0 0\nv 1 0 0\nv 0 1 0\nvn 0 0 0\nf 1//1 2//1 3//1\n")
+    r = cli("--obj", tmp_path / "zero.obj", tmp_path / "o.bin", check=False)
+    assert r.returncode == 1 and "normal is zero" in r.stderr
+    (tmp_path / "range.obj").write_text("v 0 0 0\nv 1 0 0\nv 0 1 0\nf 1 2 4\n")
+    assert cli("--obj", tmp_path / "range.obj", tmp_path / "o.bin", check=False).returncode == 1
+
+
+def test_obj_corners_keep_uv_seams(tmp_path):
+    # scene_tests.cpp:98-112: shared position indices with different UVs; negative indices
+    (tmp_path / "seam.obj").write_text("v 0 0 0\nv 1 0 0\nv 1 1 0\nv 0 1 0\n"
+                                       "vt 0 0\nvt 1 0\nvt 1 1\nvt 0.5 0.5\nvt 0 1\n"
+                                       "vn 0 0 1\nf 1/1/1 2/2/1 3/3/1\nf -4/4/1 -2/3/1 -1/5/1\n")
+    cli("--obj", tmp_path / "seam.obj", tmp_path / "seam.bin")
+    pos, tris = F.read_obj_dump(tmp_path / "seam.bin")
+    assert len(pos) == 4
+    assert tris[0][0]["position"] == tris[1][0]["position"]
+    assert tris[0][0]["uv"][0] == 0.0 and tris[1][0]["uv"][0] == 0.5
+
+
+def test_obj_quad_splits_along_shorter_diagonal(tmp_path):
+    # a kite whose 1-3 diagonal is shorter than 0-2: split (0,1,3) + (1,2,3)
+    (tmp_path / "kite.obj").write_text("v 0 0 0\nv 1 0.2 0\nv 3 0 0\nv 1 -0.2 0\nf 1 4 3 2\n")
+    cli("--obj", tmp_path / "kite.obj", tmp_path / "kite.bin")
+    _, tris = F.read_obj_dump(tmp_path / "kite.bin")
+    assert [list(t["position"]) for t in tris] == [[0, 3, 1], [3, 2, 1]]
+
+
+# ---------------------------------------------------------------------------------------------
+# JSON v1 scene (scene_io.cpp:47-162), transforms (scene.cpp:119-139), flattening (bvh.cpp:53-89)
+
+def _matrix64(t, r, s):
+    rx, ry, rz = (math.radians(a) for a in r)
+    T = np.eye(4)
+    T[:3, 3] = t
+    Rz = np.array([[math.cos(rz), -math.sin(rz), 0, 0], [math.sin(rz), math.cos(rz), 0, 0], [0, 0, 1, 0], [0, 0, 0, 1]])
+    Ry = np.array([[math.cos(ry), 0, math.sin(ry), 0], [0, 1, 0, 0], [-math.sin(ry), 0, math.cos(ry), 0], [0, 0, 0, 1]])
+    Rx = np.array([[1, 0, 0, 0], [0, math.cos(rx), -math.sin(rx), 0], [0, math.sin(rx), math.cos(rx), 0], [0, 0, 0, 1]])
+    return T @ Rz @ Ry @ Rx @ np.diag([*s, 1.0])
+
+
+def write_scene(dirpath, objects, materials=None, lights=None, environment=(0.1, 0.1, 0.1), camera=None, extra=None):
+    """A JSON v1 document in `dirpath` (meshes: name -> OBJ text)."""
+    meshes = {}
+    for o in objects:
+        if o["mesh"] not in meshes:
+            (dirpath / f"{o['mesh']}.obj").write_text(o["obj"])
+            meshes[o["mesh"]] = {"id": o["mesh"], "source": f"{o['mesh']}.obj"}
+    doc = {"version": 1,
+           "camera": camera or {"position": [0, 1, 4], "target": [0, 0.5, 0], "up": [0, 1, 0], "vertical_fov": 40,
+                                "near": 0.05, "far": 100},
+           "environment": list(environment),
+           "materials": materials or [{"id": "white", "kind": "diffuse", "base_color": [0.7, 0.7, 0.7]}],
+           "meshes": list(meshes.values()),
+           "objects": [{"id": o["id"], "mesh": o["mesh"], "material": o.get("material", "white"),
+                        "visible": o.get("visible", True),
+                        "transform": {"translation": o.get("t", [0, 0, 0]), "rotation": o.get("r", [0, 0, 0]),
+                                      "scale": o.get("s", [1, 1, 1])}} for o in objects],
+           "directional_lights": lights or []}
+    doc.update(extra or {})
+    path = dirpath / "scene.json"
+    path.write_text(json.dumps(doc, indent=2))
+    return path
+
+
+def test_flatten_transforms_and_normals(tmp_path):
+    # scene_tests.cpp:60-72: a tangent edge and a corner normal stay orthogonal after the transform
+    t, r, s = [4, -2, 1], [15, 35, -10], [2, 0.5, 3]
+    n = np.array([1, 1, 0]) / math.sqrt(2)
+    obj = "v 0 0 0\nv 1 -1 0\nv 0 0 1\nvn 1 1 0\nf 1//1 2//1 3//1\n"
+    path = write_scene(tmp_path, [{"id": "a", "mesh": "tri", "obj": obj, "t": t, "r": r, "s": s},
+                                  {"id": "hidden", "mesh": "tri", "obj": obj, "visible": False}])
+    out = cli("--scene-info", path, tmp_path / "flat.bin").stdout
+    assert "triangles: 1\n" in out and "instances: 1\n" in out
+    sc = F.read_flat_scene(tmp_path / "flat.bin")
+    M = _matrix64(t, r, s)
+    local = np.array([[0, 0, 0, 1], [1, -1, 0, 1], [0, 0, 1, 1]], float)
+    world = (M @ local.T).T[:, :3]
+    assert np.allclose(sc.positions[0], world, rtol=2e-6, atol=2e-6)
+    edge = sc.positions[0][1].astype(np.float64) - sc.positions[0][0]
+    nw = sc.normals[0][0].astype(np.float64)
+    assert abs(np.dot(nw, edge / np.linalg.norm(edge))) <= 1e-6
+    assert np.linalg.norm(nw) == pytest.approx(1.0, abs=1e-6)
+    assert np.allclose(nw, np.linalg.inv(M[:3, :3]).T @ n / np.linalg.norm(np.linalg.inv(M[:3, :3]).T @ n), atol=2e-6)
+    assert sc.materials[0]["mirrored"] == 0
+
+
+def test_flatten_mirrored_instance_and_order(tmp_path):
+    obj = "v 0 0 0\nv 1 0 0\nv 0 1 0\nf 1 2 3\n"
+    path = write_scene(tmp_path, [{"id": "a", "mesh": "m", "obj": obj, "s": [1, 1, 1]},
+                                  {"id": "b", "mesh": "m", "obj": obj, "s": [-1, 1, 1], "t": [3, 0, 0]}],
+                       lights=[{"id": "sun", "direction": [0, -1, 0], "color": [1, 1, 1], "intensity": 2}])
+    cli("--scene-info", path, tmp_path / "flat.bin")
+    sc = F.read_flat_scene(tmp_path / "flat.bin")
+    assert list(sc.instance) == [0, 1] and list(sc.triangle) == [0, 0]
+    assert list(sc.materials["mirrored"]) == [0, 1]
+    assert np.allclose(sc.positions[1][1], [2, 0, 0])
+    assert sc.directional[0]["intensity"] == 2
+
+
+@pytest.mark.parametrize("extra,message", [
+    ({"version": 2}, "Unsupported scene version"),
+    ({"media": [{"id": "fog"}]}, "media are not supported"),
+    ({"environment_map": {"texture": "sky"}}, "Environment maps are not supported"),
+])
+def test_scene_rejections(tmp_path, extra, message):
+    path = write_scene(tmp_path, [{"id": "a", "mesh": "m", "obj": "v 0 0 0\nv 1 0 0\nv 0 1 0\nf 1 2 3\n"}], extra=extra)
+    r = cli("--scene-info", path, check=False)
+    assert r.returncode == 1 and message in r.stderr
+
+
+def test_scene_validation_errors(tmp_path):
+    obj = "v 0 0 0\nv 1 0 0\nv 0 1 0\nf 1 2 3\n"
+    bad = [({"materials": [{"id": "white", "kind": "diffuse", "base_color": [1.5, 0, 0]}]}, "base color"),
+           ({"materials": [{"id": "white", "kind": "subsurface", "base_color": [0.5, 0.5, 0.5]}]}, "not supported"),
+           ({"materials": [{"id": "white", "kind": "plastic", "base_color": [0.5, 0.5, 0.5]}]}, "Unsupported material kind")]
+    for extra, message in bad:
+        path = write_scene(tmp_path, [{"id": "a", "mesh": "m", "obj": obj}], extra=extra)
+        r = cli("--scene-info", path, check=False)
+        assert r.returncode == 1 and message in r.stderr, (extra, r.stderr)
+    path = write_scene(tmp_path, [{"id": "a", "mesh": "m", "obj": obj, "s": [0, 1, 1]}])
+    r = cli("--scene-info", path, check=False)
+    assert r.returncode == 1 and "scale must be invertible" in r.stderr
+    path = write_scene(tmp_path, [{"id": "a", "mesh": "m", "obj": obj, "material": "nope"}])
+    r = cli("--scene-info", path, check=False)
+    assert r.returncode == 1 and "missing material" in r.stderr
+
+
+def test_json_escapes_and_numbers(tmp_path):
+    obj = "v 0 0 0\nv 1 0 0\nv 0 1 0\nf 1 2 3\n"
+    (tmp_path / "m.obj").write_text(obj)
+    text = ('{"version": 1, "camera": {"position": [0, 1e0, 4.0E+0], "target": [0, 0.5, -0.0], "up": [0, 1, 0], '
+            '"vertical_fov": 40, "near": 5e-2, "far": 100}, "environment": [0, 0, 0], '
+            '"materials": [{"id": "w\\u00e9\\ud83d\\ude00", "kind": "diffuse", "base_color": [0.5, 0.5, 0.5]}], '
+            '"meshes": [{"id": "m", "source": "m.obj"}], '
+            '"objects": [{"id": "a", "mesh": "m", "material": "w\\u00e9\\ud83d\\ude00", '
+            '"transform": {"translation": [0, 0, 0], "rotation": [0, 0, 0], "scale": [1, 1, 1]}}], '
+            '"directional_lights": []}')
+    (tmp_path / "s.json").write_text(text)
+    assert "triangles: 1" in cli("--scene-info", tmp_path / "s.json").stdout
+    (tmp_path / "t.json").write_text(text[:-1])
+    r = cli("--scene-info", tmp_path / "t.json", check=False)
+    assert r.returncode == 1 and "JSON parse error" in r.stderr
+
+
+@pytest.mark.skipif(not os.path.isdir(REF), reason="reference scenes not present")
+def test_reference_scenes():
+    # studio.json: 26 triangles (the reference CLI's --trace-normals check, CMakeLists.txt:205-207)
+    assert "triangles: 26\n" in cli("--scene-info", os.path.join(REF, "studio.json")).stdout
+    renderable = {"cornell-box.json": 36, "cornell-small-light.json": 36, "path-room.json": 36, "soft-shadows.json": 26,
+                  "studio-two-lights.json": 26, "cornell-materials.json": 4428, "material-balls.json": 13252}
+    for name, tris in renderable.items():
+        assert f"triangles: {tris}\n" in cli("--scene-info", os.path.join(REF, name)).stdout, name
+    unsupported = {"sss-wax.json": "subsurface", "volume-cornell.json": "media", "studio-textured.json": "textures",
+                   "white-furnace.json": "energy compensation"}
+    for name, word in unsupported.items():
+        r = cli("--scene-info", os.path.join(REF, name), check=False)
+        assert r.returncode == 1 and word in r.stderr, name
+
+
+# ---------------------------------------------------------------------------------------------
+# EXR / PNG (exr_io.cpp:36-90, raster_io.cpp:115-138, path_tracer.cpp:407-424)
+
+def _encode(tmp_path, img, exposure=None):
+    h, w, _ = img.shape
+    raw = tmp_path / "img.bin"
+    raw.write_bytes(struct.pack("<II", w, h) + np.ascontiguousarray(img, np.float32).tobytes())
+    args = ["--encode", raw, tmp_path / "out"] + (["--exposure", exposure] if exposure is not None else [])
+    return cli(*args, check=False)
+
+
+def test_exr_round_trip_is_exact(tmp_path):
+    rng = np.random.default_rng(3)
+    img = (rng.standard_normal((37, 53, 3)) * 10).astype(np.float32)
+    img[0, 0] = [0.0, -0.0, 3e38]
+    assert _encode(tmp_path, img).returncode == 0
+    back = F.exr_rgb(tmp_path / "out.exr")
+    assert back.tobytes() == img.tobytes()
+    header, _ = F.read_exr(tmp_path / "out.exr")
+    assert header["compression"][1] == b"\x03"
+    bad = img.copy()
+    bad[1, 1, 1] = np.nan
+    r = _encode(tmp_path, bad)
+    assert r.returncode == 1 and "nonfinite" in r.stderr
+
+
+@pytest.mark.skipif(not os.path.isdir(REF), reason="reference scenes not present")
+def test_exr_matches_reference_codec(tmp_path):
+    # a tinyexr-written file of the reference: our reader decodes it, our writer reproduces its header
+    ref_path = os.path.join(REF, "catalog", "motion_0.exr")
+    img = F.exr_rgb(ref_path)
+    assert np.isfinite(img).all() and img.shape == (48, 64, 3)
+    assert _encode(tmp_path, img).returncode == 0
+    assert F.exr_rgb(tmp_path / "out.exr").tobytes() == img.tobytes()
+    ours, theirs = open(tmp_path / "out.exr", "rb").read(), open(ref_path, "rb").read()
+    header_len = theirs.index(b"screenWindowWidth") + len(b"screenWindowWidth\0float\0") + 8 + 1
+    assert ours[:header_len] == theirs[:header_len]
+
+
+def _srgb8(x):
+    x = np.asarray(x, np.float32)
+    x = np.where(np.isfinite(x), np.maximum(x, 0), 0).astype(np.float32)
+    e = np.where(x <= np.float32(0.0031308), np.float32(12.92) * x,
+                 np.float32(1.055) * np.power(x, np.float32(1 / 2.4)) - np.float32(0.055)).astype(np.float32)
+    return np.floor(np.clip(e, 0, 1) * np.float32(255) + 0.5).astype(np.int32)
+
+
+@pytest.mark.parametrize("exposure", [None, 1.5, -2.0])
+def test_png_is_reinhard_then_srgb(tmp_path, exposure):
+    rng = np.random.default_rng(5)
+    img = np.abs(rng.standard_normal((19, 23, 3)) * 4).astype(np.float32)
+    img[0, :3] = [[0, 0, 0], [1, 1, 1], [-1, 2, 1e6]]
+    assert _encode(tmp_path, img, exposure).returncode == 0
+    png = F.read_png(tmp_path / "out.png").astype(np.int32)
+    scale = 2.0 ** (exposure or 0.0)
+    pos = np.maximum(img.astype(np.float64), 0) * scale
+    expect = _srgb8((pos / (1 + pos)).astype(np.float32))
+    assert np.abs(png - expect).max() <= 1 and (png != expect).mean() < 1e-3
+    assert list(png[0, 0]) == [0, 0, 0]
+    assert list(png[0, 1]) == list(_srgb8(np.float32(scale / (1 + scale)) * np.ones(3)))
+
+
+def test_cli_argument_errors(tmp_path):
+    path = write_scene(tmp_path, [{"id": "a", "mesh": "m", "obj": "v 0 0 0\nv 1 0 0\nv 0 1 0\nf 1 2 3\n"}])
+    for args, message in [(["--integrator", "path"], "not part of this build"), (["--width", "x"], "Invalid numeric"),
+                          (["--bogus", "1"], "Unknown render option"), (["--width"], "missing its value"),
+                          (["--batch", "0"], "batch in [1,1024]"), (["--exposure", "100"], "Exposure"),
+                          (["--mode", "fast"], "Sampling mode")]:
+        r = cli("--render", path, tmp_path / "out", *args, check=False)
+        assert r.returncode == 1 and message in r.stderr, (args, r.stderr)
+    assert cli(check=False).returncode == 2
