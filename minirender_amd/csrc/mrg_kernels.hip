@@ -82,6 +82,60 @@ struct WaveWork {
     }
 };
 
+// Direction octant (sign bits of x, y, z) used to group queue entries.
+__device__ __forceinline__ uint32_t octant(float x, float y, float z) {
+    return (__builtin_signbitf(x) ? 1u : 0u) | (__builtin_signbitf(y) ? 2u : 0u) | (__builtin_signbitf(z) ? 4u : 0u);
+}
+
+// Block-wide (256 lanes) compaction into two queues at once, grouped by direction octant: the
+// block's entries of each queue are written octant-major (lane order within an octant), so that a
+// 64-entry chunk of the next traversal holds rays of few octants from neighbouring paths. One
+// atomic per queue per block. Every thread of the block must call (it synchronises the block).
+struct GroupedSlots {
+    uint32_t ext, shadow;
+};
+
+__device__ __forceinline__ GroupedSlots block_reserve_grouped(uint32_t* ext_counter, bool ext, uint32_t ext_key,
+                                                              uint32_t* sh_counter, bool sh, uint32_t sh_key,
+                                                              uint32_t* s_off /*[64]*/, uint32_t* s_base /*[2]*/) {
+    const uint32_t wave = threadIdx.x >> 6;  // 0..3
+    uint64_t       ext_mask = 0, sh_mask = 0;
+    for (uint32_t b = 0; b < 8; ++b) {
+        const uint64_t me = __ballot(ext && ext_key == b);
+        const uint64_t ms = __ballot(sh && sh_key == b);
+        if (ext && ext_key == b) ext_mask = me;
+        if (sh && sh_key == b) sh_mask = ms;
+        if (lane_id() == 0) {
+            s_off[b * 4 + wave]      = static_cast<uint32_t>(__builtin_popcountll(me));
+            s_off[32 + b * 4 + wave] = static_cast<uint32_t>(__builtin_popcountll(ms));
+        }
+    }
+    __syncthreads();
+    if (wave == 0) {
+        // exclusive scan of the 64 counts (entry = queue * 32 + octant * 4 + wave), per queue
+        const uint32_t l = lane_id();
+        const uint32_t x = s_off[l];
+        uint32_t       v = x;
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t y = __shfl_up(v, d, 64);
+            if (l >= static_cast<uint32_t>(d)) v += y;
+        }
+        const uint32_t total0 = __shfl(v, 31, 64);
+        const uint32_t total1 = __shfl(v, 63, 64) - total0;
+        s_off[l] = v - x - (l >= 32 ? total0 : 0u);
+        if (l == 0) {
+            s_base[0] = total0 ? atomicAdd(ext_counter, total0) : 0u;
+            s_base[1] = total1 ? atomicAdd(sh_counter, total1) : 0u;
+        }
+    }
+    __syncthreads();
+    GroupedSlots g{0u, 0u};
+    if (ext) g.ext = s_base[0] + s_off[ext_key * 4 + wave] + prefix_in_mask(ext_mask);
+    if (sh) g.shadow = s_base[1] + s_off[32 + sh_key * 4 + wave] + prefix_in_mask(sh_mask);
+    __syncthreads();  // s_off / s_base are reused by the next call
+    return g;
+}
+
 __device__ __forceinline__ StackView make_stack(uint32_t* lds_ref, float* lds_t, const Pool& P) {
     StackView s;
     s.lds_ref   = lds_ref;
@@ -344,13 +398,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     uint32_t*       next_n  = P.counters + kCtrExt + iter + 1;
     uint32_t*       sh_n    = P.counters + kCtrShadow + iter;
     const uint32_t  stride  = gridDim.x * blockDim.x;
-    const uint32_t  wave0   = blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
     const uint32_t  depth   = iter;
     uint32_t        n_scatter = 0, n_rr = 0, n_shadow = 0;
-    for (uint32_t wb = wave0; wb < n; wb += stride) {
-        const uint32_t i        = wb + lane_id();
+    __shared__ uint32_t s_off[64];
+    __shared__ uint32_t s_base[2];
+    // block-uniform loop: the grouped compaction below synchronises the block
+    for (uint32_t bb = blockIdx.x * blockDim.x; bb < n; bb += stride) {
+        const uint32_t i        = bb + threadIdx.x;
         bool           cont     = false;
         bool           emit     = false;
+        uint32_t       ext_key  = 0, sh_key = 0;
         uint32_t       path     = 0;
         float4         so = make_float4(0, 0, 0, 0), sd = so, sc4 = so;
         if (i < n) {
@@ -475,7 +532,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
                                 float weight = 1.0f;
                                 if (mode == 2u && !ls.discrete) weight = power_heuristic(ls.probability, bsdf_pdf_n(bp, won, win), bad);
                                 const F3 c = (((beta * bsdf_eval_n(bp, won, win)) * wi.z) * ls.radiance) * (weight / ls.probability);
-                                emit = true;
+                                emit   = true;
+                                sh_key = octant(sdir.x, sdir.y, sdir.z);
                                 so   = make_float4(origin.x, origin.y, origin.z, limit);
                                 sd   = make_float4(sdir.x, sdir.y, sdir.z, 0.0f);
                                 sc4  = make_float4(c.x, c.y, c.z, 0.0f);
@@ -507,7 +565,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
                     }
                     beta = beta / survival;
                 }
-                cont = true;
+                cont    = true;
+                ext_key = octant(nd.x, nd.y, nd.z);
                 P.ray_o[path] = make_float4(no.x, no.y, no.z, 0.0f);
                 P.ray_d[path] = make_float4(nd.x, nd.y, nd.z, __builtin_inff());
                 P.beta[path]  = make_float4(beta.x, beta.y, beta.z, prev_pdf);
@@ -519,11 +578,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
             P.rad[path] = make_float4(L.x, L.y, L.z, eta_scale);
             if (bad) atomicOr(job.error_flag, 1u);
         }
-        const uint64_t cmask = __ballot(cont);
-        const uint32_t cpos  = wave_reserve(next_n, cmask);
+        const GroupedSlots g    = block_reserve_grouped(next_n, cont, ext_key, sh_n, emit, sh_key, s_off, s_base);
+        const uint32_t     cpos = g.ext, spos = g.shadow;
         if (cont) next_q[cpos] = path;
-        const uint64_t smask = __ballot(emit);
-        const uint32_t spos  = wave_reserve(sh_n, smask);
         if (emit) {
             P.q_sh[spos] = path;
             P.sh_o[spos] = so;
