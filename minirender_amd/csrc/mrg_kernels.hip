@@ -184,11 +184,13 @@ __device__ __forceinline__ StackView make_stack(uint32_t* lds_ref, float* lds_t,
 }
 
 // Persistent traversal with dynamic refill (used for the incoherent ray-query batches): every lane
-// keeps its own ray; whenever at least kRefillLanes lanes of the wave are idle, they reserve the next
-// entries with one atomic per wave and start new rays, so a wave does not wait for its slowest ray.
+// keeps its own ray; whenever at least kRefillLanes lanes of the wave are idle, they take the next
+// entries of the wave's claimed range (one atomic per kRefillClaim entries) and start new rays, so a
+// wave does not wait for its slowest ray.
 // (The path tracer's extension/shadow kernels keep the tight per-ray loop: measured faster there.)
 // `start(i, st, scene)` initialises item i; `finish(i, st)` consumes its result.
-constexpr int kRefillLanes = 32;
+constexpr int      kRefillLanes = 32;
+constexpr uint32_t kRefillClaim = 128;  // entries per atomic claim (>= 64; 128 measured best, +61% over an atomic per refill)
 
 template <bool ANY, bool STATS, class Start, class Finish>
 __device__ __forceinline__ void persistent_trace(const StackView& stk, uint32_t n, uint32_t* work, Start start, Finish finish,
@@ -198,11 +200,28 @@ __device__ __forceinline__ void persistent_trace(const StackView& stk, uint32_t 
     bool       active  = false;
     bool       drained = false;  // wave-uniform: the queue has no entries left
     uint32_t   item    = 0;
+    uint32_t   cur = 0, end = 0;  // wave-uniform: the unused part of the wave's claimed range
     for (;;) {
         const uint64_t idle = __ballot(!active);
         if (!drained && __builtin_popcountll(idle) >= kRefillLanes) {
-            const uint32_t slot = wave_reserve(work, idle);
-            drained             = __ballot(!active && slot >= n) != 0;
+            // idle lanes take the next entries of the wave's range; a new range of kRefillClaim
+            // entries is claimed (one atomic) when the current one runs short
+            const uint32_t need  = static_cast<uint32_t>(__builtin_popcountll(idle));
+            const uint32_t avail = end - cur;
+            uint32_t       nb    = 0;
+            if (avail < need) {
+                if (lane_id() == 0) nb = atomicAdd(work, kRefillClaim);
+                nb = __shfl(nb, 0, 64);
+            }
+            const uint32_t j    = prefix_in_mask(idle);
+            const uint32_t slot = j < avail ? cur + j : nb + (j - avail);
+            if (avail < need) {
+                cur = nb + (need - avail);
+                end = nb + kRefillClaim;
+            } else {
+                cur += need;
+            }
+            drained = __ballot(!active && slot >= n) != 0;
             if (!active && slot < n) {
                 item   = slot;
                 ts     = start(item, st);
