@@ -371,7 +371,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--path-capacity", type=int, default=0, help="paths in flight per batch (0 = library default)")
-    ap.add_argument("--traversal", default="auto", choices=["auto", "split", "fused"])
+    ap.add_argument("--traversal", default="auto", choices=["auto", "split", "fused", "refill"])
     ap.add_argument("--cpu-stride", type=int, default=1,
                     help="intersect workload: the FP64 oracle checks and times every k-th ray")
     ap.add_argument("--graph", action="store_true", help="capture the bounce loop in a hipGraph (no per-kernel times)")

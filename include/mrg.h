@@ -61,8 +61,9 @@ enum { MRG_MODE_BSDF = 0, MRG_MODE_LIGHT = 1, MRG_MODE_MIS = 2 };
  */
 enum { MRG_SCHEDULE_PLAIN = 0, MRG_SCHEDULE_BATCHED = 1, MRG_SCHEDULE_PLAIN_FILL = 2 };
 /* Traversal kernels of a bounce: split = one closest-hit and one any-hit launch per bounce; fused =
- * one launch tracing the previous bounce's shadow rays together with this bounce's extension rays. */
-enum { MRG_TRAVERSAL_AUTO = 0, MRG_TRAVERSAL_SPLIT = 1, MRG_TRAVERSAL_FUSED = 2 };
+ * one launch tracing the previous bounce's shadow rays together with this bounce's extension rays;
+ * refill = the fused launch where a lane that finishes its ray takes the next queue entry at once. */
+enum { MRG_TRAVERSAL_AUTO = 0, MRG_TRAVERSAL_SPLIT = 1, MRG_TRAVERSAL_FUSED = 2, MRG_TRAVERSAL_REFILL = 3 };
 
 typedef struct mrg_device mrg_device;
 typedef struct mrg_scene  mrg_scene;

@@ -86,7 +86,7 @@ class Device:
         o.max_jobs_in_flight = max_jobs_in_flight
         o.use_graph = int(bool(use_graph))
         o.persistent_blocks_per_cu = persistent_blocks_per_cu
-        o.traversal = {"auto": 0, "split": 1, "fused": 2}[traversal]
+        o.traversal = {"auto": 0, "split": 1, "fused": 2, "refill": 3}[traversal]
         h = C.c_void_p()
         N.check(L.mrg_device_open(ordinal, C.byref(o), C.byref(h)))
         self.handle = h

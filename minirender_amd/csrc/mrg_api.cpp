@@ -663,6 +663,11 @@ BatchPlan plan_batch_locked(mrg_device* d) {
 
 bool fused_traversal(const mrg_device* d) { return d->opt.traversal != MRG_TRAVERSAL_SPLIT; }
 
+using TraceLauncher = hipError_t (*)(const Pool&, const GpuJob*, uint32_t, const LaunchShape&, hipStream_t);
+TraceLauncher trace_launcher(const mrg_device* d) {
+    return d->opt.traversal == MRG_TRAVERSAL_REFILL ? launch_trace_refill : launch_trace;
+}
+
 void launch_iteration_kernels(mrg_device* d, uint32_t iters, bool timed, std::vector<std::pair<int, int>>& marks) {
     const GpuJob*   jobs   = d->jobs_buf.as<GpuJob>();
     const GpuChunk* chunks = d->chunks_buf.as<GpuChunk>();
@@ -679,14 +684,14 @@ void launch_iteration_kernels(mrg_device* d, uint32_t iters, bool timed, std::ve
     rec(0);
     if (fused_traversal(d)) {
         for (uint32_t it = 0; it < iters; ++it) {
-            check_hip(launch_trace(d->pool, jobs, it, d->shape, d->stream), "k_trace");
+            check_hip(trace_launcher(d)(d->pool, jobs, it, d->shape, d->stream), "k_trace");
             rec(1);
             check_hip(launch_shade(d->pool, jobs, it, d->shape, d->stream), "k_shade");
             rec(2);
         }
         // the last shading pass's shadow rays (its extension queue is empty: no path continues
         // beyond the batch's deepest bounce)
-        check_hip(launch_trace(d->pool, jobs, iters, d->shape, d->stream), "k_trace");
+        check_hip(trace_launcher(d)(d->pool, jobs, iters, d->shape, d->stream), "k_trace");
         rec(1);
     } else {
         for (uint32_t it = 0; it < iters; ++it) {
@@ -914,7 +919,7 @@ int mrg_device_open(int ordinal, const mrg_device_options* options, mrg_device**
         if (d->opt.path_capacity == 0)
             d->opt.path_capacity = d->props.totalGlobalMem >= (uint64_t(64) << 30) ? uint64_t(1) << 26 : uint64_t(1) << 23;
         if (d->opt.path_capacity > (uint64_t(1) << 30)) throw std::invalid_argument("path capacity too large");
-        if (d->opt.traversal > MRG_TRAVERSAL_FUSED) throw std::invalid_argument("unknown traversal kernel mode");
+        if (d->opt.traversal > MRG_TRAVERSAL_REFILL) throw std::invalid_argument("unknown traversal kernel mode");
         if (d->opt.max_jobs_in_flight == 0) d->opt.max_jobs_in_flight = 64;
         if (d->opt.max_jobs_in_flight > 4096) throw std::invalid_argument("max_jobs_in_flight too large");
         alloc_pool(d.get());

@@ -183,21 +183,21 @@ __device__ __forceinline__ StackView make_stack(uint32_t* lds_ref, float* lds_t,
     return s;
 }
 
-// Persistent traversal with dynamic refill (used for the incoherent ray-query batches): every lane
-// keeps its own ray; whenever at least kRefillLanes lanes of the wave are idle, they take the next
-// entries of the wave's claimed range (one atomic per kRefillClaim entries) and start new rays, so a
-// wave does not wait for its slowest ray.
-// (The path tracer's extension/shadow kernels keep the tight per-ray loop: measured faster there.)
-// `start(i, st, scene)` initialises item i; `finish(i, st)` consumes its result.
+// Persistent traversal with dynamic refill: every lane keeps its own ray; whenever at least
+// kRefillLanes lanes of the wave are idle, they take the next entries of the wave's claimed range
+// (one atomic per kRefillClaim entries) and start new rays, so a wave does not wait for its slowest
+// ray. `start(i, st, any)` initialises item i and says whether it is an any-hit query (lanes of one
+// wave may trace both kinds); `finish(i, st)` consumes its result.
 constexpr int      kRefillLanes = 32;
 constexpr uint32_t kRefillClaim = 128;  // entries per atomic claim (>= 64; 128 measured best, +61% over an atomic per refill)
 
-template <bool ANY, bool STATS, class Start, class Finish>
-__device__ __forceinline__ void persistent_trace(const StackView& stk, uint32_t n, uint32_t* work, Start start, Finish finish,
-                                                 uint64_t& nb, uint64_t& nt) {
+template <bool STATS, class Start, class Finish>
+__device__ __forceinline__ void persistent_trace_mixed(const StackView& stk, uint32_t n, uint32_t* work, Start start, Finish finish,
+                                                       uint64_t& nb, uint64_t& nt) {
     TraceState st;
     TraceScene ts{nullptr, nullptr, 0};
     bool       active  = false;
+    bool       any     = false;
     bool       drained = false;  // wave-uniform: the queue has no entries left
     uint32_t   item    = 0;
     uint32_t   cur = 0, end = 0;  // wave-uniform: the unused part of the wave's claimed range
@@ -224,16 +224,29 @@ __device__ __forceinline__ void persistent_trace(const StackView& stk, uint32_t 
             drained = __ballot(!active && slot >= n) != 0;
             if (!active && slot < n) {
                 item   = slot;
-                ts     = start(item, st);
+                ts     = start(item, st, any);
                 active = true;
             }
         }
         if (__ballot(active) == 0) break;
-        if (active && trace_step<ANY, STATS>(ts, st, stk, nb, nt)) {
+        if (active && trace_step_dyn<STATS>(ts, st, stk, nb, nt, any)) {
             finish(item, st);
             active = false;
         }
     }
+}
+
+// Every item of one kind: closest hit (ANY = false) or any hit (ANY = true).
+template <bool ANY, bool STATS, class Start, class Finish>
+__device__ __forceinline__ void persistent_trace(const StackView& stk, uint32_t n, uint32_t* work, Start start, Finish finish,
+                                                 uint64_t& nb, uint64_t& nt) {
+    persistent_trace_mixed<STATS>(
+        stk, n, work,
+        [&](uint32_t i, TraceState& st, bool& any) {
+            any = ANY;
+            return start(i, st);
+        },
+        finish, nb, nt);
 }
 
 // Reported distance of the winning hit: the ray/plane solve of that one triangle in FP64
@@ -439,6 +452,56 @@ __global__ void __launch_bounds__(256) k_trace(Pool P, const GpuJob* __restrict_
             if (i < n_ext) extend_one(P, jobs, queue, i, stk);
         }
     }
+}
+
+// Fused traversal of one bounce with the refill driver: the same work as k_trace (shadow rays of
+// bounce iter - 1 first, then the extension rays of bounce iter), but lanes that finish their ray
+// take the next queue entry at once, so a wave can trace shadow and extension rays side by side.
+__global__ void __launch_bounds__(256) k_trace_refill(Pool P, const GpuJob* __restrict__ jobs, uint32_t iter) {
+    __shared__ uint32_t s_ref[kStackLds * 256];
+    __shared__ float    s_t[kStackLds * 256];
+    const StackView stk   = make_stack(s_ref, s_t, P);
+    const uint32_t  n_ext = P.counters[kCtrExt + iter];
+    const uint32_t  n_sh  = iter ? P.counters[kCtrShadow + iter - 1] : 0u;
+    const uint32_t* queue = (iter & 1u) ? P.q_ext1 : P.q_ext0;
+    uint32_t*       work  = P.counters + kCtrWorkExt + iter;
+    uint64_t        nb = 0, nt = 0;
+    uint32_t        path = 0;
+    persistent_trace_mixed<false>(
+        stk, n_sh + n_ext, work,
+        [&](uint32_t i, TraceState& st, bool& any) {
+            any = i < n_sh;
+            if (any) {
+                path           = P.q_sh[i];
+                const float4 o = P.sh_o[i];
+                const float4 d = P.sh_d[i];
+                trace_init(st, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), 0.0f, o.w);
+            } else {
+                path            = queue[i - n_sh];
+                const float4 ro = P.ray_o[path];
+                const float4 rd = P.ray_d[path];
+                trace_init(st, mk(ro.x, ro.y, ro.z), mk(rd.x, rd.y, rd.z), ro.w, rd.w);
+            }
+            const uint32_t job = __float_as_uint(P.prev[path].w) >> 16;
+            return trace_scene(jobs[job].scene);
+        },
+        [&](uint32_t i, const TraceState& st) {
+            const TraceResult& res = st.res;
+            if (i < n_sh) {
+                if (res.slot == 0xffffffffu) {
+                    const float4 c = P.sh_c[i];
+                    float4       l = P.rad[path];
+                    l.x = l.x + c.x;
+                    l.y = l.y + c.y;
+                    l.z = l.z + c.z;
+                    P.rad[path] = l;
+                }
+            } else {
+                P.hit[path]      = make_float4(res.t, res.b0, res.b1, res.b2);
+                P.hit_slot[path] = res.slot;
+            }
+        },
+        nb, nt);
 }
 
 // Shading of one bounce (PathTracer::trace loop body, path_tracer.cpp:164-363).
@@ -759,6 +822,11 @@ hipError_t launch_shade(const Pool& P, const GpuJob* jobs, uint32_t iter, const 
 
 hipError_t launch_trace(const Pool& P, const GpuJob* jobs, uint32_t iter, const LaunchShape& shape, hipStream_t stream) {
     hipLaunchKernelGGL(k_trace, dim3(shape.traversal_blocks), dim3(256), 0, stream, P, jobs, iter);
+    return hipGetLastError();
+}
+
+hipError_t launch_trace_refill(const Pool& P, const GpuJob* jobs, uint32_t iter, const LaunchShape& shape, hipStream_t stream) {
+    hipLaunchKernelGGL(k_trace_refill, dim3(shape.traversal_blocks), dim3(256), 0, stream, P, jobs, iter);
     return hipGetLastError();
 }
 

@@ -215,11 +215,12 @@ __device__ __forceinline__ void trace_init(TraceState& st, F3 o, F3 d, float tmi
 }
 
 // One traversal step: an inner node (its four boxes) or a leaf (its triangles), then a pop if the
-// lane has no child to descend into. Returns true once the ray is finished: closest hit (ANY =
-// false) or any hit (ANY = true) in [tmin, tmax], result in st.res.
-template <bool ANY, bool STATS>
-__device__ __forceinline__ bool trace_step(const TraceScene& sc, TraceState& st, const StackView& stk, uint64_t& bounds_tests,
-                                           uint64_t& tri_tests) {
+// lane has no child to descend into. Returns true once the ray is finished: closest hit (any =
+// false) or any hit (any = true) in [tmin, tmax], result in st.res. `any` may differ between the
+// lanes of a wave (the mixed refill driver); trace_step<ANY> passes a constant.
+template <bool STATS>
+__device__ __forceinline__ bool trace_step_dyn(const TraceScene& sc, TraceState& st, const StackView& stk, uint64_t& bounds_tests,
+                                               uint64_t& tri_tests, const bool any) {
     if (sc.node_count == 0) return true;
     const RayPre& r = st.r;
     if (!is_leaf(st.ref)) {
@@ -276,7 +277,7 @@ __device__ __forceinline__ bool trace_step(const TraceScene& sc, TraceState& st,
                 st.res.det  = h.det;
                 st.res.slot = slot;
                 st.res.prim = prim;
-                if (ANY) {
+                if (any) {
                     finish_barycentrics(st.res);
                     return true;
                 }
@@ -293,6 +294,12 @@ __device__ __forceinline__ bool trace_step(const TraceScene& sc, TraceState& st,
         stack_pop(stk, st.sp, st.ref, t);
         if (!(t > st.res.t)) return false;
     }
+}
+
+template <bool ANY, bool STATS>
+__device__ __forceinline__ bool trace_step(const TraceScene& sc, TraceState& st, const StackView& stk, uint64_t& bounds_tests,
+                                           uint64_t& tri_tests) {
+    return trace_step_dyn<STATS>(sc, st, stk, bounds_tests, tri_tests, ANY);
 }
 
 // Closest hit (ANY = false) or any hit (ANY = true) in [tmin, tmax].

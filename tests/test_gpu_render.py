@@ -57,3 +57,20 @@ def test_constant_environment_floor_is_albedo(device):
     scene = scenegen.floor_scene(color=0.6)
     img, _ = api.render(device, scene, api.RenderSettings(8, 8, 4, 81), api.IntegratorSettings("bsdf", 1, False, 3))
     np.testing.assert_allclose(img, 0.6, atol=1e-6)
+
+
+@pytest.mark.parametrize("traversal", ["split", "fused", "refill"])
+def test_traversal_kernels_agree_with_fp32_twin(oracle, traversal):
+    """Every traversal kernel arrangement (split closest/any-hit launches, the fused bounce launch and
+    the fused launch with per-lane refill) gives the FP32 twin's image, bit for bit across modes.
+    A small path pool forces many batches, so queues of every length are traced."""
+    scene = scenegen.mixed_terrain(cells=48, spheres=16, sphere_level=2)
+    print(f"[render parity] traversal={traversal} mixed 48x27 spp=6 depth=12", flush=True)
+    rs, ins = api.RenderSettings(48, 27, 6, 11), api.IntegratorSettings("mis", 12, True, 3)
+    with api.Device(0, path_capacity=1 << 12, traversal=traversal) as dev:
+        img, prog = api.render(dev, scene, rs, ins)
+    twin, _, st = oracle.OracleScene(scene).render(scene.camera, 48, 27, 6, seed=11, mode="mis", max_depth=12, rr=True,
+                                                   precision=32, sums=True)
+    _check(img, twin)
+    assert prog.statistics["shadow_rays"] == st["shadow_rays"]
+    assert prog.statistics["extension_rays"] == st["extension_rays"]
