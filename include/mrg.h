@@ -69,7 +69,10 @@ typedef struct mrg_device mrg_device;
 typedef struct mrg_scene  mrg_scene;
 typedef struct mrg_job    mrg_job;
 
-/* Material (scene.hpp:108-129) of one instance; 44 bytes. */
+/* TextureFilter (texture.hpp:13). */
+enum { MRG_FILTER_NEAREST = 0, MRG_FILTER_BILINEAR = 1, MRG_FILTER_TRILINEAR = 2 };
+
+/* Material (scene.hpp:108-129) of one instance; 56 bytes. */
 typedef struct mrg_material {
     uint32_t kind;
     float    base_color[3];
@@ -77,8 +80,19 @@ typedef struct mrg_material {
     float    roughness;
     float    ior;
     uint32_t double_sided;
-    uint32_t mirrored; /* determinant(object_to_world) < 0 (bvh.cpp:63) */
+    uint32_t mirrored;           /* determinant(object_to_world) < 0 (bvh.cpp:63) */
+    uint32_t compensate_energy;  /* Kulla-Conty lobe of rough conductors (scene.hpp:125-126, bsdf.cpp:128) */
+    uint32_t base_color_texture; /* 1 + index into mrg_scene_desc.textures; 0 = none (RenderInstance::base_color_texture) */
+    uint32_t texture_filter;     /* MRG_FILTER_* */
 } mrg_material;
+
+/* Decoded linear-RGB texture, level 0 of the reference's mip chain (Texture, texture.hpp:22-40):
+ * row 0 is the top row of the image, as the reference's Image. */
+typedef struct mrg_texture {
+    uint32_t     width;
+    uint32_t     height;
+    const float* rgb; /* [height][width][3] */
+} mrg_texture;
 
 /* DirectionalLight (scene.hpp:180-186): direction is the propagation direction. */
 typedef struct mrg_directional_light {
@@ -114,6 +128,12 @@ typedef struct mrg_scene_desc {
     float                        environment[3];
     uint32_t                     directional_count;
     const mrg_directional_light* directional;
+    const float*                 uvs;           /* [n][3][2] per-corner texture coordinates, or NULL (all zero) */
+    uint32_t                     texture_count;
+    const mrg_texture*           textures;      /* [texture_count] */
+    uint32_t                     environment_texture; /* EnvironmentMap (scene.hpp:188-193): 1 + texture index, 0 = none */
+    float                        environment_strength;
+    float                        environment_rotation_degrees;
 } mrg_scene_desc;
 
 typedef struct mrg_scene_info {
@@ -270,6 +290,13 @@ int mrg_device_memory(mrg_device* device, uint64_t* budget_bytes, uint64_t* resi
  * Diagnostic of the builder behind mrg_scene_upload (the BvhBuildStatistics of bvh.hpp:29-34).
  */
 int mrg_bvh_build_host(const float* positions, uint64_t n, uint64_t* out);
+
+/*
+ * The Kulla-Conty energy table the compensated conductors use (conductor_energy_table(),
+ * microfacet_energy.hpp:14-15; baked as bake_ibl_brdf() at its defaults, ibl.hpp:39-45, 72):
+ * out [64][64][3] = (A, B, average) per (roughness row, cos(theta) column). Host only.
+ */
+int mrg_conductor_energy_table(float* out);
 
 #ifdef __cplusplus
 }

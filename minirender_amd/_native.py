@@ -5,6 +5,8 @@ from __future__ import annotations
 import ctypes as C
 import os
 
+import numpy as np
+
 LIB_PATH = os.environ.get("MRG_LIBRARY") or os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libmrg.so")
 
 MRG_OK, MRG_INVALID_ARGUMENT, MRG_CANCELLED, MRG_NOT_ADMITTED, MRG_NONFINITE, MRG_DEVICE_ERROR, MRG_TIMEOUT = range(7)
@@ -30,7 +32,12 @@ class NotAdmitted(MrgError):
 
 class Material(C.Structure):
     _fields_ = [("kind", C.c_uint32), ("base_color", C.c_float * 3), ("emission", C.c_float * 3),
-                ("roughness", C.c_float), ("ior", C.c_float), ("double_sided", C.c_uint32), ("mirrored", C.c_uint32)]
+                ("roughness", C.c_float), ("ior", C.c_float), ("double_sided", C.c_uint32), ("mirrored", C.c_uint32),
+                ("compensate_energy", C.c_uint32), ("base_color_texture", C.c_uint32), ("texture_filter", C.c_uint32)]
+
+
+class Texture(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("rgb", C.c_void_p)]
 
 
 class Camera(C.Structure):
@@ -43,7 +50,9 @@ class SceneDesc(C.Structure):
     _fields_ = [("triangle_count", C.c_uint64), ("positions", C.c_void_p), ("normals", C.c_void_p),
                 ("instance", C.c_void_p), ("triangle", C.c_void_p), ("instance_count", C.c_uint32),
                 ("materials", C.c_void_p), ("environment", C.c_float * 3), ("directional_count", C.c_uint32),
-                ("directional", C.c_void_p)]
+                ("directional", C.c_void_p), ("uvs", C.c_void_p), ("texture_count", C.c_uint32),
+                ("textures", C.c_void_p), ("environment_texture", C.c_uint32), ("environment_strength", C.c_float),
+                ("environment_rotation_degrees", C.c_float)]
 
 
 class SceneInfo(C.Structure):
@@ -119,6 +128,7 @@ SIGNATURES = {
     "mrg_device_kernel_times": (C.c_int, [P, C.POINTER(KernelTimes), C.c_int]),
     "mrg_bvh_build_host": (C.c_int, [P, C.c_uint64, P]),
     "mrg_device_memory": (C.c_int, [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "mrg_conductor_energy_table": (C.c_int, [P]),
 }
 
 _lib = None
@@ -148,3 +158,37 @@ def check(rc):
     if rc == MRG_NOT_ADMITTED:
         raise NotAdmitted(rc, msg)
     raise MrgError(rc, msg)
+
+
+def fill_scene_desc(d, scene, keep):
+    """Fill a scene descriptor (SceneDesc here, or the oracle's layout-identical one) from a Scene;
+    `keep` collects the arrays whose memory the descriptor points into."""
+    arrays = [np.ascontiguousarray(scene.positions, np.float32), np.ascontiguousarray(scene.normals, np.float32),
+              np.ascontiguousarray(scene.instance, np.uint32), np.ascontiguousarray(scene.triangle, np.uint32),
+              np.ascontiguousarray(scene.materials), np.ascontiguousarray(scene.directional)]
+    keep.extend(arrays)
+    ptr = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+    d.triangle_count = scene.triangle_count
+    d.positions, d.normals, d.instance, d.triangle = (ptr(a) for a in arrays[:4])
+    d.instance_count = len(scene.materials)
+    d.materials = ptr(arrays[4])
+    d.environment[:] = list(scene.environment)
+    d.directional_count = len(scene.directional)
+    d.directional = ptr(arrays[5])
+    if scene.uvs is not None:
+        uv = np.ascontiguousarray(scene.uvs, np.float32)
+        keep.append(uv)
+        d.uvs = ptr(uv)
+    if scene.textures:
+        tex = (Texture * len(scene.textures))()
+        for i, t in enumerate(scene.textures):
+            t = np.ascontiguousarray(t, np.float32)
+            keep.append(t)
+            tex[i].height, tex[i].width = t.shape[0], t.shape[1]
+            tex[i].rgb = ptr(t)
+        keep.append(tex)
+        d.texture_count = len(scene.textures)
+        d.textures = C.cast(tex, C.c_void_p)
+    d.environment_texture = scene.environment_texture + 1
+    d.environment_strength = scene.environment_strength
+    d.environment_rotation_degrees = scene.environment_rotation_degrees

@@ -133,17 +133,9 @@ class SceneBvh:
     def __init__(self, device: Device, scene: Scene):
         self.device = device
         self.scene = scene
-        keep = [np.ascontiguousarray(scene.positions, np.float32), np.ascontiguousarray(scene.normals, np.float32),
-                np.ascontiguousarray(scene.instance, np.uint32), np.ascontiguousarray(scene.triangle, np.uint32),
-                np.ascontiguousarray(scene.materials), np.ascontiguousarray(scene.directional)]
+        keep = []
         d = N.SceneDesc()
-        d.triangle_count = scene.triangle_count
-        d.positions, d.normals, d.instance, d.triangle = (_ptr(a) for a in keep[:4])
-        d.instance_count = len(scene.materials)
-        d.materials = _ptr(keep[4])
-        d.environment[:] = list(scene.environment)
-        d.directional_count = len(scene.directional)
-        d.directional = _ptr(keep[5])
+        N.fill_scene_desc(d, scene, keep)
         h = C.c_void_p()
         N.check(N.lib().mrg_scene_upload(device.handle, C.byref(d), C.byref(h)))
         self.handle = h

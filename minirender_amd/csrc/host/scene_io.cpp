@@ -289,6 +289,14 @@ std::string string_or(const Json& obj, const char* key) {
     return v ? v->string() : std::string();
 }
 
+uint32_t texture_filter(const std::string& name) {
+    // parse_texture_filter (texture.cpp:139-151)
+    if (name.empty() || name == "trilinear") return MRG_FILTER_TRILINEAR;
+    if (name == "nearest") return MRG_FILTER_NEAREST;
+    if (name == "bilinear") return MRG_FILTER_BILINEAR;
+    throw std::invalid_argument("Unknown texture filter: " + name);
+}
+
 uint32_t material_kind(const std::string& name) {
     // scene.cpp:219-239; the GPU integrator renders the first four
     if (name == "diffuse") return 0;
@@ -340,8 +348,8 @@ SceneDocument load_scene(const std::filesystem::path& path) {
                 throw std::invalid_argument("Material " + id + ": cloth / subsurface data is not supported by the GPU integrator");
             if (!string_or(v, "base_color_texture").empty() || !string_or(v, "normal_texture").empty())
                 throw std::invalid_argument("Material " + id + ": textures are not supported by the GPU integrator");
-            if (bool_or(v, "compensate_energy", false))
-                throw std::invalid_argument("Material " + id + ": energy compensation is not supported by the GPU integrator");
+            m.compensate_energy = bool_or(v, "compensate_energy", false) ? 1u : 0u;
+            m.texture_filter    = texture_filter(string_or(v, "texture_filter"));
             doc.material_ids.push_back(id);
             doc.materials.push_back(m);
         }

@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "mrg_bvh.h"
+#include "mrg_energy.h"
 #include "mrg_kernels.h"
 #include "mrg_layout.h"
 
@@ -183,6 +184,7 @@ struct mrg_device {
     Pool               pool{};
     DevBuf             pool_mem, spill_ref, spill_t, counters, stats, header, jobs_buf, chunks_buf, query_work;
     DevBuf             query_spill_ref, query_spill_t;
+    DevBuf             energy;  // conductor energy table (mrg_energy.h), float4 per node
     std::mutex         query_mutex;
     uint32_t           max_chunks = 0, max_jobs = 0;
     uint64_t           pool_bytes = 0;
@@ -316,7 +318,14 @@ mrg_scene* build_scene(mrg_device* d, const mrg_scene_desc& s, const std::string
         }
         if (!(m.roughness >= 0 && m.roughness <= 1)) throw std::invalid_argument("Roughness must be in [0,1]");
         if (!(m.ior > 0 && std::isfinite(m.ior))) throw std::invalid_argument("IOR must be positive");
+        if (m.compensate_energy > 1) throw std::invalid_argument("compensate_energy must be 0 or 1");
+        if (m.texture_filter > MRG_FILTER_TRILINEAR) throw std::invalid_argument("Unknown texture filter");
+        if (m.base_color_texture > s.texture_count) throw std::invalid_argument("Material references missing texture");
+        if (m.base_color_texture != 0) throw std::invalid_argument("base-color textures are not supported yet");
     }
+    if (s.texture_count && !s.textures) throw std::invalid_argument("textures are missing");
+    if (s.environment_texture > s.texture_count) throw std::invalid_argument("Environment references missing texture");
+    if (s.environment_texture != 0) throw std::invalid_argument("environment maps are not supported yet");
     auto sc    = std::make_unique<mrg_scene>();
     sc->device = d;
     sc->hash   = hash;
@@ -431,6 +440,10 @@ mrg_scene* build_scene(mrg_device* d, const mrg_scene_desc& s, const std::string
         g.ior          = m.ior;
         g.double_sided = m.double_sided;
         g.mirrored     = m.mirrored;
+        g.compensate   = m.compensate_energy;
+        g.texture      = int32_t(m.base_color_texture) - 1;
+        g.filter       = m.texture_filter;
+        g.pad          = 0;
     }
     check_hip(hipSetDevice(d->ordinal), "hipSetDevice");
     upload(sc->nodes, bvh.nodes);
@@ -923,6 +936,9 @@ int mrg_device_open(int ordinal, const mrg_device_options* options, mrg_device**
         if (d->opt.max_jobs_in_flight == 0) d->opt.max_jobs_in_flight = 64;
         if (d->opt.max_jobs_in_flight > 4096) throw std::invalid_argument("max_jobs_in_flight too large");
         alloc_pool(d.get());
+        upload(d->energy, conductor_energy_table());
+        d->pool.energy       = d->energy.as<float>();
+        d->query_pool.energy = d->energy.as<float>();
         d->events.resize(2 * (3 * kMaxBounces + 8));
         for (auto& e : d->events) check_hip(hipEventCreate(&e), "hipEventCreate");
         for (auto& e : d->query_events) check_hip(hipEventCreate(&e), "hipEventCreate");
@@ -965,6 +981,15 @@ int mrg_device_query(mrg_device* d, char* name, size_t cap, uint32_t* cus, uint6
     });
 }
 
+int mrg_conductor_energy_table(float* out) {
+    return guarded([&] {
+        if (!out) throw std::invalid_argument("null argument");
+        const std::vector<float>& t = conductor_energy_table();
+        for (size_t i = 0; i < t.size() / 4; ++i)
+            for (int c = 0; c < 3; ++c) out[i * 3 + c] = t[i * 4 + c];
+    });
+}
+
 int mrg_scene_upload(mrg_device* d, const mrg_scene_desc* s, mrg_scene** out) {
     return guarded([&] {
         if (!d || !s || !out) throw std::invalid_argument("null argument");
@@ -982,6 +1007,19 @@ int mrg_scene_upload(mrg_device* d, const mrg_scene_desc* s, mrg_scene** out) {
         h.bytes(s->environment, 12);
         h.bytes(&s->directional_count, 4);
         if (s->directional_count) h.bytes(s->directional, s->directional_count * sizeof(mrg_directional_light));
+        const uint32_t has_uv = s->uvs ? 1u : 0u;
+        h.bytes(&has_uv, 4);
+        if (n && s->uvs) h.bytes(s->uvs, n * 24);
+        h.bytes(&s->texture_count, 4);
+        for (uint32_t i = 0; s->textures && i < s->texture_count; ++i) {
+            const mrg_texture& t = s->textures[i];
+            h.bytes(&t.width, 4);
+            h.bytes(&t.height, 4);
+            if (t.rgb) h.bytes(t.rgb, uint64_t(t.width) * t.height * 12);
+        }
+        h.bytes(&s->environment_texture, 4);
+        h.bytes(&s->environment_strength, 4);
+        h.bytes(&s->environment_rotation_degrees, 4);
         const std::string key = h.hex();
         {
             std::lock_guard<std::mutex> lock(d->mutex);

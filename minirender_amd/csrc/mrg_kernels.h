@@ -59,6 +59,7 @@ struct Pool {
     const uint32_t* header;  // kHdr* words of the current batch
     uint32_t* spill_ref;
     float*    spill_t;
+    const float* energy;  // conductor energy table [64][64][4] of the device (mrg_energy.h)
     uint32_t  capacity;
     uint32_t  spill_lanes;
 };

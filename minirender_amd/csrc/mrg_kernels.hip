@@ -614,6 +614,9 @@ __global__ void __launch_bounds__(kShadeThreads) __attribute__((amdgpu_waves_per
                 bp.alpha = mat.alpha;
                 bp.eta_i = front_face ? 1.0f : mat.ior;
                 bp.eta_t = front_face ? mat.ior : 1.0f;
+                bp.roughness = mat.roughness;
+                bp.table     = P.energy;
+                bp.energy    = mat.kind == 1u && mat.compensate != 0u && !bp.is_delta();
                 const float light_u   = next_1d(rng);
                 const float su_x      = next_1d(rng);
                 const float su_y      = next_1d(rng);
@@ -621,6 +624,7 @@ __global__ void __launch_bounds__(kShadeThreads) __attribute__((amdgpu_waves_per
                 const float du_x      = next_1d(rng);
                 const float du_y      = next_1d(rng);
                 const F3    won       = normalize(wo);
+                bsdf_prepare(bp, won);
                 if (!bp.is_delta()) {
                     LightSample ls;
                     if (sample_light(sc, position, light_u, su_x, su_y, ls) && (mode != 0u || ls.discrete)) {

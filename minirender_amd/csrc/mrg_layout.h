@@ -72,6 +72,10 @@ struct GpuMaterial {
     float    ior;
     uint32_t double_sided;
     uint32_t mirrored;
+    uint32_t compensate;  // Kulla-Conty lobe (rough conductors only; see BsdfParams)
+    int32_t  texture;     // base-color texture index into GpuScene::textures, -1 = none
+    uint32_t filter;      // 0 nearest, 1 bilinear, 2 trilinear (lod 0: the bilinear value)
+    uint32_t pad;
 };
 
 struct GpuAreaLight {

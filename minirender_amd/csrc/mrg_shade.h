@@ -124,25 +124,77 @@ __device__ __forceinline__ F3 sample_vndf(F3 wo, float ux, float uy, float alpha
     return normalize(mk(alpha * n.x, alpha * n.y, smax(0.0f, n.z)));
 }
 
+// --- Kulla-Conty energy compensation (microfacet_energy.cpp:69-89, bsdf.cpp:128-189)
+// Bilinear lookup of the energy table ([64][64] float4 nodes (A, B, average, 0); x = cos(theta),
+// y = roughness): returns E = clamp(A + B, 0, 1) and the row average clamped to [0, 1].
+__device__ __forceinline__ void sample_energy(const float* table, float mu, float roughness, float& e, float& average) {
+    const float    x  = sclamp(mu, 0.0f, 1.0f) * 63.0f;
+    const float    y  = sclamp(roughness, 0.0f, 1.0f) * 63.0f;
+    const uint32_t ix = static_cast<uint32_t>(x), iy = static_cast<uint32_t>(y);
+    const uint32_t jx = ix + 1u < 63u ? ix + 1u : 63u, jy = iy + 1u < 63u ? iy + 1u : 63u;
+    const float4*  t  = reinterpret_cast<const float4*>(table);
+    const float4   a = gld(t + iy * 64u + ix), b = gld(t + iy * 64u + jx);
+    const float4   c = gld(t + jy * 64u + ix), d = gld(t + jy * 64u + jx);
+    const float    tx = x - static_cast<float>(ix), ty = y - static_cast<float>(iy);
+    const float    sx = 1.0f - tx, sy = 1.0f - ty;
+    const F3       lo = mk(a.x, a.y, a.z) * sx + mk(b.x, b.y, b.z) * tx;
+    const F3       hi = mk(c.x, c.y, c.z) * sx + mk(d.x, d.y, d.z) * tx;
+    const F3       v  = lo * sy + hi * ty;
+    e                 = sclamp(v.x + v.y, 0.0f, 1.0f);
+    average           = sclamp(v.z, 0.0f, 1.0f);
+}
+
+__device__ __forceinline__ F3 multiple_scattering(F3 f0, float ei, float eo, float average) {
+    if (average >= 1.0f - 1e-6f) return mk(0.0f, 0.0f, 0.0f);
+    const F3 one   = mk(1.0f, 1.0f, 1.0f);
+    const F3 f     = f0 + (one - f0) / 21.0f;
+    const F3 den   = one - f * (1.0f - average);
+    const F3 color = ((f * f) * average) / mk(smax(den.x, 1e-8f), smax(den.y, 1e-8f), smax(den.z, 1e-8f));
+    return color * (((1.0f - ei) * (1.0f - eo)) / (kPiF * (1.0f - average)));
+}
+
+// BSDF of one shading point. With energy compensation the outgoing direction's table values (E(wo),
+// the row average and the single-lobe probability clamp(E(wo), 0.05, 0.95), bsdf.cpp:170-172) are
+// looked up once per shading point by bsdf_prepare: every eval / pdf / sample of the point uses the
+// same normalised wo.
 struct BsdfParams {
-    uint32_t kind;
-    F3       color;
-    float    alpha, eta_i, eta_t;
-    __device__ __forceinline__ bool is_delta() const { return kind == 2u || kind == 3u || (kind == 1u && alpha <= 1e-6f); }
+    uint32_t     kind;
+    F3           color;
+    float        alpha, eta_i, eta_t, roughness;
+    const float* table;     // the device's energy table (kernel-uniform)
+    bool         energy;    // compensated rough conductor
+    float        e_out;     // E(wo)
+    float        average;   // row average
+    __device__ __forceinline__ bool  is_delta() const { return kind == 2u || kind == 3u || (kind == 1u && alpha <= 1e-6f); }
+    // probability of the single-scattering lobe
+    __device__ __forceinline__ float p_single() const { return sclamp(e_out, 0.05f, 0.95f); }
 };
+
+__device__ __forceinline__ void bsdf_prepare(BsdfParams& b, F3 wo) {
+    b.e_out   = 0.0f;
+    b.average = 0.0f;
+    if (b.energy) sample_energy(b.table, wo.z, b.roughness, b.e_out, b.average);
+}
 
 __device__ __forceinline__ F3 bsdf_eval_n(const BsdfParams& b, F3 wo, F3 wi) {
     if (wo.z <= 0.0f || wi.z <= 0.0f || b.is_delta()) return mk(0.0f, 0.0f, 0.0f);
     if (b.kind == 0u) return b.color / kPiF;
-    const F3 m = normalize(wo + wi);
-    return (((schlick(b.color, dot(wo, m)) * ggx_d(m, b.alpha)) * g1_over_cos(wo, b.alpha)) * g1_over_cos(wi, b.alpha)) / 4.0f;
+    const F3 m      = normalize(wo + wi);
+    const F3 single = (((schlick(b.color, dot(wo, m)) * ggx_d(m, b.alpha)) * g1_over_cos(wo, b.alpha)) * g1_over_cos(wi, b.alpha)) / 4.0f;
+    if (!b.energy) return single;
+    float ei, unused;
+    sample_energy(b.table, wi.z, b.roughness, ei, unused);
+    return single + multiple_scattering(b.color, ei, b.e_out, b.average);
 }
 
 __device__ __forceinline__ float bsdf_pdf_n(const BsdfParams& b, F3 wo, F3 wi) {
     if (wo.z <= 0.0f || wi.z <= 0.0f || b.is_delta()) return 0.0f;
     if (b.kind == 0u) return smax(0.0f, wi.z) / kPiF;
-    const F3 m = normalize(wo + wi);
-    return ggx_d(m, b.alpha) * g1_over_cos(wo, b.alpha) / 4.0f;
+    const F3    m      = normalize(wo + wi);
+    const float glossy = ggx_d(m, b.alpha) * g1_over_cos(wo, b.alpha) / 4.0f;
+    if (!b.energy) return glossy;
+    const float p = b.p_single();
+    return p * glossy + (1.0f - p) * (smax(0.0f, wi.z) / kPiF);
 }
 
 struct BsdfSample {
@@ -185,7 +237,7 @@ __device__ __forceinline__ bool bsdf_sample(const BsdfParams& b, F3 wo, float co
         return true;
     }
     F3 wi;
-    if (b.kind == 0u) {
+    if (b.kind == 0u || (b.energy && component >= b.p_single())) {
         wi = sample_cosine_hemisphere(ux, uy);
     } else {
         const F3 m = sample_vndf(wo, ux, uy, b.alpha);

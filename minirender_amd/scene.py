@@ -19,7 +19,10 @@ import numpy as np
 DIFFUSE, CONDUCTOR, MIRROR, DIELECTRIC = 0, 1, 2, 3
 KIND_NAMES = {"diffuse": DIFFUSE, "conductor": CONDUCTOR, "mirror": MIRROR, "dielectric": DIELECTRIC}
 
-# Layout shared with `mrg_material` (include/mrg.h): 44 bytes, 4-byte aligned.
+# TextureFilter (texture.hpp:13).
+FILTERS = {"nearest": 0, "bilinear": 1, "trilinear": 2}
+
+# Layout shared with `mrg_material` (include/mrg.h): 56 bytes, 4-byte aligned.
 MATERIAL_DTYPE = np.dtype(
     [
         ("kind", "<u4"),
@@ -29,6 +32,9 @@ MATERIAL_DTYPE = np.dtype(
         ("ior", "<f4"),
         ("double_sided", "<u4"),
         ("mirrored", "<u4"),
+        ("compensate_energy", "<u4"),
+        ("base_color_texture", "<u4"),  # 1 + texture index, 0 = none
+        ("texture_filter", "<u4"),
     ]
 )
 
@@ -51,8 +57,10 @@ CAMERA_DTYPE = np.dtype(
 
 
 def material(kind="diffuse", base_color=(0.7, 0.7, 0.7), emission=(0, 0, 0), roughness=0.35, ior=1.5,
-             double_sided=True, mirrored=False):
-    """One material record with the reference defaults (Material, scene.hpp:108-129)."""
+             double_sided=True, mirrored=False, compensate_energy=False, base_color_texture=-1,
+             texture_filter="trilinear"):
+    """One material record with the reference defaults (Material, scene.hpp:108-129).
+    base_color_texture indexes Scene.textures (-1 = none)."""
     m = np.zeros((), MATERIAL_DTYPE)
     m["kind"] = KIND_NAMES[kind] if isinstance(kind, str) else int(kind)
     m["base_color"] = base_color
@@ -61,6 +69,9 @@ def material(kind="diffuse", base_color=(0.7, 0.7, 0.7), emission=(0, 0, 0), rou
     m["ior"] = ior
     m["double_sided"] = 1 if double_sided else 0
     m["mirrored"] = 1 if mirrored else 0
+    m["compensate_energy"] = 1 if compensate_energy else 0
+    m["base_color_texture"] = int(base_color_texture) + 1
+    m["texture_filter"] = FILTERS[texture_filter] if isinstance(texture_filter, str) else int(texture_filter)
     return m
 
 
@@ -113,6 +124,11 @@ class Scene:
     directional: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(0, DIRECTIONAL_DTYPE))
     camera: Camera = dataclasses.field(default_factory=Camera)
     name: str = "scene"
+    uvs: np.ndarray = None           # (n, 3, 2) float32 per-corner texture coordinates, or None
+    textures: list = dataclasses.field(default_factory=list)  # linear RGB float32 (h, w, 3), row 0 = top
+    environment_texture: int = -1    # EnvironmentMap (scene.hpp:188-193): index into textures
+    environment_strength: float = 1.0
+    environment_rotation_degrees: float = 0.0
 
     def __post_init__(self):
         self.positions = np.ascontiguousarray(self.positions, np.float32).reshape(-1, 3, 3)
@@ -125,6 +141,13 @@ class Scene:
         n = self.positions.shape[0]
         if self.normals.shape[0] != n or self.instance.shape[0] != n or self.triangle.shape[0] != n:
             raise ValueError("per-triangle arrays must have the same length")
+        if self.uvs is not None:
+            self.uvs = np.ascontiguousarray(self.uvs, np.float32).reshape(-1, 3, 2)
+            if self.uvs.shape[0] != n:
+                raise ValueError("per-triangle arrays must have the same length")
+        self.textures = [np.ascontiguousarray(t, np.float32).reshape(np.shape(t)[0], np.shape(t)[1], 3) for t in self.textures]
+        self.environment_strength = float(np.float32(self.environment_strength))
+        self.environment_rotation_degrees = float(np.float32(self.environment_rotation_degrees))
         if n and int(self.instance.max()) >= len(self.materials):
             raise ValueError("instance index out of range")
 
@@ -138,6 +161,13 @@ class Scene:
         for a in (self.positions, self.normals, self.instance, self.triangle, self.materials, self.directional):
             h.update(np.ascontiguousarray(a).tobytes())
         h.update(np.asarray(self.environment, np.float32).tobytes())
+        if self.uvs is not None:
+            h.update(self.uvs.tobytes())
+        for t in self.textures:
+            h.update(np.asarray(t.shape, np.uint32).tobytes())
+            h.update(t.tobytes())
+        h.update(np.asarray([self.environment_texture], np.int32).tobytes())
+        h.update(np.asarray([self.environment_strength, self.environment_rotation_degrees], np.float32).tobytes())
         return h.hexdigest()
 
     def device_bytes(self) -> int:

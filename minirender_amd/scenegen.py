@@ -226,6 +226,28 @@ def mixed_terrain(cells=1024, spheres=64, sphere_level=3, seed=3, width=1920, he
     return concat(parts, mats, environment=(0.35, 0.45, 0.65), directional=sun, camera=cam, name="mixed")
 
 
+def furnace_spheres(sphere_level=3, roughness=(0.25, 0.65, 1.0), environment=(1.0, 1.0, 1.0), sun=True) -> Scene:
+    """Two rows of rough conductor spheres, single-scattering above and Kulla-Conty compensated
+    below (the layout of the reference's scenes/white-furnace.json), on a diffuse floor under a
+    constant environment and, optionally, a sun."""
+    unit = subdivided_sphere(sphere_level)
+    parts, mats = [], []
+    for row, compensate in enumerate((False, True)):
+        for col, r in enumerate(roughness):
+            center = (1.1 * (col - (len(roughness) - 1) / 2), 1.6 - 1.1 * row, 0.0)
+            parts.append((unit * 0.5 + center, unit))
+            mats.append(material("conductor", (0.95, 0.64, 0.54) if row else (1.0, 1.0, 1.0), roughness=r,
+                                 compensate_energy=compensate))
+    quad = np.array([[(-4, -0.5, -4), (-4, -0.5, 4), (4, -0.5, 4)], [(-4, -0.5, -4), (4, -0.5, 4), (4, -0.5, -4)]])
+    parts.append((quad, np.tile(np.array([0, 1, 0], np.float64), (2, 3, 1))))
+    mats.append(material("diffuse", (0.5, 0.5, 0.5)))
+    cam = Camera(position=(0, 1, 4.4), target=(0, 1, 0), up=(0, 1, 0), vertical_fov_degrees=32, near_plane=0.05,
+                 far_plane=100.0)
+    lights = directional_lights(((-0.5, -1.0, -0.4), (1.0, 0.9, 0.8), 2.0)) if sun else None
+    kw = dict(directional=lights) if sun else {}
+    return concat(parts, mats, environment=environment, camera=cam, name="furnace", **kw)
+
+
 # ----------------------------------------------------------------------------------------------
 # Small analytic scenes used by the reference's own tests (path_tests.cpp:23-68).
 

@@ -368,6 +368,9 @@ struct Material {
     V3<float> base_color, emission;
     float     roughness, ior;
     bool      double_sided, mirrored;
+    bool      compensate_energy = false;  // Kulla-Conty lobe for rough conductors (scene.hpp:125-126)
+    int32_t   base_color_texture = -1;    // index into Scene::textures, -1 = none
+    uint32_t  texture_filter = 2;         // 0 nearest, 1 bilinear, 2 trilinear (texture.hpp:13)
 };
 
 struct Scene;
@@ -790,33 +793,119 @@ template <class R> V3<R> sample_vndf(V3<R> wo, V2<R> u, R alpha) {
     return normalize(V3<R>{alpha * n.x, alpha * n.y, std::max(R(0), n.z)});
 }
 
+// ----------------------------------------------------------------------------------------------
+// Kulla-Conty energy compensation (microfacet_energy.cpp, ibl.cpp:28-47, :240-297). The table is
+// the reference's bake_ibl_brdf() at its defaults: 64 x 64 nodes (x = cos(theta_o) in [0,1], y =
+// roughness in [0,1]), 512 Hammersley samples per node with a per-node shift drawn from
+// Sampler::for_sample(101 + node * 0x9e3779b97f4a7c15, 0, 0), FP64 sums of the white conductor's
+// VNDF sample weights split by Fresnel (A, B), stored as float; the third channel holds the
+// cosine-weighted average 2 * int E(mu) mu dmu of the row (Simpson over the piecewise-linear E).
+constexpr int kEnergyRes     = 64;
+constexpr int kEnergySamples = 512;
+
+struct EnergyTable {
+    std::vector<std::array<float, 3>> node;  // [y * kEnergyRes + x]
+};
+
+inline double radical_inverse(uint32_t bits) {
+    bits = (bits << 16U) | (bits >> 16U);
+    bits = ((bits & 0x55555555U) << 1U) | ((bits & 0xAAAAAAAAU) >> 1U);
+    bits = ((bits & 0x33333333U) << 2U) | ((bits & 0xCCCCCCCCU) >> 2U);
+    bits = ((bits & 0x0F0F0F0FU) << 4U) | ((bits & 0xF0F0F0F0U) >> 4U);
+    bits = ((bits & 0x00FF00FFU) << 8U) | ((bits & 0xFF00FF00U) >> 8U);
+    return static_cast<double>(bits) * 0x1p-32;
+}
+
+template <class R> struct Bsdf;
+V2<double> integrate_ibl_brdf(double mu, double roughness, uint32_t samples, uint64_t seed);
+
+inline void store_average_energy(std::vector<std::array<float, 3>>& t, int width, int row) {
+    double       integral = 0;
+    const double step     = 1. / static_cast<double>(width - 1);
+    for (int x = 1; x < width; ++x) {
+        const auto&  a = t[static_cast<size_t>(row) * width + x - 1];
+        const auto&  b = t[static_cast<size_t>(row) * width + x];
+        const double left  = std::clamp(static_cast<double>(a[0] + a[1]), 0., 1.);
+        const double right = std::clamp(static_cast<double>(b[0] + b[1]), 0., 1.);
+        const double mu    = static_cast<double>(x) * step;
+        integral += step / 3 * (((mu - step) * left + 2 * (mu - step * .5) * (left + right)) + mu * right);
+    }
+    for (int x = 0; x < width; ++x) t[static_cast<size_t>(row) * width + x][2] = static_cast<float>(std::clamp(integral, 0., 1.));
+}
+
+inline const EnergyTable& energy_table();
+
+template <class R> V2<R> sample_energy(const EnergyTable& table, R mu, R roughness) {
+    const R    x  = std::clamp(mu, R(0), R(1)) * static_cast<R>(kEnergyRes - 1);
+    const R    y  = std::clamp(roughness, R(0), R(1)) * static_cast<R>(kEnergyRes - 1);
+    const auto ix = static_cast<size_t>(x), iy = static_cast<size_t>(y);
+    const auto jx = std::min<size_t>(ix + 1, kEnergyRes - 1), jy = std::min<size_t>(iy + 1, kEnergyRes - 1);
+    auto at = [&](size_t cx, size_t cy) {
+        const auto& n = table.node[cy * kEnergyRes + cx];
+        return V3<R>{R(n[0]), R(n[1]), R(n[2])};
+    };
+    auto mix = [](V3<R> a, V3<R> b, R t) { return a * (R(1) - t) + b * t; };
+    const R     tx = x - static_cast<R>(ix);
+    const V3<R> lo = mix(at(ix, iy), at(jx, iy), tx);
+    const V3<R> hi = mix(at(ix, jy), at(jx, jy), tx);
+    const V3<R> e  = mix(lo, hi, y - static_cast<R>(iy));
+    return {std::clamp(e.x + e.y, R(0), R(1)), std::clamp(e.z, R(0), R(1))};
+}
+
+template <class R> inline V3<R> operator/(V3<R> a, V3<R> b) { return {a.x / b.x, a.y / b.y, a.z / b.z}; }
+
+template <class R> V3<R> multiple_scattering(V3<R> f0, R ei, R eo, R average) {
+    if (average >= R(1) - R(1e-6)) return {0, 0, 0};
+    const V3<R> one{1, 1, 1};
+    const V3<R> f     = f0 + (one - f0) / R(21);
+    const V3<R> den   = one - f * (R(1) - average);
+    const V3<R> color = ((f * f) * average) / V3<R>{std::max(den.x, R(1e-8)), std::max(den.y, R(1e-8)), std::max(den.z, R(1e-8))};
+    return color * (((R(1) - ei) * (R(1) - eo)) / (pi_v<R>() * (R(1) - average)));
+}
+
 template <class R> struct Bsdf {
-    uint32_t kind;
-    V3<R>    color;
-    R        alpha, eta_i, eta_t;
-    Bsdf(const Material& m, bool front) : kind(m.kind) {
-        color = widen<R>(m.base_color);
+    uint32_t           kind;
+    V3<R>              color;
+    R                  alpha, eta_i, eta_t, roughness;
+    const EnergyTable* energy = nullptr;
+    // texture_color: the base-color texture sample (1 without a texture); color = base_color * clamp(texel, 0, 1)
+    // (bsdf.cpp:122-125).
+    Bsdf(const Material& m, bool front, V3<float> texture_color = {1, 1, 1}) : kind(m.kind) {
+        const V3<R> tc = widen<R>(texture_color);
+        color = widen<R>(m.base_color) * V3<R>{std::clamp(tc.x, R(0), R(1)), std::clamp(tc.y, R(0), R(1)), std::clamp(tc.z, R(0), R(1))};
         if constexpr (std::is_same_v<R, double>)
             alpha = static_cast<double>(m.roughness) * m.roughness;
         else
             alpha = m.roughness * m.roughness;
+        roughness = R(m.roughness);
         eta_i = front ? R(1) : R(m.ior);
         eta_t = front ? R(m.ior) : R(1);
         if (kind > 3) throw std::invalid_argument("Unsupported material kind in the oracle");
+        if (kind == 1 && m.compensate_energy && !is_delta()) energy = &energy_table();
     }
     bool is_delta() const { return kind == 2 || kind == 3 || (kind == 1 && alpha <= R(1e-6)); }
+    R    single_probability(R cosine) const {
+        return energy ? std::clamp(sample_energy(*energy, cosine, roughness).x, R(0.05), R(0.95)) : R(1);
+    }
 
     V3<R> eval_n(V3<R> wo, V3<R> wi) const {
         if (wo.z <= 0 || wi.z <= 0 || is_delta()) return {0, 0, 0};
         if (kind == 0) return color / pi_v<R>();
         const V3<R> m = normalize(wo + wi);
-        return (((schlick(color, dot(wo, m)) * ggx_d(m, alpha)) * g1_over_cos(wo, alpha)) * g1_over_cos(wi, alpha)) / R(4);
+        const V3<R> single = (((schlick(color, dot(wo, m)) * ggx_d(m, alpha)) * g1_over_cos(wo, alpha)) * g1_over_cos(wi, alpha)) / R(4);
+        if (!energy) return single;
+        const V2<R> outgoing = sample_energy(*energy, wo.z, roughness);
+        const V2<R> incoming = sample_energy(*energy, wi.z, roughness);
+        return single + multiple_scattering(color, incoming.x, outgoing.x, outgoing.y);
     }
     R pdf_n(V3<R> wo, V3<R> wi) const {
         if (wo.z <= 0 || wi.z <= 0 || is_delta()) return 0;
         if (kind == 0) return cosine_hemisphere_pdf(wi);
-        const V3<R> m = normalize(wo + wi);
-        return ggx_d(m, alpha) * g1_over_cos(wo, alpha) / R(4);
+        const V3<R> m      = normalize(wo + wi);
+        const R     glossy = ggx_d(m, alpha) * g1_over_cos(wo, alpha) / R(4);
+        if (!energy) return glossy;
+        const R weight = single_probability(wo.z);
+        return weight * glossy + (R(1) - weight) * cosine_hemisphere_pdf(wi);
     }
     V3<R> eval(V3<R> wo, V3<R> wi) const { return eval_n(unit_direction(wo), unit_direction(wi)); }
     R     pdf(V3<R> wo, V3<R> wi) const { return pdf_n(unit_direction(wo), unit_direction(wi)); }
@@ -843,7 +932,7 @@ template <class R> struct Bsdf {
             return BsdfSample<R>{reflected, w, R(1), true, specular_reflection, R(1)};
         }
         V3<R> wi;
-        if (kind == 0) {
+        if (kind == 0 || (energy && component >= single_probability(wo.z))) {
             wi = sample_cosine_hemisphere(u);
         } else {
             const V3<R> m = sample_vndf(wo, u, alpha);
@@ -856,6 +945,51 @@ template <class R> struct Bsdf {
         return BsdfSample<R>{wi, weight, density, false, kind == 0 ? diffuse_reflection : glossy_reflection, R(1)};
     }
 };
+
+// integrate_ibl_brdf (ibl.cpp:240-277): Fresnel-split albedo of the white conductor at one node.
+V2<double> integrate_ibl_brdf(double mu, double roughness, uint32_t samples, uint64_t seed) {
+    if (roughness * roughness <= 1e-6) {
+        const double fresnel = std::pow(1 - mu, 5);
+        return {1 - fresnel, fresnel};
+    }
+    const double   cosine = std::max(mu, 1e-6);
+    const V3<double> wo{std::sqrt(std::max(0.0, 1 - cosine * cosine)), 0, cosine};
+    Material       white{1, {1, 1, 1}, {0, 0, 0}, static_cast<float>(roughness), 1.5f, true, false};
+    const Bsdf<double> bsdf(white, true);
+    Sampler        shift_rng = Sampler::for_sample(seed, 0, 0);
+    const double   sx        = shift_rng.next_1d<double>();
+    const double   sy        = shift_rng.next_1d<double>();
+    V2<double>     sum{0, 0};
+    for (uint32_t i = 0; i < samples; ++i) {
+        const double px = (static_cast<double>(i) + 0.5) / samples + sx;
+        const double py = radical_inverse(i) + sy;
+        const V2<double> u{px - std::floor(px), py - std::floor(py)};
+        const auto smp = bsdf.sample(wo, 0.5, u);
+        if (!smp) continue;
+        const V3<double> halfway = normalize(wo + smp->direction);
+        const double     fc      = std::pow(1 - std::clamp(dot(wo, halfway), 0.0, 1.0), 5);
+        sum.x += smp->weight.x * (1 - fc);
+        sum.y += smp->weight.x * fc;
+    }
+    return {sum.x / static_cast<double>(samples), sum.y / static_cast<double>(samples)};
+}
+
+inline const EnergyTable& energy_table() {
+    static const EnergyTable table = [] {
+        EnergyTable t;
+        t.node.resize(static_cast<size_t>(kEnergyRes) * kEnergyRes);
+        for (int y = 0; y < kEnergyRes; ++y) {
+            for (int x = 0; x < kEnergyRes; ++x) {
+                const auto v = integrate_ibl_brdf(static_cast<double>(x) / (kEnergyRes - 1), static_cast<double>(y) / (kEnergyRes - 1),
+                                                  kEnergySamples, 101 + static_cast<uint64_t>(y * kEnergyRes + x) * 0x9e3779b97f4a7c15ULL);
+                t.node[static_cast<size_t>(y) * kEnergyRes + x] = {static_cast<float>(v.x), static_cast<float>(v.y), 0.0f};
+            }
+            store_average_energy(t.node, kEnergyRes, y);
+        }
+        return t;
+    }();
+    return table;
+}
 
 template <class R> R fresnel_dielectric(R cosine, R eta_i, R eta_t) {
     if (!std::isfinite(cosine) || cosine < 0 || cosine > 1 || !(eta_i > 0) || !(eta_t > 0))
@@ -1115,6 +1249,32 @@ template <class F> static int guarded(F&& f) {
     }
 }
 
+static Material to_material(const OrcMaterial& m) {
+    if (m.base_color_texture != 0) throw std::invalid_argument("textures are not supported by this oracle build");
+    return Material{m.kind, {m.base_color[0], m.base_color[1], m.base_color[2]}, {m.emission[0], m.emission[1], m.emission[2]},
+                    m.roughness, m.ior, m.double_sided != 0, m.mirrored != 0, m.compensate_energy != 0,
+                    static_cast<int32_t>(m.base_color_texture) - 1, m.texture_filter};
+}
+
+extern "C" void orc_energy_table(float* out) {
+    const auto& t = energy_table();
+    for (size_t i = 0; i < t.node.size(); ++i)
+        for (int c = 0; c < 3; ++c) out[i * 3 + c] = t.node[i][c];
+}
+
+extern "C" void orc_integrate_ibl_brdf(double mu, double roughness, uint32_t samples, uint64_t seed, double* out) {
+    const auto v = integrate_ibl_brdf(mu, roughness, samples, seed);
+    out[0]       = v.x;
+    out[1]       = v.y;
+}
+
+extern "C" double orc_average_energy_of_ramp(const float* e, uint32_t width) {
+    std::vector<std::array<float, 3>> row(width);
+    for (uint32_t x = 0; x < width; ++x) row[x] = {e[x], 0.0f, 0.0f};
+    store_average_energy(row, static_cast<int>(width), 0);
+    return row[0][2];
+}
+
 extern "C" void* orc_scene_new(const OrcSceneDesc* d) {
     Scene* s = nullptr;
     const int rc = guarded([&] {
@@ -1132,9 +1292,7 @@ extern "C" void* orc_scene_new(const OrcSceneDesc* d) {
         }
         for (uint32_t i = 0; i < d->instance_count; ++i) {
             const auto& m = d->materials[i];
-            sc->materials.push_back({m.kind, {m.base_color[0], m.base_color[1], m.base_color[2]},
-                                     {m.emission[0], m.emission[1], m.emission[2]}, m.roughness, m.ior,
-                                     m.double_sided != 0, m.mirrored != 0});
+            sc->materials.push_back(to_material(m));
         }
         sc->environment = {d->environment[0], d->environment[1], d->environment[2]};
         sc->directional.assign(d->directional, d->directional + d->directional_count);
@@ -1469,10 +1627,7 @@ extern "C" double orc_fresnel(double cosine, double eta_i, double eta_t) {
     return v;
 }
 
-static Material to_material(const OrcMaterial* m) {
-    return {m->kind, {m->base_color[0], m->base_color[1], m->base_color[2]}, {m->emission[0], m->emission[1], m->emission[2]},
-            m->roughness, m->ior, m->double_sided != 0, m->mirrored != 0};
-}
+static Material to_material(const OrcMaterial* m) { return to_material(*m); }
 
 template <class R> static int bsdf_sample_impl(const OrcMaterial* m, int front, const double* wo, double comp, const double* u, double* out) {
     const Bsdf<R> b(to_material(m), front != 0);

@@ -37,12 +37,21 @@ struct OrcMaterial {
     float    ior;
     uint32_t double_sided;
     uint32_t mirrored;       // determinant(object_to_world) < 0
+    uint32_t compensate_energy;   // Kulla-Conty multiple-scattering lobe (scene.hpp:125-126)
+    uint32_t base_color_texture;  // 1 + texture index, 0 = none
+    uint32_t texture_filter;      // 0 nearest, 1 bilinear, 2 trilinear
 };
 
 struct OrcDirectional {
     float direction[3];
     float color[3];
     float intensity;
+};
+
+// Linear RGB texture, level 0 (row 0 = top row of the image).
+struct OrcTexture {
+    uint32_t     width, height;
+    const float* rgb;  // [height][width][3]
 };
 
 struct OrcSceneDesc {
@@ -56,6 +65,12 @@ struct OrcSceneDesc {
     float                  environment[3];
     uint32_t               directional_count;
     const OrcDirectional*  directional;
+    const float*           uvs;             // [n][3][2] per-corner texture coordinates or null
+    uint32_t               texture_count;
+    const OrcTexture*      textures;
+    uint32_t               environment_texture;  // 1 + texture index, 0 = none
+    float                  environment_strength;
+    float                  environment_rotation_degrees;
 };
 
 struct OrcCamera {
@@ -120,6 +135,13 @@ int orc_sample_pixel(void* scene, int precision, const OrcCamera* camera,
 int orc_trace(void* scene, int precision, const OrcRender* settings, const double* ray,
               double tmin, double tmax, int sampler_kind, uint64_t a, uint64_t b, uint64_t c,
               double* out, OrcStats* stats);
+
+// Kulla-Conty energy table [64][64][3] (A, B, row average; microfacet_energy.cpp, ibl.cpp:279-297),
+// one node's Fresnel-split albedo (ibl.cpp:240-277) and store_average_energy of a one-row table
+// whose A channel is e[] (B = 0); returns the row average.
+void   orc_energy_table(float* out);
+void   orc_integrate_ibl_brdf(double mu, double roughness, uint32_t samples, uint64_t seed, double* out);
+double orc_average_energy_of_ramp(const float* e, uint32_t width);
 
 // Primitive pieces for known-answer tests.
 void   orc_pcg(uint64_t seed, uint64_t sequence, uint64_t n, uint32_t* out);

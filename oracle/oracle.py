@@ -31,7 +31,17 @@ class _SceneDesc(C.Structure):
         ("environment", C.c_float * 3),
         ("directional_count", C.c_uint32),
         ("directional", C.c_void_p),
+        ("uvs", C.c_void_p),
+        ("texture_count", C.c_uint32),
+        ("textures", C.c_void_p),
+        ("environment_texture", C.c_uint32),
+        ("environment_strength", C.c_float),
+        ("environment_rotation_degrees", C.c_float),
     ]
+
+
+class _Texture(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("rgb", C.c_void_p)]
 
 
 class Stats(C.Structure):
@@ -91,6 +101,10 @@ def lib():
         L.orc_light_count.restype = C.c_uint64
         L.orc_light_count.argtypes = [P]
         L.orc_camera_ray.argtypes = [P, C.c_int, P, C.c_double, P, P]
+        L.orc_energy_table.argtypes = [P]
+        L.orc_integrate_ibl_brdf.argtypes = [C.c_double, C.c_double, C.c_uint32, C.c_uint64, P]
+        L.orc_average_energy_of_ramp.restype = C.c_double
+        L.orc_average_energy_of_ramp.argtypes = [P, C.c_uint32]
         _lib = L
     return _lib
 
@@ -125,6 +139,24 @@ class OracleScene:
         d.environment[:] = list(scene.environment)
         d.directional_count = len(scene.directional)
         d.directional = _ptr(self._keep[5])
+        if getattr(scene, "uvs", None) is not None:
+            uv = np.ascontiguousarray(scene.uvs, np.float32)
+            self._keep.append(uv)
+            d.uvs = _ptr(uv)
+        textures = getattr(scene, "textures", [])
+        if textures:
+            tex = (_Texture * len(textures))()
+            for i, t in enumerate(textures):
+                t = np.ascontiguousarray(t, np.float32)
+                self._keep.append(t)
+                tex[i].height, tex[i].width = t.shape[0], t.shape[1]
+                tex[i].rgb = _ptr(t)
+            self._keep.append(tex)
+            d.texture_count = len(textures)
+            d.textures = C.cast(tex, C.c_void_p)
+        d.environment_texture = getattr(scene, "environment_texture", -1) + 1
+        d.environment_strength = getattr(scene, "environment_strength", 1.0)
+        d.environment_rotation_degrees = getattr(scene, "environment_rotation_degrees", 0.0)
         self.handle = lib().orc_scene_new(C.byref(d))
         if not self.handle:
             raise ValueError(lib().orc_last_error().decode())
@@ -324,3 +356,21 @@ def camera_ray(camera, uv, aspect, lens=(0.5, 0.5), precision=64):
     ln = np.asarray(lens, np.float64)
     _check(lib().orc_camera_ray(_ptr(cam), _prec(precision), _ptr(uvv), aspect, _ptr(ln), _ptr(out)))
     return out[:3], out[3:]
+
+
+def energy_table():
+    """The Kulla-Conty table [64][64][3]: (A, B, row average) per (roughness row, cos column)."""
+    out = np.zeros((64, 64, 3), np.float32)
+    lib().orc_energy_table(_ptr(out))
+    return out
+
+
+def integrate_ibl_brdf(mu, roughness, samples, seed=101):
+    out = np.zeros(2, np.float64)
+    lib().orc_integrate_ibl_brdf(float(mu), float(roughness), int(samples), int(seed), _ptr(out))
+    return out
+
+
+def average_energy_of_ramp(e):
+    e = np.ascontiguousarray(e, np.float32)
+    return lib().orc_average_energy_of_ramp(_ptr(e), len(e))

@@ -74,3 +74,13 @@ def test_traversal_kernels_agree_with_fp32_twin(oracle, traversal):
     _check(img, twin)
     assert prog.statistics["shadow_rays"] == st["shadow_rays"]
     assert prog.statistics["extension_rays"] == st["extension_rays"]
+
+
+@pytest.mark.parametrize("mode", ["mis", "bsdf", "light"])
+def test_energy_compensated_conductors_match_fp32_twin(device, oracle, mode):
+    """Kulla-Conty compensated rough conductors (bsdf.cpp:128-189): mixed lobe sampling, mixed pdf
+    in MIS, multiple-scattering evaluation, against the FP32 twin within 1e-4."""
+    scene = scenegen.furnace_spheres()
+    img, twin, prog, st = _render_pair(device, oracle, scene, 40, 32, 8, 6, mode=mode, seed=9)
+    _check(img, twin)
+    assert prog.statistics["scattering_events"] == st["scattering_events"]

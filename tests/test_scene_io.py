@@ -213,11 +213,11 @@ def test_reference_scenes():
     # studio.json: 26 triangles (the reference CLI's --trace-normals check, CMakeLists.txt:205-207)
     assert "triangles: 26\n" in cli("--scene-info", os.path.join(REF, "studio.json")).stdout
     renderable = {"cornell-box.json": 36, "cornell-small-light.json": 36, "path-room.json": 36, "soft-shadows.json": 26,
-                  "studio-two-lights.json": 26, "cornell-materials.json": 4428, "material-balls.json": 13252}
+                  "studio-two-lights.json": 26, "cornell-materials.json": 4428, "material-balls.json": 13252,
+                  "white-furnace.json": 13248}
     for name, tris in renderable.items():
         assert f"triangles: {tris}\n" in cli("--scene-info", os.path.join(REF, name)).stdout, name
-    unsupported = {"sss-wax.json": "subsurface", "volume-cornell.json": "media", "studio-textured.json": "textures",
-                   "white-furnace.json": "energy compensation"}
+    unsupported = {"sss-wax.json": "subsurface", "volume-cornell.json": "media", "studio-textured.json": "textures"}
     for name, word in unsupported.items():
         r = cli("--scene-info", os.path.join(REF, name), check=False)
         assert r.returncode == 1 and word in r.stderr, name
