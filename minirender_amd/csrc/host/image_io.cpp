@@ -6,8 +6,10 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <iterator>
 #include <limits>
 #include <stdexcept>
 #include <string>
@@ -178,6 +180,133 @@ void write_png(const RgbImage& image, const std::filesystem::path& path) {
     chunk(png, "IDAT", deflate(raw));
     chunk(png, "IEND", std::string());
     write_file(path, png);
+}
+
+}  // namespace mrg_host
+
+namespace mrg_host {
+
+float srgb_to_linear(float encoded) {
+    if (!std::isfinite(encoded)) return 0.0f;
+    encoded = std::max(encoded, 0.0f);
+    return encoded <= 0.04045f ? encoded / 12.92f : std::pow((encoded + 0.055f) / 1.055f, 2.4f);
+}
+
+namespace {
+
+uint32_t be32(const unsigned char* p) { return (uint32_t(p[0]) << 24) | (uint32_t(p[1]) << 16) | (uint32_t(p[2]) << 8) | p[3]; }
+
+int paeth(int a, int b, int c) {
+    const int p = a + b - c, pa = std::abs(p - a), pb = std::abs(p - b), pc = std::abs(p - c);
+    return pa <= pb && pa <= pc ? a : (pb <= pc ? b : c);
+}
+
+RgbImage decode_png(const std::string& bytes, bool srgb) {
+    static const unsigned char sig[8] = {137, 80, 78, 71, 13, 10, 26, 10};
+    if (bytes.size() < 8 || std::memcmp(bytes.data(), sig, 8) != 0) throw std::runtime_error("Unsupported or corrupt texture image");
+    const auto* b = reinterpret_cast<const unsigned char*>(bytes.data());
+    size_t      p = 8;
+    uint32_t    w = 0, h = 0, depth = 0, type = 0, interlace = 0;
+    std::string idat, palette;
+    bool        header = false;
+    while (p + 8 <= bytes.size()) {
+        const uint32_t len = be32(b + p);
+        const std::string tag(bytes.data() + p + 4, 4);
+        if (p + 12 + uint64_t(len) > bytes.size()) throw std::runtime_error("Unsupported or corrupt texture image");
+        const unsigned char* d = b + p + 8;
+        if (tag == "IHDR") {
+            if (len < 13) throw std::runtime_error("Unsupported or corrupt texture image");
+            w = be32(d), h = be32(d + 4), depth = d[8], type = d[9], interlace = d[12];
+            header = true;
+        } else if (tag == "PLTE") {
+            palette.assign(reinterpret_cast<const char*>(d), len);
+        } else if (tag == "IDAT") {
+            idat.append(reinterpret_cast<const char*>(d), len);
+        } else if (tag == "IEND") {
+            break;
+        }
+        p += 12 + len;
+    }
+    if (!header || w == 0 || h == 0) throw std::runtime_error("Unsupported or corrupt texture image");
+    if (w > 16384 || h > 16384 || uint64_t(w) * h > 64ull * 1024 * 1024) throw std::invalid_argument("Texture exceeds the 64-megapixel budget");
+    if (interlace != 0) throw std::invalid_argument("Interlaced PNG textures are not supported by this loader");
+    uint32_t channels = 0;
+    switch (type) {
+    case 0: channels = 1; break;
+    case 2: channels = 3; break;
+    case 3: channels = 1; break;
+    case 4: channels = 2; break;
+    case 6: channels = 4; break;
+    default: throw std::runtime_error("Unsupported or corrupt texture image");
+    }
+    if (!(depth == 8 || (depth == 16 && type != 3)))
+        throw std::invalid_argument("PNG textures of bit depth " + std::to_string(depth) + " are not supported by this loader");
+    if (type == 3 && palette.size() < 3) throw std::runtime_error("Unsupported or corrupt texture image");
+    const uint32_t bpp      = channels * depth / 8;  // bytes per pixel
+    const uint64_t row      = uint64_t(w) * bpp;
+    uLongf         raw_size = uLongf((row + 1) * h);
+    std::string    raw(raw_size, '\0');
+    if (uncompress(reinterpret_cast<Bytef*>(&raw[0]), &raw_size, reinterpret_cast<const Bytef*>(idat.data()), uLong(idat.size())) != Z_OK ||
+        raw_size != (row + 1) * h)
+        throw std::runtime_error("Image decode failed");
+    std::vector<unsigned char> cur(row), prev(row, 0);
+    RgbImage img;
+    img.width  = w;
+    img.height = h;
+    img.rgb.resize(size_t(w) * h * 3);
+    const float scale = depth == 16 ? 65535.0f : 255.0f;
+    for (uint32_t y = 0; y < h; ++y) {
+        const auto*         line   = reinterpret_cast<const unsigned char*>(raw.data()) + y * (row + 1);
+        const unsigned char filter = line[0];
+        for (uint64_t i = 0; i < row; ++i) {
+            const int x = line[1 + i];
+            const int a = i >= bpp ? cur[i - bpp] : 0, up = prev[i], c = i >= bpp ? prev[i - bpp] : 0;
+            int       v;
+            switch (filter) {
+            case 0: v = x; break;
+            case 1: v = x + a; break;
+            case 2: v = x + up; break;
+            case 3: v = x + (a + up) / 2; break;
+            case 4: v = x + paeth(a, up, c); break;
+            default: throw std::runtime_error("Image decode failed");
+            }
+            cur[i] = static_cast<unsigned char>(v & 0xff);
+        }
+        for (uint32_t x = 0; x < w; ++x) {
+            float rgb[3];
+            auto  sample = [&](uint32_t ch) {
+                const unsigned char* q = &cur[size_t(x) * bpp + ch * (depth / 8)];
+                return depth == 16 ? float((uint32_t(q[0]) << 8) | q[1]) : float(q[0]);
+            };
+            if (type == 3) {
+                const uint32_t index = cur[x];
+                if (3 * index + 2 >= palette.size()) throw std::runtime_error("Image decode failed");
+                for (int k = 0; k < 3; ++k) rgb[k] = float(static_cast<unsigned char>(palette[3 * index + k]));
+            } else if (channels <= 2) {
+                rgb[0] = rgb[1] = rgb[2] = sample(0);
+            } else {
+                for (uint32_t k = 0; k < 3; ++k) rgb[k] = sample(k);
+            }
+            for (int k = 0; k < 3; ++k) {
+                const float value                       = rgb[k] / scale;
+                img.rgb[(size_t(y) * w + x) * 3 + k] = srgb ? srgb_to_linear(value) : value;
+            }
+        }
+        std::swap(cur, prev);
+    }
+    return img;
+}
+
+}  // namespace
+
+RgbImage read_texture(const std::filesystem::path& path, bool srgb) {
+    std::string ext = path.extension().string();
+    std::transform(ext.begin(), ext.end(), ext.begin(), [](unsigned char c) { return char(std::tolower(c)); });
+    if (ext != ".png") throw std::invalid_argument("Texture format " + ext + " is not supported by this loader (PNG only)");
+    std::ifstream in(path, std::ios::binary);
+    if (!in) throw std::runtime_error("Cannot open texture image");
+    const std::string bytes((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+    return decode_png(bytes, srgb);
 }
 
 }  // namespace mrg_host

@@ -24,4 +24,12 @@ RgbImage display_image(const RgbImage& linear, double exposure);
 void write_exr(const RgbImage& image, const std::filesystem::path& path);
 void write_png(const RgbImage& image, const std::filesystem::path& path);
 
+float srgb_to_linear(float encoded);   // image.cpp:34-40
+
+// Texture image decoding (Image::read, raster_io.cpp:40-113) for PNG files: bit depth 8 or 16, gray,
+// gray + alpha, RGB, RGBA or 8-bit palette, non-interlaced, decoded per the PNG specification and
+// delivered as three channels (gray replicated, alpha ignored); each channel is value / 255 (or
+// / 65535 for 16-bit data), then srgb_to_linear when `srgb`. Other formats are refused by name.
+RgbImage read_texture(const std::filesystem::path& path, bool srgb);
+
 }  // namespace mrg_host

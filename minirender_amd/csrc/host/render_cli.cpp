@@ -61,7 +61,8 @@ void put_vec(std::ofstream& out, const std::vector<T>& v) {
 }
 
 // u64 n, u32 instances, u32 directional, f32 env[3], mrg_camera, then positions, normals,
-// instance, triangle, mrg_material[instances], mrg_directional_light[directional]
+// instance, triangle, mrg_material[instances], mrg_directional_light[directional], uvs [n][3][2],
+// u32 textures, then per texture u32 width, u32 height, f32 rgb[height][width][3]
 void dump_flat(const mrg_host::FlatScene& f, const std::string& path) {
     std::ofstream out(path, std::ios::binary);
     out.exceptions(std::ios::badbit | std::ios::failbit);
@@ -76,6 +77,13 @@ void dump_flat(const mrg_host::FlatScene& f, const std::string& path) {
     put_vec(out, f.triangle);
     put_vec(out, f.materials);
     put_vec(out, f.lights);
+    put_vec(out, f.uvs);
+    put(out, uint32_t(f.textures.size()));
+    for (const auto& t : f.textures) {
+        put(out, t.width);
+        put(out, t.height);
+        put_vec(out, t.rgb);
+    }
 }
 
 int scene_info(int argc, char** argv) {
@@ -85,6 +93,20 @@ int scene_info(int argc, char** argv) {
     std::cout << "triangles: " << flat.instance.size() << "\ninstances: " << flat.materials.size()
               << "\nmeshes: " << doc.meshes.size() << "\ndirectional lights: " << flat.lights.size() << '\n';
     if (argc >= 2) dump_flat(flat, argv[1]);
+    return 0;
+}
+
+// u32 width, u32 height, f32 rgb[height][width][3]: the decoded linear texture
+int dump_texture(int argc, char** argv) {
+    if (argc < 3) throw std::invalid_argument("Usage: --texture image.png srgb|linear texture.bin");
+    const std::string encoding = argv[1];
+    if (encoding != "srgb" && encoding != "linear") throw std::invalid_argument("Unknown texture encoding");
+    const auto    img = mrg_host::read_texture(std::filesystem::u8path(argv[0]), encoding == "srgb");
+    std::ofstream out(argv[2], std::ios::binary);
+    out.exceptions(std::ios::badbit | std::ios::failbit);
+    put(out, img.width);
+    put(out, img.height);
+    put_vec(out, img.rgb);
     return 0;
 }
 
@@ -180,7 +202,7 @@ int render(int argc, char** argv) {
         throw std::invalid_argument("Session requires <=32 threads, tile size in [1,128], batch in [1,1024]");
     if (!std::isfinite(exposure) || std::abs(exposure) > 64) throw std::invalid_argument("Exposure must be finite and in [-64,64]");
 
-    const auto flat  = mrg_host::flatten(mrg_host::load_scene(std::filesystem::u8path(argv[0])));
+    auto       flat  = mrg_host::flatten(mrg_host::load_scene(std::filesystem::u8path(argv[0])));
     const auto start = std::chrono::steady_clock::now();
     gpu::Device device(0);
     auto        scene = std::make_shared<gpu::SceneBvh>(device, flat.desc());
@@ -219,6 +241,7 @@ int main(int argc, char** argv) {
             if (mode == "--render") return render(argc - 2, argv + 2);
             if (mode == "--scene-info") return scene_info(argc - 2, argv + 2);
             if (mode == "--obj") return dump_obj(argc - 2, argv + 2);
+            if (mode == "--texture") return dump_texture(argc - 2, argv + 2);
             if (mode == "--encode") return encode(argc - 2, argv + 2);
         }
         std::cerr << "Usage: mrg_render --render scene.json output-prefix [options] | --scene-info scene.json [flat.bin] | "

@@ -346,8 +346,9 @@ SceneDocument load_scene(const std::filesystem::path& path) {
             m.double_sided = bool_or(v, "double_sided", true) ? 1u : 0u;
             if (v.contains("cloth") || v.contains("subsurface"))
                 throw std::invalid_argument("Material " + id + ": cloth / subsurface data is not supported by the GPU integrator");
-            if (!string_or(v, "base_color_texture").empty() || !string_or(v, "normal_texture").empty())
-                throw std::invalid_argument("Material " + id + ": textures are not supported by the GPU integrator");
+            if (!string_or(v, "normal_texture").empty())
+                throw std::invalid_argument("Material " + id + ": normal maps are not supported by the GPU integrator");
+            doc.material_textures.push_back(string_or(v, "base_color_texture"));
             m.compensate_energy = bool_or(v, "compensate_energy", false) ? 1u : 0u;
             m.texture_filter    = texture_filter(string_or(v, "texture_filter"));
             doc.material_ids.push_back(id);
@@ -359,6 +360,24 @@ SceneDocument load_scene(const std::filesystem::path& path) {
             const std::string source = v.at("source").string();
             doc.meshes.emplace(id, load_obj(dir / std::filesystem::u8path(source)));
             doc.mesh_order.push_back(id);
+        }
+        if (const Json* textures = json.find("textures")) {
+            for (const Json& v : textures->array()) {
+                TextureAsset t;
+                t.id                       = v.at("id").string();
+                const std::string encoding = v.contains("encoding") ? v.at("encoding").string() : std::string("srgb");
+                if (encoding != "srgb" && encoding != "linear") throw std::invalid_argument("Unknown texture encoding");
+                t.image = read_texture(dir / std::filesystem::u8path(v.at("source").string()), encoding == "srgb");
+                doc.textures.push_back(std::move(t));
+            }
+        }
+        for (size_t i = 0; i < doc.materials.size(); ++i) {
+            const std::string& tex = doc.material_textures[i];
+            if (tex.empty()) continue;
+            size_t k = 0;
+            while (k < doc.textures.size() && doc.textures[k].id != tex) ++k;
+            if (k == doc.textures.size()) throw std::invalid_argument("Material references missing texture: " + tex);
+            doc.materials[i].base_color_texture = uint32_t(k + 1);
         }
         for (const Json& v : json.at("objects").array()) {
             Object o;
@@ -409,6 +428,8 @@ void SceneDocument::validate() const {
             "Camera lens requires nonnegative aperture and positive focus distance");
     require(finite3(environment) && environment[0] >= 0 && environment[1] >= 0 && environment[2] >= 0,
             "Environment must be finite and nonnegative");
+    std::set<std::string> texture_ids;
+    for (const TextureAsset& t : textures) require(texture_ids.insert(t.id).second, "Duplicate Texture id: " + t.id);
     // Material::validate (scene.cpp:241-266), unique ids
     std::set<std::string> ids;
     for (size_t i = 0; i < materials.size(); ++i) {
@@ -449,6 +470,7 @@ FlatScene flatten(const SceneDocument& doc) {
     flat.camera      = doc.camera;
     flat.environment = doc.environment;
     flat.lights      = doc.lights;
+    for (const TextureAsset& t : doc.textures) flat.textures.push_back(t.image);
     uint32_t instance = 0;
     for (const Object& o : doc.objects) {
         if (!o.visible) continue;
@@ -477,6 +499,8 @@ FlatScene flatten(const SceneDocument& doc) {
                     n[r] = (nm[0][r] * double(c.normal[0]) + nm[1][r] * double(c.normal[1])) + nm[2][r] * double(c.normal[2]);
                 const double inv = 1.0 / std::sqrt((n[0] * n[0] + n[1] * n[1]) + n[2] * n[2]);
                 for (int r = 0; r < 3; ++r) flat.normals.push_back(float(n[r] * inv));
+                flat.uvs.push_back(c.uv[0]);
+                flat.uvs.push_back(c.uv[1]);
             }
             flat.instance.push_back(instance);
             flat.triangle.push_back(t);
@@ -486,8 +510,13 @@ FlatScene flatten(const SceneDocument& doc) {
     return flat;
 }
 
-mrg_scene_desc FlatScene::desc() const {
+mrg_scene_desc FlatScene::desc() {
+    texture_views.clear();
+    for (const RgbImage& t : textures) texture_views.push_back(mrg_texture{t.width, t.height, t.rgb.data()});
     mrg_scene_desc d{};
+    d.uvs           = uvs.empty() ? nullptr : uvs.data();
+    d.texture_count = uint32_t(texture_views.size());
+    d.textures      = texture_views.empty() ? nullptr : texture_views.data();
     d.triangle_count    = instance.size();
     d.positions         = positions.data();
     d.normals           = normals.data();

@@ -3,9 +3,10 @@
 // object transforms (src/scene/scene.cpp:119-139) and the snapshot + SceneBvh flattening
 // (scene.cpp:364-405, src/render/bvh.cpp:53-89) into the arrays of mrg_scene_desc.
 //
-// Features of the document that the GPU integrator does not render (textures, environment maps,
-// participating media, cloth, subsurface materials and normal maps) are rejected with
-// std::invalid_argument naming the feature, instead of being dropped silently.
+// Base-color textures are decoded from PNG (image_io.hpp). Features of the document that the GPU
+// integrator does not render (environment maps, participating media, cloth, subsurface materials
+// and normal maps) are rejected with std::invalid_argument naming the feature, instead of being
+// dropped silently.
 #pragma once
 
 #include <array>
@@ -15,6 +16,7 @@
 #include <string>
 #include <vector>
 
+#include "image_io.hpp"
 #include "mrg.h"
 
 namespace mrg_host {
@@ -49,11 +51,19 @@ struct Object {
     Transform   transform;
 };
 
+// TextureAsset (scene.hpp:146-156) with its decoded level-0 image.
+struct TextureAsset {
+    std::string id;
+    RgbImage    image;
+};
+
 struct SceneDocument {
     mrg_camera                              camera{};
     std::array<float, 3>                    environment{0.0f, 0.0f, 0.0f};
     std::vector<std::string>                material_ids;
-    std::vector<mrg_material>               materials;
+    std::vector<mrg_material>               materials;  // base_color_texture resolved against `textures`
+    std::vector<std::string>                material_textures;  // base_color_texture id per material ("" = none)
+    std::vector<TextureAsset>               textures;
     std::map<std::string, Mesh>             meshes;
     std::vector<std::string>                mesh_order;
     std::vector<Object>                     objects;
@@ -66,12 +76,15 @@ struct SceneDocument {
 struct FlatScene {
     std::vector<float>                 positions;  // [n][3][3]
     std::vector<float>                 normals;    // [n][3][3]
+    std::vector<float>                 uvs;        // [n][3][2]
+    std::vector<RgbImage>              textures;
+    std::vector<mrg_texture>           texture_views;  // filled by desc()
     std::vector<uint32_t>              instance, triangle;
     std::vector<mrg_material>          materials;  // per instance
     std::vector<mrg_directional_light> lights;
     std::array<float, 3>               environment{0.0f, 0.0f, 0.0f};
     mrg_camera                         camera{};
-    mrg_scene_desc                     desc() const;
+    mrg_scene_desc                     desc();
 };
 
 Mesh          load_obj(const std::filesystem::path& path);

@@ -145,6 +145,7 @@ struct mrg_scene {
     GpuScene       gpu{};
     mrg_scene_info info{};
     DevBuf         nodes, tris, normals, slot_instance, slot_emitter, materials, choices, cdf, areas, directional;
+    DevBuf         uvs, textures, texels;
 };
 
 struct mrg_job {
@@ -321,9 +322,20 @@ mrg_scene* build_scene(mrg_device* d, const mrg_scene_desc& s, const std::string
         if (m.compensate_energy > 1) throw std::invalid_argument("compensate_energy must be 0 or 1");
         if (m.texture_filter > MRG_FILTER_TRILINEAR) throw std::invalid_argument("Unknown texture filter");
         if (m.base_color_texture > s.texture_count) throw std::invalid_argument("Material references missing texture");
-        if (m.base_color_texture != 0) throw std::invalid_argument("base-color textures are not supported yet");
     }
     if (s.texture_count && !s.textures) throw std::invalid_argument("textures are missing");
+    for (uint32_t i = 0; i < s.texture_count; ++i) {
+        // Texture::Texture (texture.cpp:53-72)
+        const mrg_texture& t = s.textures[i];
+        if (t.width == 0 || t.height == 0 || !t.rgb) throw std::invalid_argument("Texture must not be empty");
+        if (t.width > 16384 || t.height > 16384 || uint64_t(t.width) * t.height > 64ull * 1024 * 1024)
+            throw std::invalid_argument("Texture exceeds the 64-megapixel budget");
+        for (uint64_t k = 0; k < uint64_t(t.width) * t.height * 3; ++k)
+            if (!std::isfinite(t.rgb[k])) throw std::invalid_argument("Texture contains nonfinite samples");
+    }
+    if (n && s.uvs)
+        for (uint64_t k = 0; k < n * 6; ++k)
+            if (!std::isfinite(s.uvs[k])) throw std::invalid_argument("Texture coordinates must be finite");
     if (s.environment_texture > s.texture_count) throw std::invalid_argument("Environment references missing texture");
     if (s.environment_texture != 0) throw std::invalid_argument("environment maps are not supported yet");
     auto sc    = std::make_unique<mrg_scene>();
@@ -428,6 +440,20 @@ mrg_scene* build_scene(mrg_device* d, const mrg_scene_desc& s, const std::string
         slot_instance[slot] = s.instance[p];
         slot_emitter[slot]  = area_choice[p];
     }
+    // per-corner uvs in leaf order; level-0 texels of every texture
+    std::vector<float> uvs(s.uvs ? n * 6 : 0);
+    if (s.uvs)
+        for (uint64_t slot = 0; slot < n; ++slot) std::memcpy(&uvs[slot * 6], s.uvs + uint64_t(bvh.order[slot]) * 6, 24);
+    std::vector<GpuTexture> texs(s.texture_count);
+    uint64_t                texel_count = 0;
+    for (uint32_t i = 0; i < s.texture_count; ++i) {
+        texs[i] = GpuTexture{texel_count, s.textures[i].width, s.textures[i].height};
+        texel_count += uint64_t(s.textures[i].width) * s.textures[i].height;
+    }
+    std::vector<float> texels(texel_count * 4, 0.0f);
+    for (uint32_t i = 0; i < s.texture_count; ++i)
+        for (uint64_t k = 0; k < uint64_t(texs[i].width) * texs[i].height; ++k)
+            std::memcpy(&texels[(texs[i].offset + k) * 4], s.textures[i].rgb + k * 3, 12);
     std::vector<GpuMaterial> mats(s.instance_count);
     for (uint32_t i = 0; i < s.instance_count; ++i) {
         const mrg_material& m = s.materials[i];
@@ -456,6 +482,9 @@ mrg_scene* build_scene(mrg_device* d, const mrg_scene_desc& s, const std::string
     upload(sc->cdf, cdf);
     upload(sc->areas, areas);
     upload(sc->directional, dirs);
+    upload(sc->uvs, uvs);
+    upload(sc->textures, texs);
+    upload(sc->texels, texels);
     GpuScene& g     = sc->gpu;
     g.nodes         = sc->nodes.as<GpuNode4>();
     g.tris          = sc->tris.as<GpuTri>();
@@ -467,6 +496,9 @@ mrg_scene* build_scene(mrg_device* d, const mrg_scene_desc& s, const std::string
     g.cdf           = sc->cdf.as<float>();
     g.areas         = sc->areas.as<GpuAreaLight>();
     g.directional   = sc->directional.as<GpuDirectional>();
+    g.uvs           = s.uvs ? sc->uvs.as<float>() : nullptr;
+    g.textures      = sc->textures.as<GpuTexture>();
+    g.texels        = sc->texels.as<float>();
     g.node_count    = uint32_t(bvh.nodes.size());
     g.tri_count     = uint32_t(n);
     g.choice_count  = uint32_t(choices.size());
@@ -479,7 +511,8 @@ mrg_scene* build_scene(mrg_device* d, const mrg_scene_desc& s, const std::string
     info.leaves          = bvh.leaves;
     info.maximum_depth   = bvh.max_depth;
     info.device_bytes    = sc->nodes.bytes + sc->tris.bytes + sc->normals.bytes + sc->slot_instance.bytes + sc->slot_emitter.bytes +
-                        sc->materials.bytes + sc->choices.bytes + sc->cdf.bytes + sc->areas.bytes + sc->directional.bytes;
+                        sc->materials.bytes + sc->choices.bytes + sc->cdf.bytes + sc->areas.bytes + sc->directional.bytes +
+                        sc->uvs.bytes + sc->textures.bytes + sc->texels.bytes;
     info.build_seconds = seconds_since(t0);
     std::snprintf(info.content_hash, sizeof info.content_hash, "%s", hash.c_str());
     return sc.release();

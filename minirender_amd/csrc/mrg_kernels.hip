@@ -589,6 +589,20 @@ __global__ void __launch_bounds__(kShadeThreads) __attribute__((amdgpu_waves_per
                     shading = normalize(shading);
                     if (dot(shading, geometric) < 0.0f) shading = -shading;
                 }
+                // base color: the material's, times the clamped texel at the hit uv = sum of b_i uv_i
+                // (bvh.cpp:318-321, path_tracer.cpp:279-282, bsdf.cpp:122-125)
+                F3 base_color = ld3(mat.color);
+                if (mat.texture >= 0) {
+                    float u = 0.0f, v = 0.0f;
+                    if (sc.uvs) {
+                        const float2* q  = reinterpret_cast<const float2*>(sc.uvs) + static_cast<uint64_t>(slot) * 3u;
+                        const float2  q0 = gld(q + 0), q1 = gld(q + 1), q2 = gld(q + 2);
+                        u = ((u + b0 * q0.x) + b1 * q1.x) + b2 * q2.x;
+                        v = ((v + b0 * q0.y) + b1 * q1.y) + b2 * q2.y;
+                    }
+                    const F3 tc = texture_sample(sc, mat.texture, mat.filter, u, v);
+                    base_color  = base_color * mk(sclamp(tc.x, 0.0f, 1.0f), sclamp(tc.y, 0.0f, 1.0f), sclamp(tc.z, 0.0f, 1.0f));
+                }
                 const bool front_face = dot(ray_d, geometric) < 0.0f;
                 const F3   perr = absolute * (7.0f * kEpsF);
                 const F3   ng   = geometric;
@@ -610,7 +624,7 @@ __global__ void __launch_bounds__(kShadeThreads) __attribute__((amdgpu_waves_per
                 const F3    wo = to_local(f, wo_world);
                 BsdfParams  bp;
                 bp.kind  = mat.kind;
-                bp.color = ld3(mat.color);
+                bp.color = base_color;
                 bp.alpha = mat.alpha;
                 bp.eta_i = front_face ? 1.0f : mat.ior;
                 bp.eta_t = front_face ? mat.ior : 1.0f;

@@ -9,6 +9,8 @@
 //   normals    [n]             GpuTri layout, per-corner world normals in leaf order.
 //   slot_info  [n]             uint2 (instance, area-light choice or -1) in leaf order.
 //   materials  [instances]     GpuMaterial.
+//   uvs        [n]             per-corner texture coordinates in leaf order (3 x float2), if any.
+//   textures                   level-0 texels of the base-color textures (float4 each).
 //   lights                     light choices, area emitters, directional lights and the selection
 //                              CDF, all float, built on the host exactly as the FP32 twin builds them.
 #pragma once
@@ -78,6 +80,13 @@ struct GpuMaterial {
     uint32_t pad;
 };
 
+// Level 0 of a base-color texture: texels[offset + y * width + x] (float4, rgb + 0), row 0 at the
+// top of the image.
+struct GpuTexture {
+    uint64_t offset;
+    uint32_t width, height;
+};
+
 struct GpuAreaLight {
     float    v0[3], v1[3], v2[3];
     float    normal[3];
@@ -109,6 +118,9 @@ struct GpuScene {
     const float*          cdf;          // [choice_count + 1]
     const GpuAreaLight*   areas;
     const GpuDirectional* directional;
+    const float*          uvs;          // [tri_count][3][2] per-corner texture coordinates in leaf order, or null
+    const GpuTexture*     textures;
+    const float*          texels;       // float4 per texel
     uint32_t              node_count;
     uint32_t              tri_count;
     uint32_t              choice_count;

@@ -61,6 +61,11 @@ __device__ __forceinline__ uint4 gld(const uint4* p) {
     const gu4 v = *(const __attribute__((address_space(1))) gu4*)p;
     return make_uint4(v.x, v.y, v.z, v.w);
 }
+typedef float    gf2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 gld(const float2* p) {
+    const gf2 v = *(const __attribute__((address_space(1))) gf2*)p;
+    return make_float2(v.x, v.y);
+}
 __device__ __forceinline__ float gld(const float* p) { return *(const __attribute__((address_space(1))) float*)p; }
 __device__ __forceinline__ uint32_t gld(const uint32_t* p) { return *(const __attribute__((address_space(1))) uint32_t*)p; }
 __device__ __forceinline__ void gst(float* p, float v) { *(__attribute__((address_space(1))) float*)p = v; }

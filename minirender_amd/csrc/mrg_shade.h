@@ -124,6 +124,37 @@ __device__ __forceinline__ F3 sample_vndf(F3 wo, float ux, float uy, float alpha
     return normalize(mk(alpha * n.x, alpha * n.y, smax(0.0f, n.z)));
 }
 
+// --- base-color texture (texture.cpp:74-137), float arithmetic with repeat addressing; uv has its
+// origin at the bottom-left, the image at the top-left. Trilinear filtering at lod 0 (the only lod
+// the path tracer asks for, path_tracer.cpp:279-282) is the level-0 bilinear value.
+__device__ __forceinline__ int wrap_texel(int c, int extent) { return (c % extent + extent) % extent; }
+
+__device__ __forceinline__ F3 texel(const GpuTexture& t, const float* texels, int x, int y) {
+    const float4 v = gld(reinterpret_cast<const float4*>(texels) + t.offset +
+                         static_cast<uint64_t>(wrap_texel(y, int(t.height))) * t.width + uint32_t(wrap_texel(x, int(t.width))));
+    return mk(v.x, v.y, v.z);
+}
+
+__device__ __forceinline__ F3 texture_sample(const GpuScene& sc, int32_t index, uint32_t filter, float u, float v) {
+    const GpuTexture t  = sc.textures[index];
+    const float      cu = u - __builtin_floorf(u);
+    const float      cv = 1.0f - (v - __builtin_floorf(v));
+    if (filter == 0u) {
+        const int x = static_cast<int>(__builtin_floorf(cu * static_cast<float>(t.width)));
+        const int y = static_cast<int>(__builtin_floorf(cv * static_cast<float>(t.height)));
+        return texel(t, sc.texels, x, y);
+    }
+    const float x  = cu * static_cast<float>(t.width) - 0.5f;
+    const float y  = cv * static_cast<float>(t.height) - 0.5f;
+    const float fx = __builtin_floorf(x), fy = __builtin_floorf(y);
+    const int   ix = static_cast<int>(fx), iy = static_cast<int>(fy);
+    const float tx = x - fx, ty = y - fy;
+    const float sx = 1.0f - tx, sy = 1.0f - ty;
+    const F3    top    = texel(t, sc.texels, ix, iy) * sx + texel(t, sc.texels, ix + 1, iy) * tx;
+    const F3    bottom = texel(t, sc.texels, ix, iy + 1) * sx + texel(t, sc.texels, ix + 1, iy + 1) * tx;
+    return top * sy + bottom * ty;
+}
+
 // --- Kulla-Conty energy compensation (microfacet_energy.cpp:69-89, bsdf.cpp:128-189)
 // Bilinear lookup of the energy table ([64][64] float4 nodes (A, B, average, 0); x = cos(theta),
 // y = roughness): returns E = clamp(A + B, 0, 1) and the row average clamped to [0, 1].

@@ -226,6 +226,42 @@ def mixed_terrain(cells=1024, spheres=64, sphere_level=3, seed=3, width=1920, he
     return concat(parts, mats, environment=(0.35, 0.45, 0.65), directional=sun, camera=cam, name="mixed")
 
 
+def textured_scene(sphere_level=3, seed=6) -> Scene:
+    """Base-color textures on a floor (uv tiled 3x, so repeat addressing wraps), a back wall and three
+    spheres (latitude-longitude uv) of diffuse, rough conductor and glass material, one per filter,
+    under a sky and a sun."""
+    r = np.random.default_rng(seed)
+    checker = np.zeros((8, 8, 3), np.float32)
+    checker[(np.arange(8)[:, None] + np.arange(8)[None, :]) % 2 == 0] = (0.9, 0.2, 0.1)
+    checker[(np.arange(8)[:, None] + np.arange(8)[None, :]) % 2 == 1] = (0.1, 0.6, 0.9)
+    noise = r.random((12, 20, 3)).astype(np.float32) * 1.2 - 0.1  # values outside [0, 1] exercise the clamp
+    textures = [checker, noise]
+    parts, mats, uvs = [], [], []
+    floor = np.array([[(-4, 0, -4), (-4, 0, 4), (4, 0, 4)], [(-4, 0, -4), (4, 0, 4), (4, 0, -4)]], np.float64)
+    parts.append((floor, np.tile(np.array([0, 1, 0], np.float64), (2, 3, 1))))
+    uvs.append((floor[..., [0, 2]] + 4) / 8 * 3 - 0.3)
+    mats.append(material("diffuse", (0.9, 0.9, 0.9), base_color_texture=0, texture_filter="nearest"))
+    wall = np.array([[(-4, 0, -3), (4, 0, -3), (4, 4, -3)], [(-4, 0, -3), (4, 4, -3), (-4, 4, -3)]], np.float64)
+    parts.append((wall, np.tile(np.array([0, 0, 1], np.float64), (2, 3, 1))))
+    uvs.append((wall[..., [0, 1]] + (4, 0)) / 8)
+    mats.append(material("diffuse", (1.0, 1.0, 1.0), base_color_texture=1, texture_filter="bilinear"))
+    unit = subdivided_sphere(sphere_level)
+    lon = np.arctan2(unit[..., 2], unit[..., 0]) / (2 * math.pi) + 0.5
+    lat = np.arccos(np.clip(unit[..., 1], -1, 1)) / math.pi
+    for i, (kind, tex, filt) in enumerate((("diffuse", 1, "trilinear"), ("conductor", 0, "bilinear"),
+                                            ("dielectric", 1, "nearest"))):
+        parts.append((unit * 0.7 + (-1.6 + 1.6 * i, 0.7, 0.0), unit))
+        uvs.append(np.stack([lon * 2, 1 - lat], -1))
+        mats.append(material(kind, (0.95, 0.9, 0.85), roughness=0.3, base_color_texture=tex, texture_filter=filt))
+    cam = Camera(position=(0, 2.0, 5.5), target=(0, 0.7, 0), up=(0, 1, 0), vertical_fov_degrees=45)
+    sun = directional_lights(((-0.4, -1.0, -0.6), (1.0, 0.95, 0.9), 2.5))
+    sc = concat(parts, mats, environment=(0.3, 0.35, 0.45), directional=sun, camera=cam, name="textured")
+    sc.uvs = np.concatenate([np.asarray(u, np.float32).reshape(-1, 3, 2) for u in uvs])
+    sc.textures = textures
+    sc.__post_init__()
+    return sc
+
+
 def furnace_spheres(sphere_level=3, roughness=(0.25, 0.65, 1.0), environment=(1.0, 1.0, 1.0), sun=True) -> Scene:
     """Two rows of rough conductor spheres, single-scattering above and Kulla-Conty compensated
     below (the layout of the reference's scenes/white-furnace.json), on a diffuse floor under a

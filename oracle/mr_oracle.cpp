@@ -481,9 +481,100 @@ template <class R> struct Lights {
 
 template <class R> R luminance(V3<R> c) { return dot(c, V3<R>{R(0.2126), R(0.7152), R(0.0722)}); }
 
+// ----------------------------------------------------------------------------------------------
+// Texture (texture.cpp): an immutable linear-RGB mip chain, level 0 = the decoded image (row 0 at
+// the top), each further level the pixel-area average of the previous one in FP64 (odd sizes keep
+// their last row and column). Sampling is float arithmetic with repeat addressing; uv has its
+// origin at the bottom-left (OBJ convention), images at the top-left.
+struct TexLevel {
+    uint32_t               w = 0, h = 0;
+    std::vector<V3<float>> px;  // [y * w + x]
+    const V3<float>& at(uint32_t x, uint32_t y) const { return px[static_cast<size_t>(y) * w + x]; }
+};
+
+inline TexLevel downsample(const TexLevel& src) {
+    TexLevel t;
+    t.w = std::max<uint32_t>(1, src.w / 2);
+    t.h = std::max<uint32_t>(1, src.h / 2);
+    t.px.resize(static_cast<size_t>(t.w) * t.h);
+    for (uint32_t y = 0; y < t.h; ++y) {
+        for (uint32_t x = 0; x < t.w; ++x) {
+            const double left   = double(x) * double(src.w) / double(t.w);
+            const double right  = double(x + 1) * double(src.w) / double(t.w);
+            const double top    = double(y) * double(src.h) / double(t.h);
+            const double bottom = double(y + 1) * double(src.h) / double(t.h);
+            V3<double>   sum{0, 0, 0};
+            double       weight_sum = 0;
+            for (size_t sy = static_cast<size_t>(top); sy < static_cast<size_t>(std::ceil(bottom)); ++sy) {
+                for (size_t sx = static_cast<size_t>(left); sx < static_cast<size_t>(std::ceil(right)); ++sx) {
+                    const double weight = (std::min(right, double(sx + 1)) - std::max(left, double(sx))) *
+                                          (std::min(bottom, double(sy + 1)) - std::max(top, double(sy)));
+                    sum        = sum + widen<double>(src.at(uint32_t(sx), uint32_t(sy))) * weight;
+                    weight_sum += weight;
+                }
+            }
+            t.px[static_cast<size_t>(y) * t.w + x] = to_float(sum / weight_sum);
+        }
+    }
+    return t;
+}
+
+struct Texture {
+    std::vector<TexLevel> levels;
+    Texture(uint32_t w, uint32_t h, const float* rgb) {
+        if (w == 0 || h == 0) throw std::invalid_argument("Texture must not be empty");
+        if (w > 16384 || h > 16384 || uint64_t(w) * h > 64ull * 1024 * 1024)
+            throw std::invalid_argument("Texture exceeds the 64-megapixel budget");
+        TexLevel l0;
+        l0.w = w;
+        l0.h = h;
+        l0.px.resize(static_cast<size_t>(w) * h);
+        for (size_t i = 0; i < l0.px.size(); ++i) {
+            l0.px[i] = {rgb[i * 3], rgb[i * 3 + 1], rgb[i * 3 + 2]};
+            if (!finite(l0.px[i])) throw std::invalid_argument("Texture contains nonfinite samples");
+        }
+        levels.push_back(std::move(l0));
+        while (levels.back().w > 1 || levels.back().h > 1) levels.push_back(downsample(levels.back()));
+    }
+    static int wrap(int c, int extent) { return (c % extent + extent) % extent; }
+    static V2<float> image_uv(V2<float> uv) {
+        if (!std::isfinite(uv.x) || !std::isfinite(uv.y)) throw std::invalid_argument("Texture coordinates must be finite");
+        return {uv.x - std::floor(uv.x), 1.0f - (uv.y - std::floor(uv.y))};
+    }
+    static V3<float> mix(V3<float> a, V3<float> b, float t) { return a * (1.0f - t) + b * t; }
+    V3<float> bilinear(V2<float> uv, size_t level) const {
+        const TexLevel& im = levels.at(level);
+        const V2<float> c  = image_uv(uv);
+        const float     x  = c.x * static_cast<float>(im.w) - 0.5f;
+        const float     y  = c.y * static_cast<float>(im.h) - 0.5f;
+        const int       ix = static_cast<int>(std::floor(x)), iy = static_cast<int>(std::floor(y));
+        const float     tx = x - std::floor(x), ty = y - std::floor(y);
+        auto texel = [&](int px, int py) { return im.at(uint32_t(wrap(px, int(im.w))), uint32_t(wrap(py, int(im.h)))); };
+        return mix(mix(texel(ix, iy), texel(ix + 1, iy), tx), mix(texel(ix, iy + 1), texel(ix + 1, iy + 1), tx), ty);
+    }
+    V3<float> sample(V2<float> uv, float lod, uint32_t filter) const {
+        if (!std::isfinite(lod)) throw std::invalid_argument("Texture LOD must be finite");
+        if (filter == 0) {
+            const V2<float> c  = image_uv(uv);
+            const TexLevel& im = levels.front();
+            const int       x  = static_cast<int>(std::floor(c.x * static_cast<float>(im.w)));
+            const int       y  = static_cast<int>(std::floor(c.y * static_cast<float>(im.h)));
+            return im.at(uint32_t(wrap(x, int(im.w))), uint32_t(wrap(y, int(im.h))));
+        }
+        if (filter == 1) return bilinear(uv, 0);
+        if (filter != 2) throw std::invalid_argument("Unknown texture filter");
+        const float  clamped = std::clamp(lod, 0.0f, static_cast<float>(levels.size() - 1));
+        const size_t lower   = static_cast<size_t>(std::floor(clamped));
+        const size_t upper   = std::min(lower + 1, levels.size() - 1);
+        return mix(bilinear(uv, lower), bilinear(uv, upper), clamped - static_cast<float>(lower));
+    }
+};
+
 struct Scene {
     std::vector<std::array<V3<float>, 3>> pos;
     std::vector<std::array<V3<float>, 3>> nrm;
+    std::vector<std::array<V2<float>, 3>> uv;   // per-corner texture coordinates (empty: all zero)
+    std::vector<Texture>                  textures;
     std::vector<uint32_t>                 inst, tri;
     std::vector<Material>                 materials;
     V3<float>                             environment;
@@ -651,6 +742,7 @@ template <class R> struct SurfaceHit {
     R         t;
     V3<R>     position;
     V3<float> geometric_normal, shading_normal;
+    V2<float> uv;
     uint32_t  prim;
     bool      front_face;
     V3<R>     position_error;
@@ -664,9 +756,14 @@ template <class R> SurfaceHit<R> surface_hit(const Scene& s, const Ray<R>& ray, 
     V3<R>       geometric = normalize(cross(e1, e2));
     if (s.materials[s.inst[h.prim]].mirrored) geometric = -geometric;
     V3<R> shading{0, 0, 0}, position{0, 0, 0}, absolute{0, 0, 0};
+    V2<R> uv{0, 0};
     for (int k = 0; k < 3; ++k) {
         const R w = h.hit.bary[k];
         shading   = shading + w * normalize(widen<R>(N[k]));
+        if (!s.uv.empty()) {
+            uv.x = uv.x + w * R(s.uv[h.prim][k].x);
+            uv.y = uv.y + w * R(s.uv[h.prim][k].y);
+        }
         const V3<R> term = w * widen<R>(P[k]);
         position         = position + term;
         absolute         = absolute + vabs(term);
@@ -682,6 +779,7 @@ template <class R> SurfaceHit<R> surface_hit(const Scene& s, const Ray<R>& ray, 
     out.position         = position;
     out.geometric_normal = to_float(geometric);
     out.shading_normal   = to_float(shading);
+    out.uv               = {static_cast<float>(uv.x), static_cast<float>(uv.y)};
     out.prim             = h.prim;
     out.front_face       = dot(ray.d, geometric) < 0;
     out.position_error   = absolute * (R(7) * std::numeric_limits<R>::epsilon());
@@ -1133,7 +1231,10 @@ V3<R> trace(const Scene& s, const Settings& cfg, Ray<R> ray, Sampler& sampler, R
         if (dot(normal, wo_world) < 0) normal = -normal;
         const Frame<R> frame(normal);
         const V3<R>    wo = frame.to_local(wo_world);
-        const Bsdf<R>  bsdf(mat, hit.front_face);
+        const V3<float> texture_color =
+            mat.base_color_texture >= 0 ? s.textures.at(size_t(mat.base_color_texture)).sample(hit.uv, 0.0f, mat.texture_filter)
+                                        : V3<float>{1, 1, 1};
+        const Bsdf<R>  bsdf(mat, hit.front_face, texture_color);
         const R        light_u   = sampler.next_1d<R>();
         const R        su_x      = sampler.next_1d<R>();
         const R        su_y      = sampler.next_1d<R>();
@@ -1250,10 +1351,32 @@ template <class F> static int guarded(F&& f) {
 }
 
 static Material to_material(const OrcMaterial& m) {
-    if (m.base_color_texture != 0) throw std::invalid_argument("textures are not supported by this oracle build");
     return Material{m.kind, {m.base_color[0], m.base_color[1], m.base_color[2]}, {m.emission[0], m.emission[1], m.emission[2]},
                     m.roughness, m.ior, m.double_sided != 0, m.mirrored != 0, m.compensate_energy != 0,
                     static_cast<int32_t>(m.base_color_texture) - 1, m.texture_filter};
+}
+
+extern "C" int orc_texture_sample(uint32_t w, uint32_t h, const float* rgb, const float* uv, uint64_t n, float lod, uint32_t filter,
+                                  float* out) {
+    return guarded([&] {
+        const Texture t(w, h, rgb);
+        for (uint64_t i = 0; i < n; ++i) {
+            const V3<float> v = t.sample(V2<float>{uv[i * 2], uv[i * 2 + 1]}, lod, filter);
+            out[i * 3] = v.x; out[i * 3 + 1] = v.y; out[i * 3 + 2] = v.z;
+        }
+    });
+}
+
+extern "C" int orc_texture_level(uint32_t w, uint32_t h, const float* rgb, uint32_t level, uint32_t* out_w, uint32_t* out_h,
+                                 float* out) {
+    return guarded([&] {
+        const Texture   t(w, h, rgb);
+        const TexLevel& l = t.levels.at(level);
+        *out_w = l.w;
+        *out_h = l.h;
+        if (out)
+            for (size_t i = 0; i < l.px.size(); ++i) { out[i * 3] = l.px[i].x; out[i * 3 + 1] = l.px[i].y; out[i * 3 + 2] = l.px[i].z; }
+    });
 }
 
 extern "C" void orc_energy_table(float* out) {
@@ -1294,6 +1417,15 @@ extern "C" void* orc_scene_new(const OrcSceneDesc* d) {
             const auto& m = d->materials[i];
             sc->materials.push_back(to_material(m));
         }
+        for (uint32_t i = 0; i < d->texture_count; ++i) sc->textures.emplace_back(d->textures[i].width, d->textures[i].height, d->textures[i].rgb);
+        for (const auto& m : sc->materials)
+            if (m.base_color_texture >= int64_t(sc->textures.size())) throw std::invalid_argument("Material references missing texture");
+        if (d->uvs) {
+            sc->uv.resize(d->triangle_count);
+            for (uint64_t p = 0; p < d->triangle_count; ++p)
+                for (int k = 0; k < 3; ++k) sc->uv[p][k] = {d->uvs[p * 6 + k * 2], d->uvs[p * 6 + k * 2 + 1]};
+        }
+        if (d->environment_texture != 0) throw std::invalid_argument("environment maps are not supported by this oracle build");
         sc->environment = {d->environment[0], d->environment[1], d->environment[2]};
         sc->directional.assign(d->directional, d->directional + d->directional_count);
         sc->build();

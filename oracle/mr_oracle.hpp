@@ -140,6 +140,13 @@ int orc_trace(void* scene, int precision, const OrcRender* settings, const doubl
 // one node's Fresnel-split albedo (ibl.cpp:240-277) and store_average_energy of a one-row table
 // whose A channel is e[] (B = 0); returns the row average.
 void   orc_energy_table(float* out);
+// Texture (texture.cpp) of a [h][w][3] linear image: sample n uvs ([n][2]) at lod with filter
+// (0 nearest, 1 bilinear, 2 trilinear) into out [n][3]; or read mip level `level` (out may be null
+// to query its size). Return 0 on success.
+int    orc_texture_sample(uint32_t w, uint32_t h, const float* rgb, const float* uv, uint64_t n, float lod,
+                          uint32_t filter, float* out);
+int    orc_texture_level(uint32_t w, uint32_t h, const float* rgb, uint32_t level, uint32_t* out_w,
+                         uint32_t* out_h, float* out);
 void   orc_integrate_ibl_brdf(double mu, double roughness, uint32_t samples, uint64_t seed, double* out);
 double orc_average_energy_of_ramp(const float* e, uint32_t width);
 

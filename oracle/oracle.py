@@ -102,6 +102,8 @@ def lib():
         L.orc_light_count.argtypes = [P]
         L.orc_camera_ray.argtypes = [P, C.c_int, P, C.c_double, P, P]
         L.orc_energy_table.argtypes = [P]
+        L.orc_texture_sample.argtypes = [C.c_uint32, C.c_uint32, P, P, C.c_uint64, C.c_float, C.c_uint32, P]
+        L.orc_texture_level.argtypes = [C.c_uint32, C.c_uint32, P, C.c_uint32, P, P, P]
         L.orc_integrate_ibl_brdf.argtypes = [C.c_double, C.c_double, C.c_uint32, C.c_uint64, P]
         L.orc_average_energy_of_ramp.restype = C.c_double
         L.orc_average_energy_of_ramp.argtypes = [P, C.c_uint32]
@@ -374,3 +376,26 @@ def integrate_ibl_brdf(mu, roughness, samples, seed=101):
 def average_energy_of_ramp(e):
     e = np.ascontiguousarray(e, np.float32)
     return lib().orc_average_energy_of_ramp(_ptr(e), len(e))
+
+
+FILTERS = {"nearest": 0, "bilinear": 1, "trilinear": 2}
+
+
+def texture_sample(image, uv, lod=0.0, filter="trilinear"):
+    """Texture::sample (texture.cpp:115-137) of a linear [h][w][3] image at uvs [n][2] (float)."""
+    im = np.ascontiguousarray(image, np.float32)
+    uv = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+    out = np.zeros((len(uv), 3), np.float32)
+    _check(lib().orc_texture_sample(im.shape[1], im.shape[0], _ptr(im), _ptr(uv), len(uv), float(lod), FILTERS[filter],
+                                    _ptr(out)))
+    return out
+
+
+def texture_level(image, level):
+    """Mip level `level` of the texture made from a linear [h][w][3] image."""
+    im = np.ascontiguousarray(image, np.float32)
+    w, h = C.c_uint32(), C.c_uint32()
+    _check(lib().orc_texture_level(im.shape[1], im.shape[0], _ptr(im), level, C.byref(w), C.byref(h), None))
+    out = np.zeros((h.value, w.value, 3), np.float32)
+    _check(lib().orc_texture_level(im.shape[1], im.shape[0], _ptr(im), level, C.byref(w), C.byref(h), _ptr(out)))
+    return out

@@ -141,12 +141,23 @@ def read_flat_scene(path, name="flattened"):
     p += ni * S.MATERIAL_DTYPE.itemsize
     dirs = np.frombuffer(b, S.DIRECTIONAL_DTYPE, nd, p)
     p += nd * S.DIRECTIONAL_DTYPE.itemsize
+    uvs = np.frombuffer(b, "<f4", n * 6, p).reshape(n, 3, 2)
+    p += n * 24
+    (nt,) = struct.unpack_from("<I", b, p)
+    p += 4
+    textures = []
+    for _ in range(nt):
+        w, h = struct.unpack_from("<II", b, p)
+        p += 8
+        textures.append(np.frombuffer(b, "<f4", w * h * 3, p).reshape(h, w, 3).copy())
+        p += w * h * 12
     assert p == len(b)
     camera = S.Camera(tuple(map(float, cam["position"])), tuple(map(float, cam["target"])), tuple(map(float, cam["up"])),
                       float(cam["vertical_fov_degrees"]), float(cam["near_plane"]), float(cam["far_plane"]),
                       float(cam["aperture_radius"]), float(cam["focus_distance"]))
     return S.Scene(name=name, positions=pos.copy(), normals=nrm.copy(), instance=inst.copy(), triangle=tri.copy(),
-                   materials=mats.copy(), environment=tuple(map(float, env)), directional=dirs.copy(), camera=camera)
+                   materials=mats.copy(), environment=tuple(map(float, env)), directional=dirs.copy(), camera=camera,
+                   uvs=uvs.copy(), textures=textures)
 
 
 def read_obj_dump(path):

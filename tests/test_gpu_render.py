@@ -84,3 +84,24 @@ def test_energy_compensated_conductors_match_fp32_twin(device, oracle, mode):
     img, twin, prog, st = _render_pair(device, oracle, scene, 40, 32, 8, 6, mode=mode, seed=9)
     _check(img, twin)
     assert prog.statistics["scattering_events"] == st["scattering_events"]
+
+
+@pytest.mark.parametrize("mode", ["mis", "light"])
+def test_base_color_textures_match_fp32_twin(device, oracle, mode):
+    """Textured diffuse, conductor and glass (nearest, bilinear and trilinear filters, uv repeat,
+    texels outside [0, 1] clamped) against the FP32 twin within 1e-4."""
+    scene = scenegen.textured_scene()
+    img, twin, prog, st = _render_pair(device, oracle, scene, 48, 32, 6, 6, mode=mode, seed=21)
+    _check(img, twin)
+    assert prog.statistics["shadow_rays"] == st["shadow_rays"]
+
+
+def test_textures_change_the_image(device):
+    """The same scene without its textures renders differently (the texture path is live)."""
+    scene = scenegen.textured_scene()
+    rs, ins = api.RenderSettings(24, 16, 2, 3), api.IntegratorSettings("mis", 4, True, 3)
+    a, _ = api.render(device, scene, rs, ins)
+    plain = scenegen.textured_scene()
+    plain.materials["base_color_texture"] = 0
+    b, _ = api.render(device, plain, rs, ins)
+    assert np.abs(a - b).max() > 1e-2

@@ -214,10 +214,10 @@ def test_reference_scenes():
     assert "triangles: 26\n" in cli("--scene-info", os.path.join(REF, "studio.json")).stdout
     renderable = {"cornell-box.json": 36, "cornell-small-light.json": 36, "path-room.json": 36, "soft-shadows.json": 26,
                   "studio-two-lights.json": 26, "cornell-materials.json": 4428, "material-balls.json": 13252,
-                  "white-furnace.json": 13248}
+                  "white-furnace.json": 13248, "studio-textured.json": 26, "lightfield-charts.json": 4}
     for name, tris in renderable.items():
         assert f"triangles: {tris}\n" in cli("--scene-info", os.path.join(REF, name)).stdout, name
-    unsupported = {"sss-wax.json": "subsurface", "volume-cornell.json": "media", "studio-textured.json": "textures"}
+    unsupported = {"sss-wax.json": "subsurface", "volume-cornell.json": "media"}
     for name, word in unsupported.items():
         r = cli("--scene-info", os.path.join(REF, name), check=False)
         assert r.returncode == 1 and word in r.stderr, name

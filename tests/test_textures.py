@@ -1,0 +1,161 @@
+"""Base-color textures: the command line's PNG decoder (Image::read, raster_io.cpp:40-113) and the
+oracle's Texture (texture.cpp), pinned by the reference's own known answers (texture_tests.cpp:40-78,
+on the reference's checker.png / linear16.png kept as fixtures) and by PNG files written here from
+known pixel arrays with every filter type, bit depth and colour type the decoder accepts."""
+import os
+import struct
+import subprocess
+import zlib
+
+import numpy as np
+import pytest
+
+from minirender_amd import build as B
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def decode(path, encoding="srgb", check=True):
+    out = str(path) + ".bin"
+    r = subprocess.run([B.RENDER_CLI, "--texture", str(path), encoding, out], capture_output=True, text=True, timeout=60)
+    if not check:
+        return r
+    assert r.returncode == 0, r.stderr
+    b = open(out, "rb").read()
+    w, h = struct.unpack_from("<II", b, 0)
+    return np.frombuffer(b, "<f4", w * h * 3, 8).reshape(h, w, 3)
+
+
+def _chunk(tag, data):
+    return struct.pack(">I", len(data)) + tag + data + struct.pack(">I", zlib.crc32(tag + data) & 0xFFFFFFFF)
+
+
+def _paeth(a, b, c):
+    p = a + b - c
+    pa, pb, pc = abs(p - a), abs(p - b), abs(p - c)
+    return a if pa <= pb and pa <= pc else (b if pb <= pc else c)
+
+
+def write_png(path, samples, color_type, depth, palette=None):
+    """samples: [h][w][channels] integers; rows filtered with filter (row % 5)."""
+    h, w, ch = samples.shape
+    bpp = max(1, ch * depth // 8)
+    raw = bytearray()
+    prev = bytearray(w * bpp)
+    for y in range(h):
+        row = bytearray()
+        for x in range(w):
+            for c in range(ch):
+                v = int(samples[y, x, c])
+                row += struct.pack(">H", v) if depth == 16 else bytes([v])
+        f = y % 5
+        out = bytearray()
+        for i in range(len(row)):
+            a = row[i - bpp] if i >= bpp else 0
+            up = prev[i]
+            cc = prev[i - bpp] if i >= bpp else 0
+            pred = [0, a, up, (a + up) // 2, _paeth(a, up, cc)][f]
+            out.append((row[i] - pred) & 0xFF)
+        raw += bytes([f]) + out
+        prev = row
+    png = b"\x89PNG\r\n\x1a\n" + _chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, depth, color_type, 0, 0, 0))
+    if palette is not None:
+        png += _chunk(b"PLTE", bytes(np.asarray(palette, np.uint8).reshape(-1)))
+    png += _chunk(b"IDAT", zlib.compress(bytes(raw))) + _chunk(b"IEND", b"")
+    open(path, "wb").write(png)
+
+
+def srgb_to_linear(v):
+    v = np.float32(v)
+    return np.where(v <= np.float32(0.04045), v / np.float32(12.92),
+                    np.power((v + np.float32(0.055)) / np.float32(1.055), np.float32(2.4))).astype(np.float32)
+
+
+@pytest.mark.parametrize("color_type,depth,channels", [(0, 8, 1), (2, 8, 3), (4, 8, 2), (6, 8, 4), (0, 16, 1), (2, 16, 3),
+                                                       (6, 16, 4), (3, 8, 1)])
+def test_png_decoding_matches_known_pixels(tmp_path, color_type, depth, channels):
+    rng = np.random.default_rng(color_type * 100 + depth)
+    h, w = 11, 7
+    top = 65535 if depth == 16 else 255
+    palette = None
+    if color_type == 3:
+        palette = rng.integers(0, 256, (16, 3))
+        samples = rng.integers(0, 16, (h, w, 1))
+        expect = palette[samples[..., 0]].astype(np.float32) / np.float32(255)
+    else:
+        samples = rng.integers(0, top + 1, (h, w, channels))
+        rgb = samples[..., :1].repeat(3, axis=-1) if channels <= 2 else samples[..., :3]
+        expect = rgb.astype(np.float32) / np.float32(top)
+    path = tmp_path / f"t{color_type}_{depth}.png"
+    write_png(path, samples, color_type, depth, palette)
+    np.testing.assert_array_equal(decode(path, "linear"), expect)
+
+
+def test_srgb_decoding(tmp_path):
+    samples = np.arange(256).reshape(16, 16, 1)
+    write_png(tmp_path / "ramp.png", samples, 0, 8)
+    got = decode(tmp_path / "ramp.png", "srgb")[..., 0].reshape(-1)
+    want = srgb_to_linear(np.arange(256, dtype=np.float32) / np.float32(255))
+    ulps = np.abs(got.view(np.int32).astype(np.int64) - want.view(np.int32).astype(np.int64))
+    assert ulps.max() <= 1  # pow is the C library's on both sides
+    assert got[0] == 0 and got[255] == 1
+
+
+def test_reference_fixtures_decode():
+    """texture_tests.cpp:40-45: linear16.png at (1, 0) = 32768/65535; checker.png's first texel is red."""
+    lin = decode(os.path.join(GOLDEN, "linear16.png"), "linear")
+    assert abs(lin[0, 1, 0] - 32768.0 / 65535.0) < 1e-6
+    chk = decode(os.path.join(GOLDEN, "checker.png"), "srgb")
+    assert chk.shape == (8, 8, 3) and tuple(chk[0, 0]) == (1.0, 0.0, 0.0)
+
+
+def test_unsupported_texture_formats_are_refused(tmp_path):
+    (tmp_path / "x.jpg").write_bytes(b"\xff\xd8\xff")
+    r = decode(tmp_path / "x.jpg", check=False)
+    assert r.returncode == 1 and "not supported" in r.stderr
+    write_png(tmp_path / "bad.png", np.zeros((2, 2, 1), int), 0, 8)
+    b = bytearray(open(tmp_path / "bad.png", "rb").read())
+    b[28] = 1  # interlaced
+    open(tmp_path / "bad.png", "wb").write(bytes(b))
+    r = decode(tmp_path / "bad.png", check=False)
+    assert r.returncode == 1 and "Interlaced" in r.stderr
+
+
+def test_texture_filters_and_mips_of_the_checker(oracle):
+    """texture_tests.cpp:51-63 on the oracle's Texture."""
+    chk = decode(os.path.join(GOLDEN, "checker.png"), "srgb")
+    assert [oracle.texture_level(chk, k).shape[:2] for k in range(4)] == [(8, 8), (4, 4), (2, 2), (1, 1)]
+    assert tuple(oracle.texture_sample(chk, [0.0625, 0.9375], filter="nearest")[0]) == (1, 0, 0)
+    assert tuple(oracle.texture_sample(chk, [1.0625, -0.0625], filter="nearest")[0]) == (1, 0, 0)
+    assert np.linalg.norm(oracle.texture_sample(chk, [0.125, 0.875], filter="bilinear")[0] - 0.5) < 1e-6
+    assert tuple(oracle.texture_level(chk, 1)[0, 0]) == (0.5, 0.5, 0.5)
+    assert np.linalg.norm(oracle.texture_sample(chk, [0.0625, 0.9375], lod=0.5)[0] - [0.75, 0.25, 0.25]) < 1e-6
+    assert tuple(oracle.texture_sample(chk, [12.1, -31.2], lod=20)[0]) == (0.5, 0.5, 0.5)
+
+
+def test_odd_mip_dimensions_keep_last_row_and_column(oracle):
+    """texture_tests.cpp:65-78."""
+    img = np.zeros((5, 3, 3), np.float32)
+    for y in range(5):
+        for x in range(3):
+            img[y, x] = (x, y, 1.0 if (x == 2 or y == 4) else 0.0)
+    top = oracle.texture_level(img, 2)
+    assert top.shape[:2] == (1, 1)
+    np.testing.assert_allclose(top[0, 0], [1, 2, 7 / 15], rtol=1e-6)
+
+
+def test_bilinear_is_exact_interpolation_with_repeat(oracle):
+    """Closed form: between texel centres the bilinear filter is the tensor-product interpolation of
+    the four neighbours, wrapping across the image border in both directions."""
+    rng = np.random.default_rng(5)
+    img = rng.random((4, 6, 3)).astype(np.float32)
+    uv = np.array([[0.01, 0.02], [0.99, 0.97], [-0.3, 2.6], [0.5, 0.5]], np.float32)
+    got = oracle.texture_sample(img, uv, filter="bilinear")
+    for k, (u, v) in enumerate(uv.astype(np.float64)):
+        cx = (u - np.floor(u)) * 6 - 0.5
+        cy = (1 - (v - np.floor(v))) * 4 - 0.5
+        ix, iy = int(np.floor(cx)), int(np.floor(cy))
+        tx, ty = cx - ix, cy - iy
+        t = lambda x, y: img[y % 4, x % 6].astype(np.float64)  # noqa: E731
+        want = (t(ix, iy) * (1 - tx) + t(ix + 1, iy) * tx) * (1 - ty) + (t(ix, iy + 1) * (1 - tx) + t(ix + 1, iy + 1) * tx) * ty
+        np.testing.assert_allclose(got[k], want, atol=2e-6)
