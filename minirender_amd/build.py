@@ -20,7 +20,7 @@ ARCH = os.environ.get("MRG_OFFLOAD_ARCH", "gfx950")
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall", "-Wno-unused-function"]
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-fhip-fp32-correctly-rounded-divide-sqrt"]
-SOURCES = [("mrg_kernels.hip", True), ("mrg_api.cpp", False), ("mrg_bvh.cpp", False), ("mrg_energy.cpp", False)]
+SOURCES = [("mrg_kernels.hip", True), ("mrg_api.cpp", False), ("mrg_bvh.cpp", False), ("mrg_energy.cpp", False), ("mrg_envmap.cpp", False)]
 
 
 def _compile(src, device):

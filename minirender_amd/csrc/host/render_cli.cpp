@@ -62,7 +62,8 @@ void put_vec(std::ofstream& out, const std::vector<T>& v) {
 
 // u64 n, u32 instances, u32 directional, f32 env[3], mrg_camera, then positions, normals,
 // instance, triangle, mrg_material[instances], mrg_directional_light[directional], uvs [n][3][2],
-// u32 textures, then per texture u32 width, u32 height, f32 rgb[height][width][3]
+// u32 textures, then per texture u32 width, u32 height, f32 rgb[height][width][3], then
+// u32 environment texture (1 + index, 0 = none), f32 strength, f32 rotation degrees
 void dump_flat(const mrg_host::FlatScene& f, const std::string& path) {
     std::ofstream out(path, std::ios::binary);
     out.exceptions(std::ios::badbit | std::ios::failbit);
@@ -84,6 +85,9 @@ void dump_flat(const mrg_host::FlatScene& f, const std::string& path) {
         put(out, t.height);
         put_vec(out, t.rgb);
     }
+    put(out, f.environment_texture);
+    put(out, f.environment_strength);
+    put(out, f.environment_rotation_degrees);
 }
 
 int scene_info(int argc, char** argv) {

@@ -331,8 +331,11 @@ SceneDocument load_scene(const std::filesystem::path& path) {
         doc.environment                 = vector3(json.at("environment"));
         if (const Json* media = json.find("media"); media && media->size() != 0)
             throw std::invalid_argument("Participating media are not supported by the GPU integrator");
-        if (const Json* env = json.find("environment_map"); env && !string_or(*env, "texture").empty())
-            throw std::invalid_argument("Environment maps are not supported by the GPU integrator");
+        if (const Json* env = json.find("environment_map")) {
+            doc.environment_texture          = string_or(*env, "texture");
+            doc.environment_strength         = number_or(*env, "strength", 1.0f);
+            doc.environment_rotation_degrees = number_or(*env, "rotation_degrees", 0.0f);
+        }
         for (const Json& v : json.at("materials").array()) {
             mrg_material m{};
             const std::string id = v.at("id").string();
@@ -430,6 +433,10 @@ void SceneDocument::validate() const {
             "Environment must be finite and nonnegative");
     std::set<std::string> texture_ids;
     for (const TextureAsset& t : textures) require(texture_ids.insert(t.id).second, "Duplicate Texture id: " + t.id);
+    require(environment_texture.empty() || texture_ids.count(environment_texture) != 0,
+            "Environment references missing texture: " + environment_texture);
+    require(std::isfinite(environment_strength) && environment_strength >= 0 && std::isfinite(environment_rotation_degrees),
+            "Environment strength and rotation must be finite; strength must be nonnegative");
     // Material::validate (scene.cpp:241-266), unique ids
     std::set<std::string> ids;
     for (size_t i = 0; i < materials.size(); ++i) {
@@ -471,6 +478,10 @@ FlatScene flatten(const SceneDocument& doc) {
     flat.environment = doc.environment;
     flat.lights      = doc.lights;
     for (const TextureAsset& t : doc.textures) flat.textures.push_back(t.image);
+    for (size_t k = 0; k < doc.textures.size(); ++k)
+        if (!doc.environment_texture.empty() && doc.textures[k].id == doc.environment_texture) flat.environment_texture = uint32_t(k + 1);
+    flat.environment_strength         = doc.environment_strength;
+    flat.environment_rotation_degrees = doc.environment_rotation_degrees;
     uint32_t instance = 0;
     for (const Object& o : doc.objects) {
         if (!o.visible) continue;
@@ -517,6 +528,9 @@ mrg_scene_desc FlatScene::desc() {
     d.uvs           = uvs.empty() ? nullptr : uvs.data();
     d.texture_count = uint32_t(texture_views.size());
     d.textures      = texture_views.empty() ? nullptr : texture_views.data();
+    d.environment_texture          = environment_texture;
+    d.environment_strength         = environment_strength;
+    d.environment_rotation_degrees = environment_rotation_degrees;
     d.triangle_count    = instance.size();
     d.positions         = positions.data();
     d.normals           = normals.data();

@@ -3,10 +3,10 @@
 // object transforms (src/scene/scene.cpp:119-139) and the snapshot + SceneBvh flattening
 // (scene.cpp:364-405, src/render/bvh.cpp:53-89) into the arrays of mrg_scene_desc.
 //
-// Base-color textures are decoded from PNG (image_io.hpp). Features of the document that the GPU
-// integrator does not render (environment maps, participating media, cloth, subsurface materials
-// and normal maps) are rejected with std::invalid_argument naming the feature, instead of being
-// dropped silently.
+// Base-color and environment textures are decoded from PNG (image_io.hpp). Features of the
+// document that the GPU integrator does not render (participating media, cloth, subsurface
+// materials and normal maps) are rejected with std::invalid_argument naming the feature, instead of
+// being dropped silently.
 #pragma once
 
 #include <array>
@@ -64,6 +64,9 @@ struct SceneDocument {
     std::vector<mrg_material>               materials;  // base_color_texture resolved against `textures`
     std::vector<std::string>                material_textures;  // base_color_texture id per material ("" = none)
     std::vector<TextureAsset>               textures;
+    std::string                             environment_texture;  // EnvironmentMap (scene.hpp:188-193)
+    float                                   environment_strength = 1.0f;
+    float                                   environment_rotation_degrees = 0.0f;
     std::map<std::string, Mesh>             meshes;
     std::vector<std::string>                mesh_order;
     std::vector<Object>                     objects;
@@ -83,6 +86,8 @@ struct FlatScene {
     std::vector<mrg_material>          materials;  // per instance
     std::vector<mrg_directional_light> lights;
     std::array<float, 3>               environment{0.0f, 0.0f, 0.0f};
+    uint32_t                           environment_texture = 0;  // 1 + texture index, 0 = none
+    float                              environment_strength = 1.0f, environment_rotation_degrees = 0.0f;
     mrg_camera                         camera{};
     mrg_scene_desc                     desc();
 };

@@ -26,6 +26,7 @@
 
 #include "mrg_bvh.h"
 #include "mrg_energy.h"
+#include "mrg_envmap.h"
 #include "mrg_kernels.h"
 #include "mrg_layout.h"
 
@@ -145,7 +146,8 @@ struct mrg_scene {
     GpuScene       gpu{};
     mrg_scene_info info{};
     DevBuf         nodes, tris, normals, slot_instance, slot_emitter, materials, choices, cdf, areas, directional;
-    DevBuf         uvs, textures, texels;
+    DevBuf         uvs, textures, texels, env_cdf, env_rows;
+    bool           extended = false;  // uses textures, an environment map or energy compensation
 };
 
 struct mrg_job {
@@ -201,7 +203,7 @@ struct mrg_device {
     mrg_kernel_times                   times{};
     std::vector<hipEvent_t>            events;
     hipEvent_t                         query_events[2] = {nullptr, nullptr};
-    std::map<uint32_t, hipGraphExec_t> graphs;  // by bounce count
+    std::map<uint32_t, hipGraphExec_t> graphs;  // by 2 x bounce count + extended shading
 };
 
 namespace {
@@ -337,7 +339,9 @@ mrg_scene* build_scene(mrg_device* d, const mrg_scene_desc& s, const std::string
         for (uint64_t k = 0; k < n * 6; ++k)
             if (!std::isfinite(s.uvs[k])) throw std::invalid_argument("Texture coordinates must be finite");
     if (s.environment_texture > s.texture_count) throw std::invalid_argument("Environment references missing texture");
-    if (s.environment_texture != 0) throw std::invalid_argument("environment maps are not supported yet");
+    if (s.environment_texture != 0 && !(std::isfinite(s.environment_strength) && s.environment_strength >= 0 &&
+                                        std::isfinite(s.environment_rotation_degrees)))
+        throw std::invalid_argument("Environment strength and rotation must be finite; strength must be nonnegative");
     auto sc    = std::make_unique<mrg_scene>();
     sc->device = d;
     sc->hash   = hash;
@@ -355,7 +359,10 @@ mrg_scene* build_scene(mrg_device* d, const mrg_scene_desc& s, const std::string
     std::vector<float>          weights;
     std::vector<int32_t>        area_choice(n, -1);
     const V3f env      = ld(s.environment);
-    const float env_integral = 4.0f * kPiF * luminance(env);
+    const mrg_texture* env_texture = s.environment_texture ? &s.textures[s.environment_texture - 1] : nullptr;
+    const EnvTables    env_tables  = build_env_tables(s.environment, env_texture, env_texture ? s.environment_strength : 1.0f,
+                                                      env_texture ? s.environment_rotation_degrees : 0.0f);
+    const float env_integral = env_tables.integral;
     float lo[3] = {std::numeric_limits<float>::infinity(), std::numeric_limits<float>::infinity(), std::numeric_limits<float>::infinity()};
     float hi[3] = {-lo[0], -lo[1], -lo[2]};
     for (uint64_t p = 0; p < n; ++p) {
@@ -485,6 +492,8 @@ mrg_scene* build_scene(mrg_device* d, const mrg_scene_desc& s, const std::string
     upload(sc->uvs, uvs);
     upload(sc->textures, texs);
     upload(sc->texels, texels);
+    upload(sc->env_cdf, env_tables.cdf);
+    upload(sc->env_rows, env_tables.rows);
     GpuScene& g     = sc->gpu;
     g.nodes         = sc->nodes.as<GpuNode4>();
     g.tris          = sc->tris.as<GpuTri>();
@@ -505,6 +514,18 @@ mrg_scene* build_scene(mrg_device* d, const mrg_scene_desc& s, const std::string
     g.env_choice    = env_choice;
     st(g.env, env);
     g.env_pdf = env_choice >= 0 ? (cdf[env_choice + 1] - cdf[env_choice]) * (1.0f / (4.0f * kPiF)) : 0.0f;
+    g.env_sel      = env_choice >= 0 ? cdf[env_choice + 1] - cdf[env_choice] : 0.0f;
+    g.env_map      = env_texture ? int32_t(s.environment_texture) - 1 : -1;
+    g.env_uniform  = env_tables.uniform ? 1u : 0u;
+    g.env_strength = env_tables.strength;
+    g.env_rotation = env_tables.rotation;
+    g.env_gw       = env_tables.gw;
+    g.env_gh       = env_tables.gh;
+    g.env_cdf      = sc->env_cdf.as<float>();
+    g.env_rows     = sc->env_rows.as<float>();
+    sc->extended   = env_texture != nullptr || s.texture_count != 0;
+    for (uint32_t i = 0; i < s.instance_count; ++i)
+        sc->extended = sc->extended || s.materials[i].compensate_energy != 0 || s.materials[i].base_color_texture != 0;
     mrg_scene_info& info = sc->info;
     info.triangles       = n;
     info.nodes           = bvh.nodes.size();
@@ -512,7 +533,7 @@ mrg_scene* build_scene(mrg_device* d, const mrg_scene_desc& s, const std::string
     info.maximum_depth   = bvh.max_depth;
     info.device_bytes    = sc->nodes.bytes + sc->tris.bytes + sc->normals.bytes + sc->slot_instance.bytes + sc->slot_emitter.bytes +
                         sc->materials.bytes + sc->choices.bytes + sc->cdf.bytes + sc->areas.bytes + sc->directional.bytes +
-                        sc->uvs.bytes + sc->textures.bytes + sc->texels.bytes;
+                        sc->uvs.bytes + sc->textures.bytes + sc->texels.bytes + sc->env_cdf.bytes + sc->env_rows.bytes;
     info.build_seconds = seconds_since(t0);
     std::snprintf(info.content_hash, sizeof info.content_hash, "%s", hash.c_str());
     return sc.release();
@@ -567,6 +588,7 @@ struct BatchPlan {
     std::vector<GpuJob>    gjobs;
     std::vector<GpuChunk>  chunks;
     uint32_t               slots = 0, pixels = 0, max_depth = 0;
+    bool                   extended = false;  // some job's scene needs the extended shading kernel
 };
 
 bool runnable(const mrg_job* j) {
@@ -656,6 +678,7 @@ BatchPlan plan_batch_locked(mrg_device* d) {
         if (!plan_chunk(j, slot, std::min(cap - b.slots, share), fill, b)) continue;
         b.jobs.push_back(j);
         b.gjobs.push_back(gpu_job(j));
+        b.extended = b.extended || j->scene->extended;
         b.max_depth = std::max(b.max_depth, j->desc.maximum_depth);
     }
     if (batched && !b.jobs.empty() && b.slots < cap) {
@@ -714,7 +737,7 @@ TraceLauncher trace_launcher(const mrg_device* d) {
     return d->opt.traversal == MRG_TRAVERSAL_REFILL ? launch_trace_refill : launch_trace;
 }
 
-void launch_iteration_kernels(mrg_device* d, uint32_t iters, bool timed, std::vector<std::pair<int, int>>& marks) {
+void launch_iteration_kernels(mrg_device* d, uint32_t iters, bool extended, bool timed, std::vector<std::pair<int, int>>& marks) {
     const GpuJob*   jobs   = d->jobs_buf.as<GpuJob>();
     const GpuChunk* chunks = d->chunks_buf.as<GpuChunk>();
     int             ev     = 0;
@@ -732,7 +755,7 @@ void launch_iteration_kernels(mrg_device* d, uint32_t iters, bool timed, std::ve
         for (uint32_t it = 0; it < iters; ++it) {
             check_hip(trace_launcher(d)(d->pool, jobs, it, d->shape, d->stream), "k_trace");
             rec(1);
-            check_hip(launch_shade(d->pool, jobs, it, d->shape, d->stream), "k_shade");
+            check_hip(launch_shade(d->pool, jobs, it, extended, d->shape, d->stream), "k_shade");
             rec(2);
         }
         // the last shading pass's shadow rays (its extension queue is empty: no path continues
@@ -743,7 +766,7 @@ void launch_iteration_kernels(mrg_device* d, uint32_t iters, bool timed, std::ve
         for (uint32_t it = 0; it < iters; ++it) {
             check_hip(launch_extend(d->pool, jobs, it, d->shape, d->stream), "k_extend");
             rec(1);
-            check_hip(launch_shade(d->pool, jobs, it, d->shape, d->stream), "k_shade");
+            check_hip(launch_shade(d->pool, jobs, it, extended, d->shape, d->stream), "k_shade");
             rec(2);
             check_hip(launch_shadow(d->pool, jobs, it, d->shape, d->stream), "k_shadow");
             rec(3);
@@ -763,20 +786,21 @@ void run_batch(mrg_device* d, BatchPlan& b) {
     std::vector<std::pair<int, int>> marks;
     const bool timed = d->opt.use_graph == 0;
     if (d->opt.use_graph) {
-        auto it = d->graphs.find(iters);
+        const uint32_t key = iters * 2u + (b.extended ? 1u : 0u);
+        auto           it  = d->graphs.find(key);
         if (it == d->graphs.end()) {
             hipGraph_t graph;
             check_hip(hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-            launch_iteration_kernels(d, iters, false, marks);
+            launch_iteration_kernels(d, iters, b.extended, false, marks);
             check_hip(hipStreamEndCapture(d->stream, &graph), "hipStreamEndCapture");
             hipGraphExec_t exec;
             check_hip(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
             (void)hipGraphDestroy(graph);
-            it = d->graphs.emplace(iters, exec).first;
+            it = d->graphs.emplace(key, exec).first;
         }
         check_hip(hipGraphLaunch(it->second, d->stream), "hipGraphLaunch");
     } else {
-        launch_iteration_kernels(d, iters, timed, marks);
+        launch_iteration_kernels(d, iters, b.extended, timed, marks);
     }
     std::vector<uint32_t> ctr(kCounterWords);
     uint64_t              st[kStatWords];

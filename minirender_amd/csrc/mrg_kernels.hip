@@ -507,6 +507,9 @@ __global__ void __launch_bounds__(256) k_trace_refill(Pool P, const GpuJob* __re
 // Shading of one bounce (PathTracer::trace loop body, path_tracer.cpp:164-363).
 // At most 128 VGPRs (4 waves per SIMD): the kernel is bound by the latency of its path-state
 // and scene loads, so occupancy pays more than the registers the compiler would otherwise use.
+// EXT = false compiles out base-color textures, environment maps and energy compensation: scenes
+// without them run a kernel with fewer live registers.
+template <bool EXT>
 __global__ void __launch_bounds__(kShadeThreads) __attribute__((amdgpu_waves_per_eu(4))) k_shade(Pool P, const GpuJob* __restrict__ jobs, uint32_t iter) {
     const uint32_t  n       = P.counters[kCtrExt + iter];
     const uint32_t* queue   = (iter & 1u) ? P.q_ext1 : P.q_ext0;
@@ -558,8 +561,8 @@ __global__ void __launch_bounds__(kShadeThreads) __attribute__((amdgpu_waves_per
                 const bool unit_weight = depth == 0 || prev_delta || mode == 0u;
                 if (slot == 0xffffffffu) {
                     float w = 1.0f;
-                    if (!unit_weight) w = mode == 2u ? power_heuristic(prev_pdf, sc.env_pdf, bad) : 0.0f;
-                    L = L + (beta * ld3(sc.env)) * w;
+                    if (!unit_weight) w = mode == 2u ? power_heuristic(prev_pdf, EXT ? env_light_pdf(sc, ray_d) : sc.env_pdf, bad) : 0.0f;
+                    L = L + (beta * (EXT ? env_evaluate(sc, ray_d) : ld3(sc.env))) * w;
                     break;
                 }
                 // surface_hit (bvh.cpp:302-370)
@@ -592,7 +595,7 @@ __global__ void __launch_bounds__(kShadeThreads) __attribute__((amdgpu_waves_per
                 // base color: the material's, times the clamped texel at the hit uv = sum of b_i uv_i
                 // (bvh.cpp:318-321, path_tracer.cpp:279-282, bsdf.cpp:122-125)
                 F3 base_color = ld3(mat.color);
-                if (mat.texture >= 0) {
+                if (EXT && mat.texture >= 0) {
                     float u = 0.0f, v = 0.0f;
                     if (sc.uvs) {
                         const float2* q  = reinterpret_cast<const float2*>(sc.uvs) + static_cast<uint64_t>(slot) * 3u;
@@ -630,7 +633,7 @@ __global__ void __launch_bounds__(kShadeThreads) __attribute__((amdgpu_waves_per
                 bp.eta_t = front_face ? mat.ior : 1.0f;
                 bp.roughness = mat.roughness;
                 bp.table     = P.energy;
-                bp.energy    = mat.kind == 1u && mat.compensate != 0u && !bp.is_delta();
+                bp.energy    = EXT && mat.kind == 1u && mat.compensate != 0u && !bp.is_delta();
                 const float light_u   = next_1d(rng);
                 const float su_x      = next_1d(rng);
                 const float su_y      = next_1d(rng);
@@ -641,7 +644,7 @@ __global__ void __launch_bounds__(kShadeThreads) __attribute__((amdgpu_waves_per
                 bsdf_prepare(bp, won);
                 if (!bp.is_delta()) {
                     LightSample ls;
-                    if (sample_light(sc, position, light_u, su_x, su_y, ls) && (mode != 0u || ls.discrete)) {
+                    if (sample_light<EXT>(sc, position, light_u, su_x, su_y, ls) && (mode != 0u || ls.discrete)) {
                         const F3   wi   = to_local(f, ls.dir);
                         const bool side = dot(wo_world, ng) * dot(ls.dir, ng) > 0.0f;
                         if (wi.z > 0.0f && side) {
@@ -833,8 +836,10 @@ hipError_t launch_extend(const Pool& P, const GpuJob* jobs, uint32_t iter, const
     return hipGetLastError();
 }
 
-hipError_t launch_shade(const Pool& P, const GpuJob* jobs, uint32_t iter, const LaunchShape& shape, hipStream_t stream) {
-    hipLaunchKernelGGL(k_shade, dim3(shape.shade_blocks * 256 / kShadeThreads), dim3(kShadeThreads), 0, stream, P, jobs, iter);
+hipError_t launch_shade(const Pool& P, const GpuJob* jobs, uint32_t iter, bool extended, const LaunchShape& shape, hipStream_t stream) {
+    const dim3 grid(shape.shade_blocks * 256 / kShadeThreads), block(kShadeThreads);
+    if (extended) hipLaunchKernelGGL(k_shade<true>, grid, block, 0, stream, P, jobs, iter);
+    else hipLaunchKernelGGL(k_shade<false>, grid, block, 0, stream, P, jobs, iter);
     return hipGetLastError();
 }
 

@@ -127,6 +127,15 @@ struct GpuScene {
     int32_t               env_choice;
     float                 env[3];
     float                 env_pdf;      // selection(env) * 1/(4 pi), 0 without environment
+    // environment map (EnvironmentLight, light.cpp:61-191); env_map < 0: constant environment only
+    int32_t               env_map;      // texture index
+    uint32_t              env_uniform;  // map without power: uniform sphere sampling
+    float                 env_strength;
+    float                 env_rotation; // turns
+    float                 env_sel;      // selection probability of the environment choice
+    uint32_t              env_gw, env_gh;
+    const float*          env_cdf;      // [gw * gh + 1]
+    const float*          env_rows;     // [gh][4]: solid angle, cos(theta0), cos(theta1), cos difference
 };
 
 // Camera constants computed on the host with the FP32 twin's operations (scene.cpp:171-199).

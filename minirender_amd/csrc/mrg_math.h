@@ -33,6 +33,7 @@ __device__ __forceinline__ F3 normalize(F3 v) { return v * (1.0f / __builtin_sqr
 __device__ __forceinline__ F3 vabs(F3 v) { return F3{__builtin_fabsf(v.x), __builtin_fabsf(v.y), __builtin_fabsf(v.z)}; }
 __device__ __forceinline__ float smax(float a, float b) { return a < b ? b : a; }
 __device__ __forceinline__ float smin(float a, float b) { return b < a ? b : a; }
+__device__ __forceinline__ uint32_t smin_u(uint32_t a, uint32_t b) { return b < a ? b : a; }
 __device__ __forceinline__ float max3(float a, float b, float c) {
     float m = a;
     if (m < b) m = b;

@@ -155,6 +155,123 @@ __device__ __forceinline__ F3 texture_sample(const GpuScene& sc, int32_t index, 
     return top * sy + bottom * ty;
 }
 
+// --- environment map (EnvironmentLight, light.cpp:61-191) in the FP32 twin's form: directions to
+// latitude-longitude coordinates with atan2_turns, bilinear lookup of max(texel, 0) on level 0
+// (longitude wraps, latitude clamps), importance sampling of the grid texels with the host-built
+// float CDF, 5% uniform sphere sampling.
+
+// atan2(y, x) / (2 pi): octant reduction to t in [0, 1], t > tan(pi/8) mapped through
+// atan(t) = pi/4 + atan((t - 1) / (t + 1)), then the Maclaurin series of arctan (coefficients
+// (-1)^k / (2k + 1), k <= 8) in Horner form with fmaf. IEEE sign conventions for zeros.
+__device__ __forceinline__ float atan2_turns(float y, float x) {
+    const float ax = __builtin_fabsf(x), ay = __builtin_fabsf(y);
+    if (ax == 0.0f && ay == 0.0f) return __builtin_signbitf(x) ? (__builtin_signbitf(y) ? -0.5f : 0.5f) : y * 0.0f;
+    const bool  swap = ay > ax;
+    const float t    = (swap ? ax : ay) / (swap ? ay : ax);
+    const bool  big  = t > 0x1.a8279ap-2f;
+    const float r    = big ? (t - 1.0f) / (t + 1.0f) : t;
+    const float r2   = r * r;
+    float p = __builtin_fmaf(r2, 0x1.e1e1e2p-5f, -0x1.111112p-4f);
+    p       = __builtin_fmaf(r2, p, 0x1.3b13b2p-4f);
+    p       = __builtin_fmaf(r2, p, -0x1.745d18p-4f);
+    p       = __builtin_fmaf(r2, p, 0x1.c71c72p-4f);
+    p       = __builtin_fmaf(r2, p, -0x1.24924ap-3f);
+    p       = __builtin_fmaf(r2, p, 0x1.99999ap-3f);
+    p       = __builtin_fmaf(r2, p, -0x1.555556p-2f);
+    p       = __builtin_fmaf(r2, p, 1.0f);
+    float a = (r * p) * 0x1.45f306p-3f;
+    if (big) a = 0.125f + a;
+    if (swap) a = 0.25f - a;
+    if (__builtin_signbitf(x)) a = 0.5f - a;
+    return __builtin_signbitf(y) ? -a : a;
+}
+
+__device__ __forceinline__ void env_coordinates(const GpuScene& sc, F3 d, float& u, float& v) {
+    d = normalize(d);
+    u = atan2_turns(d.z, d.x) + sc.env_rotation;
+    u = u - __builtin_floorf(u);
+    v = 2.0f * atan2_turns(__builtin_sqrtf(d.x * d.x + d.z * d.z), d.y);
+}
+
+__device__ __forceinline__ F3 env_texel(const GpuTexture& t, const float* texels, int col, int row) {
+    const int    w = int(t.width), h = int(t.height);
+    const int    r = row < 0 ? 0 : (row > h - 1 ? h - 1 : row);
+    const float4 c = gld(reinterpret_cast<const float4*>(texels) + t.offset + static_cast<uint64_t>(r) * t.width +
+                         uint32_t((col % w + w) % w));
+    return mk(smax(c.x, 0.0f), smax(c.y, 0.0f), smax(c.z, 0.0f));
+}
+
+// Radiance arriving from direction d (EnvironmentLight::evaluate).
+__device__ __forceinline__ F3 env_evaluate(const GpuScene& sc, F3 d) {
+    if (sc.env_map < 0) return ld3(sc.env);
+    float u, v;
+    env_coordinates(sc, d, u, v);
+    const GpuTexture t  = sc.textures[sc.env_map];
+    const float      x  = u * static_cast<float>(t.width) - 0.5f;
+    const float      y  = v * static_cast<float>(t.height) - 0.5f;
+    const float      fx = __builtin_floorf(x), fy = __builtin_floorf(y);
+    const int        ix = static_cast<int>(fx), iy = static_cast<int>(fy);
+    const float      tx = x - fx, ty = y - fy, sx = 1.0f - tx, sy = 1.0f - ty;
+    const F3 top    = env_texel(t, sc.texels, ix, iy) * sx + env_texel(t, sc.texels, ix + 1, iy) * tx;
+    const F3 bottom = env_texel(t, sc.texels, ix, iy + 1) * sx + env_texel(t, sc.texels, ix + 1, iy + 1) * tx;
+    return ld3(sc.env) + sc.env_strength * (top * sy + bottom * ty);
+}
+
+// Solid-angle density of sampling d (EnvironmentLight::pdf) for a map with power.
+__device__ __forceinline__ float env_map_pdf(const GpuScene& sc, F3 d) {
+    float u, v;
+    env_coordinates(sc, d, u, v);
+    const uint32_t x  = smin_u(static_cast<uint32_t>(u * static_cast<float>(sc.env_gw)), sc.env_gw - 1u);
+    const uint32_t y  = smin_u(static_cast<uint32_t>(v * static_cast<float>(sc.env_gh)), sc.env_gh - 1u);
+    const uint32_t i  = y * sc.env_gw + x;
+    const float    pr = gld(sc.env_cdf + i + 1) - gld(sc.env_cdf + i);
+    return 0.05f / (4.0f * kPiF) + ((1.0f - 0.05f) * pr) / gld(sc.env_rows + y * 4u);
+}
+
+// Selection probability times the solid-angle density of the environment for direction d
+// (LightSampler::environment_pdf).
+__device__ __forceinline__ float env_light_pdf(const GpuScene& sc, F3 d) {
+    if (sc.env_map < 0 || sc.env_uniform) return sc.env_pdf;
+    return sc.env_sel * env_map_pdf(sc, d);
+}
+
+// EnvironmentLight::sample: direction, radiance and density.
+__device__ __forceinline__ F3 env_sample(const GpuScene& sc, float ux, float uy, F3& radiance, float& pdf) {
+    F3 d;
+    if (sc.env_map < 0 || sc.env_uniform || ux < 0.05f) {
+        d = sample_sphere(sc.env_map < 0 || sc.env_uniform ? ux : ux / 0.05f, uy);
+    } else {
+        const float t  = (ux - 0.05f) / (1.0f - 0.05f);
+        uint32_t    lo = 0, hi = sc.env_gw * sc.env_gh + 1u;  // first index with t < cdf[index]
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (t < gld(sc.env_cdf + mid)) hi = mid; else lo = mid + 1;
+        }
+        const uint32_t index = lo - 1u;
+        const float    c0    = gld(sc.env_cdf + index);
+        const float    remap = smin((t - c0) / (gld(sc.env_cdf + index + 1) - c0), 0x1.fffffep-1f);
+        const uint32_t x = index % sc.env_gw, y = index / sc.env_gw;
+        const float4   row = gld(reinterpret_cast<const float4*>(sc.env_rows) + y);
+        const float    v   = sclamp(uy, 0x1p-24f, 0x1.fffffep-1f);
+        float cosine, sine;
+        if (y < sc.env_gh / 2u) {
+            const float h = (1.0f - row.y) + v * row.w;
+            cosine        = 1.0f - h;
+            sine          = __builtin_sqrtf(smax(0.0f, h * (2.0f - h)));
+        } else {
+            const float h = (1.0f + row.z) + (1.0f - v) * row.w;
+            cosine        = -1.0f + h;
+            sine          = __builtin_sqrtf(smax(0.0f, h * (2.0f - h)));
+        }
+        float sp, cp;
+        sincos_turns((static_cast<float>(x) + remap) / static_cast<float>(sc.env_gw) - sc.env_rotation, sp, cp);
+        d = mk(sine * cp, cosine, sine * sp);
+    }
+    radiance = env_evaluate(sc, d);
+    pdf      = sc.env_map < 0 || sc.env_uniform ? 1.0f / (4.0f * kPiF) : env_map_pdf(sc, d);
+    return d;
+}
+
 // --- Kulla-Conty energy compensation (microfacet_energy.cpp:69-89, bsdf.cpp:128-189)
 // Bilinear lookup of the energy table ([64][64] float4 nodes (A, B, average, 0); x = cos(theta),
 // y = roughness): returns E = clamp(A + B, 0, 1) and the row average clamped to [0, 1].
@@ -313,6 +430,7 @@ __device__ __forceinline__ uint32_t select_light(const GpuScene& sc, float u, fl
     return index;
 }
 
+template <bool EXT>
 __device__ __forceinline__ bool sample_light(const GpuScene& sc, F3 ref, float light_u, float sux, float suy, LightSample& out) {
     if (sc.choice_count == 0) return false;
     float          mass;
@@ -328,10 +446,15 @@ __device__ __forceinline__ bool sample_light(const GpuScene& sc, F3 ref, float l
         return true;
     }
     if (ch.kind == 2u) {
-        out.dir         = sample_sphere(sux, suy);
-        out.radiance    = ld3(sc.env);
+        float pdf = 1.0f / (4.0f * kPiF);
+        if (EXT) {
+            out.dir = env_sample(sc, sux, suy, out.radiance, pdf);
+        } else {
+            out.dir      = sample_sphere(sux, suy);
+            out.radiance = ld3(sc.env);
+        }
         out.distance    = __builtin_inff();
-        out.probability = mass * (1.0f / (4.0f * kPiF));
+        out.probability = mass * pdf;
         out.discrete    = false;
         return true;
     }

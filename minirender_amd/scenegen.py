@@ -262,6 +262,39 @@ def textured_scene(sphere_level=3, seed=6) -> Scene:
     return sc
 
 
+def sky_map(width=64, height=32, seed=7, sun=(0.3, 0.28), sun_radiance=60.0):
+    """A procedural HDR latitude-longitude sky: a vertical gradient, seeded clouds and a bright
+    sun disk at (u, v) = `sun`; linear RGB float32 [height][width][3]."""
+    r = np.random.default_rng(seed)
+    v = (np.arange(height) + 0.5) / height
+    u = (np.arange(width) + 0.5) / width
+    sky = np.zeros((height, width, 3), np.float32)
+    sky[:] = np.stack([0.25 + 0.5 * v, 0.35 + 0.4 * v, 0.6 + 0.2 * v], -1)[:, None, :]
+    sky *= (0.7 + 0.6 * r.random((height, width, 1))).astype(np.float32)
+    sky[v > 0.5] *= 0.3  # darker ground half
+    du = np.minimum(np.abs(u[None, :] - sun[0]), 1 - np.abs(u[None, :] - sun[0]))
+    disk = (du ** 2 + (v[:, None] - sun[1]) ** 2) < (1.5 / height) ** 2
+    sky[disk] = (sun_radiance, 0.9 * sun_radiance, 0.75 * sun_radiance)
+    return sky
+
+
+def envmap_scene(map_width=64, map_height=32, strength=1.5, rotation=30.0, sphere_level=3) -> Scene:
+    """A floor and three spheres (diffuse, rough conductor, mirror) lit by an importance-sampled
+    HDR environment map (with a sun disk), rotated, scaled and added to a small constant term."""
+    unit = subdivided_sphere(sphere_level)
+    parts, mats = [], []
+    floor = np.array([[(-5, 0, -5), (-5, 0, 5), (5, 0, 5)], [(-5, 0, -5), (5, 0, 5), (5, 0, -5)]], np.float64)
+    parts.append((floor, np.tile(np.array([0, 1, 0], np.float64), (2, 3, 1))))
+    mats.append(material("diffuse", (0.6, 0.6, 0.6)))
+    for i, (kind, rough) in enumerate((("diffuse", 0.3), ("conductor", 0.25), ("mirror", 0.0))):
+        parts.append((unit * 0.6 + (-1.4 + 1.4 * i, 0.6, 0.0), unit))
+        mats.append(material(kind, (0.85, 0.7, 0.5), roughness=rough))
+    cam = Camera(position=(0, 1.6, 4.8), target=(0, 0.5, 0), up=(0, 1, 0), vertical_fov_degrees=45)
+    return concat(parts, mats, environment=(0.02, 0.02, 0.03), camera=cam, name="envmap",
+                  textures=[sky_map(map_width, map_height)], environment_texture=0, environment_strength=strength,
+                  environment_rotation_degrees=rotation)
+
+
 def furnace_spheres(sphere_level=3, roughness=(0.25, 0.65, 1.0), environment=(1.0, 1.0, 1.0), sun=True) -> Scene:
     """Two rows of rough conductor spheres, single-scattering above and Kulla-Conty compensated
     below (the layout of the reference's scenes/white-furnace.json), on a diffuse floor under a

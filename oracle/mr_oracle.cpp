@@ -110,6 +110,33 @@ template <class R> inline void sincos_2pi(R u, R& s, R& c) {
     }
 }
 
+// atan2(y, x) / (2 pi) for the FP32 twin (IEEE sign conventions for zeros): octant reduction to
+// t = min/max in [0, 1], t > tan(pi/8) mapped through atan(t) = pi/4 + atan((t - 1) / (t + 1)), then
+// the Maclaurin series of arctan (coefficients (-1)^k / (2k + 1), k <= 8; truncation below 3e-9 for
+// |r| <= tan(pi/8)) in Horner form with fmaf. The HIP kernels use the same method and constants.
+inline float atan2_turns(float y, float x) {
+    const float ax = std::fabs(x), ay = std::fabs(y);
+    if (ax == 0.0f && ay == 0.0f) return std::signbit(x) ? (std::signbit(y) ? -0.5f : 0.5f) : y * 0.0f;
+    const bool  swap = ay > ax;
+    const float t    = (swap ? ax : ay) / (swap ? ay : ax);
+    const bool  big  = t > 0x1.a8279ap-2f;  // tan(pi/8) = 0.41421356
+    const float r    = big ? (t - 1.0f) / (t + 1.0f) : t;
+    const float r2   = r * r;
+    float p = std::fma(r2, 0x1.e1e1e2p-5f, -0x1.111112p-4f);   // 1/17, -1/15
+    p       = std::fma(r2, p, 0x1.3b13b2p-4f);                  // 1/13
+    p       = std::fma(r2, p, -0x1.745d18p-4f);                 // -1/11
+    p       = std::fma(r2, p, 0x1.c71c72p-4f);                  // 1/9
+    p       = std::fma(r2, p, -0x1.24924ap-3f);                 // -1/7
+    p       = std::fma(r2, p, 0x1.99999ap-3f);                  // 1/5
+    p       = std::fma(r2, p, -0x1.555556p-2f);                 // -1/3
+    p       = std::fma(r2, p, 1.0f);
+    float a = (r * p) * 0x1.45f306p-3f;                         // / (2 pi)
+    if (big) a = 0.125f + a;
+    if (swap) a = 0.25f - a;
+    if (std::signbit(x)) a = 0.5f - a;
+    return std::signbit(y) ? -a : a;
+}
+
 // ----------------------------------------------------------------------------------------------
 // Sampler (sampling.cpp:28-65).
 inline uint64_t mix64(uint64_t v) {
@@ -465,6 +492,29 @@ struct Counters {
 };
 
 // Light sampler data per precision (light.cpp:197-264).
+template <class R> V3<R> vpositive(V3<R> c) { return {std::max(c.x, R(0)), std::max(c.y, R(0)), std::max(c.z, R(0))}; }
+template <class R> R luminance(V3<R> c);
+
+// EnvironmentLight (light.cpp:61-191): the constant environment plus an optional latitude-longitude
+// map (strength, rotation about +Y) importance-sampled by texel luminance x solid angle on the first
+// mip level of at most 1024 x 512, mixed with 5% uniform sphere sampling. The FP64 path follows the
+// reference; the FP32 twin evaluates directions with atan2_turns / sincos_turns and float
+// arithmetic and takes its per-map tables (selection CDF, per-row solid angles, cos(theta) of the
+// row boundaries and their difference 2 sin((t0 + t1) / 2) sin((t1 - t0) / 2)) from the FP64
+// construction rounded to float.
+struct Texture;
+template <class R> struct EnvMap {
+    V3<R>              constant{0, 0, 0};
+    const Texture*     texture = nullptr;
+    R                  strength = 1, rotation = 0;  // FP64: radians; FP32 twin: turns
+    R                  integral = 0;
+    bool               uniform = true;
+    size_t             grid_level = 0;
+    uint32_t           gw = 0, gh = 0;
+    std::vector<R>     cdf;                          // texel selection CDF over the grid level
+    std::vector<R>     solid, cos0, cos1, diff;      // per grid row
+};
+
 template <class R> struct Lights {
     struct Area { V3<R> v[3]; V3<R> normal; V3<R> radiance; R area; bool double_sided; };
     struct Dir { V3<R> direction; V3<R> radiance; };
@@ -477,6 +527,7 @@ template <class R> struct Lights {
     std::vector<int64_t> area_choice_of_prim;  // replaces the (instance, triangle) map
     int64_t             env_choice = -1;
     Discrete<R>         selection;
+    EnvMap<R>           envmap;
 };
 
 template <class R> R luminance(V3<R> c) { return dot(c, V3<R>{R(0.2126), R(0.7152), R(0.0722)}); }
@@ -570,6 +621,154 @@ struct Texture {
     }
 };
 
+template <class R> V3<R> env_unit(V3<R> d) {
+    if (!finite(d) || std::abs(dot(d, d) - R(1)) > R(1e-4)) throw std::invalid_argument("Light direction must be finite and have unit length");
+    return normalize(d);
+}
+
+template <class R> V2<R> env_coordinates(const EnvMap<R>& E, V3<R> d) {
+    d = env_unit(d);
+    if constexpr (std::is_same_v<R, double>) {
+        double u = (std::atan2(d.z, d.x) + E.rotation) / (2 * kPi);
+        u -= std::floor(u);
+        return {u, std::atan2(std::hypot(d.x, d.z), d.y) / kPi};
+    } else {
+        float u = atan2_turns(d.z, d.x) + E.rotation;
+        u       = u - std::floor(u);
+        return {u, 2.0f * atan2_turns(std::sqrt(d.x * d.x + d.z * d.z), d.y)};
+    }
+}
+
+template <class R> V3<R> env_lookup(const EnvMap<R>& E, V2<R> uv) {
+    const TexLevel& im = E.texture->levels.front();
+    const R   x  = uv.x * static_cast<R>(im.w) - R(0.5);
+    const R   y  = uv.y * static_cast<R>(im.h) - R(0.5);
+    const int ix = static_cast<int>(std::floor(x)), iy = static_cast<int>(std::floor(y));
+    const R   tx = x - std::floor(x), ty = y - std::floor(y);
+    auto texel = [&](int col, int row) {
+        const int w = int(im.w), h = int(im.h);
+        return vpositive(widen<R>(im.at(uint32_t((col % w + w) % w), uint32_t(std::clamp(row, 0, h - 1)))));
+    };
+    auto mix = [](V3<R> a, V3<R> b, R t) { return a * (R(1) - t) + b * t; };
+    return mix(mix(texel(ix, iy), texel(ix + 1, iy), tx), mix(texel(ix, iy + 1), texel(ix + 1, iy + 1), tx), ty);
+}
+
+template <class R> V3<R> env_evaluate(const EnvMap<R>& E, V3<R> d) {
+    if (!E.texture) return E.constant;
+    return E.constant + E.strength * env_lookup(E, env_coordinates(E, d));
+}
+
+template <class R> R env_pdf(const EnvMap<R>& E, V3<R> d) {
+    if (E.uniform) return R(1) / (R(4) * pi_v<R>());
+    const V2<R>  uv = env_coordinates(E, d);
+    const size_t x  = std::min(static_cast<size_t>(uv.x * static_cast<R>(E.gw)), size_t(E.gw) - 1);
+    const size_t y  = std::min(static_cast<size_t>(uv.y * static_cast<R>(E.gh)), size_t(E.gh) - 1);
+    const R      probability = E.cdf[y * E.gw + x + 1] - E.cdf[y * E.gw + x];
+    return R(0.05) / (R(4) * pi_v<R>()) + ((R(1) - R(0.05)) * probability) / E.solid[y];
+}
+
+template <class R> struct EnvSample { V3<R> direction, radiance; R pdf; };
+
+template <class R> EnvSample<R> env_sample(const EnvMap<R>& E, V2<R> u) {
+    check_unit(u.x);
+    check_unit(u.y);
+    V3<R> d;
+    if (E.uniform || u.x < R(0.05)) {
+        d = sample_sphere(V2<R>{E.uniform ? u.x : u.x / R(0.05), u.y});
+    } else {
+        const R      t     = (u.x - R(0.05)) / (R(1) - R(0.05));
+        check_unit(t);
+        const auto   upper = std::upper_bound(E.cdf.begin(), E.cdf.end(), t);
+        const size_t index = static_cast<size_t>(upper - E.cdf.begin() - 1);
+        const R      mass  = E.cdf[index + 1] - E.cdf[index];
+        const R      remap = std::min((t - E.cdf[index]) / mass, std::nextafter(R(1), R(0)));
+        const size_t x = index % E.gw, y = index / E.gw;
+        const R v = std::clamp(u.y, std::numeric_limits<R>::epsilon() * R(0.5), std::nextafter(R(1), R(0)));
+        R cosine, sine;
+        if (y < E.gh / 2) {
+            const R h = (R(1) - E.cos0[y]) + v * E.diff[y];
+            cosine    = R(1) - h;
+            sine      = std::sqrt(std::max(R(0), h * (R(2) - h)));
+        } else {
+            const R h = (R(1) + E.cos1[y]) + (R(1) - v) * E.diff[y];
+            cosine    = R(-1) + h;
+            sine      = std::sqrt(std::max(R(0), h * (R(2) - h)));
+        }
+        R sp, cp;
+        if constexpr (std::is_same_v<R, double>) {
+            const double phi = 2 * kPi * (static_cast<double>(x) + remap) / static_cast<double>(E.gw) - E.rotation;
+            cp               = std::cos(phi);
+            sp               = std::sin(phi);
+        } else {
+            sincos_turns((static_cast<float>(x) + remap) / static_cast<float>(E.gw) - E.rotation, sp, cp);
+        }
+        d = {sine * cp, cosine, sine * sp};
+    }
+    return {d, env_evaluate(E, d), env_pdf(E, d)};
+}
+
+// EnvironmentLight construction (light.cpp:61-114) in FP64; the FP32 twin's tables are its values
+// rounded to float.
+inline void build_envmap(EnvMap<double>& E, EnvMap<float>& T, V3<float> constant, const Texture* texture, float strength,
+                         float rotation_degrees) {
+    const double turns = std::remainder(static_cast<double>(rotation_degrees), 360.0);
+    E.constant = widen<double>(constant);
+    E.texture  = texture;
+    E.strength = strength;
+    E.rotation = turns * kPi / 180;
+    E.integral = 4 * kPi * luminance(E.constant);
+    T.constant = constant;
+    T.texture  = texture;
+    T.strength = strength;
+    T.rotation = static_cast<float>(turns / 360.0);
+    T.integral = 4.0f * pi_v<float>() * luminance(constant);
+    if (!texture || strength == 0) return;
+    while (texture->levels[E.grid_level].w > 1024 || texture->levels[E.grid_level].h > 512) ++E.grid_level;
+    const TexLevel&     image = texture->levels[E.grid_level];
+    std::vector<double> weights;
+    weights.reserve(static_cast<size_t>(image.w) * image.h);
+    double map_integral = 0;
+    for (uint32_t y = 0; y < image.h; ++y) {
+        const double t0    = kPi * static_cast<double>(y) / static_cast<double>(image.h);
+        const double t1    = kPi * static_cast<double>(y + 1) / static_cast<double>(image.h);
+        const double angle = (2 * kPi / static_cast<double>(image.w)) * 2 * std::sin((t0 + t1) * 0.5) * std::sin((t1 - t0) * 0.5);
+        E.solid.push_back(angle);
+        E.cos0.push_back(std::cos(t0));
+        E.cos1.push_back(std::cos(t1));
+        E.diff.push_back(2 * std::sin((t0 + t1) * 0.5) * std::sin((t1 - t0) * 0.5));
+        for (uint32_t x = 0; x < image.w; ++x) {
+            const double weight = luminance(vpositive(widen<double>(image.at(x, y)))) * angle;
+            weights.push_back(weight);
+            map_integral += weight;
+        }
+    }
+    E.cdf     = Discrete<double>(weights).cdf;
+    E.uniform = map_integral == 0;
+    E.gw      = image.w;
+    E.gh      = image.h;
+    double          full_integral = 0;
+    const TexLevel& full          = texture->levels.front();
+    for (uint32_t y = 0; y < full.h; ++y) {
+        const double t0    = kPi * static_cast<double>(y) / static_cast<double>(full.h);
+        const double t1    = kPi * static_cast<double>(y + 1) / static_cast<double>(full.h);
+        const double angle = (2 * kPi / static_cast<double>(full.w)) * 2 * std::sin((t0 + t1) * 0.5) * std::sin((t1 - t0) * 0.5);
+        for (uint32_t x = 0; x < full.w; ++x) full_integral += luminance(vpositive(widen<double>(full.at(x, y)))) * angle;
+    }
+    E.integral += E.strength * full_integral;
+    T.integral = T.integral + T.strength * static_cast<float>(full_integral);
+    T.uniform  = E.uniform;
+    T.grid_level = E.grid_level;
+    T.gw = E.gw;
+    T.gh = E.gh;
+    for (double v : E.cdf) T.cdf.push_back(static_cast<float>(v));
+    for (size_t y = 0; y < E.solid.size(); ++y) {
+        T.solid.push_back(static_cast<float>(E.solid[y]));
+        T.cos0.push_back(static_cast<float>(E.cos0[y]));
+        T.cos1.push_back(static_cast<float>(E.cos1[y]));
+        T.diff.push_back(static_cast<float>(E.diff[y]));
+    }
+}
+
 struct Scene {
     std::vector<std::array<V3<float>, 3>> pos;
     std::vector<std::array<V3<float>, 3>> nrm;
@@ -578,6 +777,8 @@ struct Scene {
     std::vector<uint32_t>                 inst, tri;
     std::vector<Material>                 materials;
     V3<float>                             environment;
+    int32_t                               environment_texture = -1;
+    float                                 environment_strength = 1, environment_rotation = 0;
     std::vector<OrcDirectional>           directional;
     Bvh                                   bvh;
     Lights<double>                        lights64;
@@ -589,7 +790,7 @@ struct Scene {
 
     template <class R> void build_lights(Lights<R>& L) {
         L.env          = widen<R>(environment);
-        L.env_integral = R(4) * pi_v<R>() * luminance(L.env);
+        L.env_integral = L.envmap.integral;
         L.area_choice_of_prim.assign(pos.size(), -1);
         Bounds3        bounds;
         std::vector<R> weights;
@@ -657,6 +858,9 @@ struct Scene {
             bvh.nodes.reserve(pos.size() * 2);
             bvh.build(0, static_cast<uint32_t>(pos.size()), 0, prims);
         }
+        build_envmap(lights64.envmap, lights32.envmap, environment,
+                     environment_texture >= 0 ? &textures.at(size_t(environment_texture)) : nullptr, environment_strength,
+                     environment_rotation);
         build_lights(lights64);
         build_lights(lights32);
     }
@@ -1131,9 +1335,8 @@ template <class R> std::optional<LightSample<R>> sample_light(const Lights<R>& L
         return out;
     }
     if (ch.kind == 2) {
-        const V3<R> dir = sample_sphere(V2<R>{su.x, su.y});
-        const R     pdf = R(1) / (R(4) * pi_v<R>());
-        out             = {dir, L.env, inf, sel->mass * pdf, false, 2, sel->index, {}, {}, {}};
+        const EnvSample<R> e = env_sample(L.envmap, V2<R>{su.x, su.y});
+        out                  = {e.direction, e.radiance, inf, sel->mass * e.pdf, false, 2, sel->index, {}, {}, {}};
         return out;
     }
     const auto& a  = L.areas[ch.index];
@@ -1157,8 +1360,8 @@ template <class R> R area_pdf(const Lights<R>& L, V3<R> ref, const SurfaceHit<R>
     return L.selection.probability(static_cast<size_t>(c)) * area_density(ref, h.position, a.normal, a.area);
 }
 
-template <class R> R environment_pdf(const Lights<R>& L) {
-    return L.env_choice >= 0 ? L.selection.probability(static_cast<size_t>(L.env_choice)) * (R(1) / (R(4) * pi_v<R>())) : R(0);
+template <class R> R environment_pdf(const Lights<R>& L, V3<R> direction) {
+    return L.env_choice >= 0 ? L.selection.probability(static_cast<size_t>(L.env_choice)) * env_pdf(L.envmap, direction) : R(0);
 }
 
 // ----------------------------------------------------------------------------------------------
@@ -1215,8 +1418,8 @@ V3<R> trace(const Scene& s, const Settings& cfg, Ray<R> ray, Sampler& sampler, R
             return cfg.mode == 2 ? power_heuristic(prev_pdf, light_pdf) : R(0);
         };
         if (!ph) {
-            const R w = emission_weight(environment_pdf(L));
-            result    = result + (beta * L.env) * w;
+            const R w = emission_weight(environment_pdf(L, ray.d));
+            result    = result + (beta * env_evaluate(L.envmap, ray.d)) * w;
             break;
         }
         const SurfaceHit<R> hit = surface_hit(s, ray, *ph);
@@ -1379,6 +1582,49 @@ extern "C" int orc_texture_level(uint32_t w, uint32_t h, const float* rgb, uint3
     });
 }
 
+extern "C" void orc_atan2_turns(const float* y, const float* x, uint64_t n, float* out) {
+    for (uint64_t i = 0; i < n; ++i) out[i] = atan2_turns(y[i], x[i]);
+}
+
+template <class R> static void env_query(const Scene& s, const double* dir, uint64_t n, double* radiance, double* pdf) {
+    const Lights<R>& L = s.lights<R>();
+    for (uint64_t i = 0; i < n; ++i) {
+        const V3<R> d{R(dir[i * 3]), R(dir[i * 3 + 1]), R(dir[i * 3 + 2])};
+        const V3<R> v = env_evaluate(L.envmap, d);
+        radiance[i * 3] = v.x; radiance[i * 3 + 1] = v.y; radiance[i * 3 + 2] = v.z;
+        pdf[i] = env_pdf(L.envmap, d);
+    }
+}
+
+template <class R> static void env_draw(const Scene& s, const double* u, uint64_t n, double* out) {
+    const Lights<R>& L = s.lights<R>();
+    for (uint64_t i = 0; i < n; ++i) {
+        const EnvSample<R> e = env_sample(L.envmap, V2<R>{R(u[i * 2]), R(u[i * 2 + 1])});
+        double* o = out + i * 7;
+        o[0] = e.direction.x; o[1] = e.direction.y; o[2] = e.direction.z;
+        o[3] = e.radiance.x; o[4] = e.radiance.y; o[5] = e.radiance.z; o[6] = e.pdf;
+    }
+}
+
+extern "C" int orc_env_eval(void* sp, int precision, const double* dir, uint64_t n, double* radiance, double* pdf) {
+    return guarded([&] {
+        if (precision == 32) env_query<float>(*static_cast<Scene*>(sp), dir, n, radiance, pdf);
+        else env_query<double>(*static_cast<Scene*>(sp), dir, n, radiance, pdf);
+    });
+}
+
+extern "C" int orc_env_sample(void* sp, int precision, const double* u, uint64_t n, double* out) {
+    return guarded([&] {
+        if (precision == 32) env_draw<float>(*static_cast<Scene*>(sp), u, n, out);
+        else env_draw<double>(*static_cast<Scene*>(sp), u, n, out);
+    });
+}
+
+extern "C" double orc_env_integral(void* sp, int precision) {
+    const Scene* s = static_cast<Scene*>(sp);
+    return precision == 32 ? double(s->lights32.envmap.integral) : s->lights64.envmap.integral;
+}
+
 extern "C" void orc_energy_table(float* out) {
     const auto& t = energy_table();
     for (size_t i = 0; i < t.node.size(); ++i)
@@ -1425,7 +1671,12 @@ extern "C" void* orc_scene_new(const OrcSceneDesc* d) {
             for (uint64_t p = 0; p < d->triangle_count; ++p)
                 for (int k = 0; k < 3; ++k) sc->uv[p][k] = {d->uvs[p * 6 + k * 2], d->uvs[p * 6 + k * 2 + 1]};
         }
-        if (d->environment_texture != 0) throw std::invalid_argument("environment maps are not supported by this oracle build");
+        if (d->environment_texture > sc->textures.size()) throw std::invalid_argument("Environment references missing texture");
+        if (!(std::isfinite(d->environment_strength) && d->environment_strength >= 0 && std::isfinite(d->environment_rotation_degrees)))
+            throw std::invalid_argument("Environment strength and rotation must be finite; strength must be nonnegative");
+        sc->environment_texture  = static_cast<int32_t>(d->environment_texture) - 1;
+        sc->environment_strength = d->environment_texture ? d->environment_strength : 1.0f;
+        sc->environment_rotation = d->environment_texture ? d->environment_rotation_degrees : 0.0f;
         sc->environment = {d->environment[0], d->environment[1], d->environment[2]};
         sc->directional.assign(d->directional, d->directional + d->directional_count);
         sc->build();

@@ -150,6 +150,14 @@ int    orc_texture_level(uint32_t w, uint32_t h, const float* rgb, uint32_t leve
 void   orc_integrate_ibl_brdf(double mu, double roughness, uint32_t samples, uint64_t seed, double* out);
 double orc_average_energy_of_ramp(const float* e, uint32_t width);
 
+// Environment light of a scene (light.cpp:61-191): radiance and pdf of n directions ([n][3]),
+// n samples of u ([n][2]) -> out [n][7] (direction, radiance, pdf), and the luminance integral.
+// atan2_turns: the FP32 twin's atan2(y, x) / (2 pi).
+int    orc_env_eval(void* scene, int precision, const double* dir, uint64_t n, double* radiance, double* pdf);
+int    orc_env_sample(void* scene, int precision, const double* u, uint64_t n, double* out);
+double orc_env_integral(void* scene, int precision);
+void   orc_atan2_turns(const float* y, const float* x, uint64_t n, float* out);
+
 // Primitive pieces for known-answer tests.
 void   orc_pcg(uint64_t seed, uint64_t sequence, uint64_t n, uint32_t* out);
 void   orc_for_sample(uint64_t seed, uint64_t pixel, uint64_t sample, uint64_t n, uint32_t* out);

@@ -102,6 +102,11 @@ def lib():
         L.orc_light_count.argtypes = [P]
         L.orc_camera_ray.argtypes = [P, C.c_int, P, C.c_double, P, P]
         L.orc_energy_table.argtypes = [P]
+        L.orc_env_eval.argtypes = [P, C.c_int, P, C.c_uint64, P, P]
+        L.orc_env_sample.argtypes = [P, C.c_int, P, C.c_uint64, P]
+        L.orc_env_integral.restype = C.c_double
+        L.orc_env_integral.argtypes = [P, C.c_int]
+        L.orc_atan2_turns.argtypes = [P, P, C.c_uint64, P]
         L.orc_texture_sample.argtypes = [C.c_uint32, C.c_uint32, P, P, C.c_uint64, C.c_float, C.c_uint32, P]
         L.orc_texture_level.argtypes = [C.c_uint32, C.c_uint32, P, C.c_uint32, P, P, P]
         L.orc_integrate_ibl_brdf.argtypes = [C.c_double, C.c_double, C.c_uint32, C.c_uint64, P]
@@ -258,6 +263,23 @@ class OracleScene:
         return dict(direction=out[0:3], radiance=out[3:6], distance=out[6], probability=out[7],
                     discrete=bool(out[8]), kind=int(out[9]), emitter=int(out[10]))
 
+    def env_eval(self, directions, precision=64):
+        """EnvironmentLight::evaluate and ::pdf of unit directions [n][3]."""
+        d = np.ascontiguousarray(directions, np.float64).reshape(-1, 3)
+        rad, pdf = np.zeros((len(d), 3)), np.zeros(len(d))
+        _check(lib().orc_env_eval(self.handle, _prec(precision), _ptr(d), len(d), _ptr(rad), _ptr(pdf)))
+        return rad, pdf
+
+    def env_sample(self, u, precision=64):
+        """EnvironmentLight::sample of u [n][2] -> (directions [n][3], radiance [n][3], pdf [n])."""
+        uu = np.ascontiguousarray(u, np.float64).reshape(-1, 2)
+        out = np.zeros((len(uu), 7))
+        _check(lib().orc_env_sample(self.handle, _prec(precision), _ptr(uu), len(uu), _ptr(out)))
+        return out[:, :3], out[:, 3:6], out[:, 6]
+
+    def env_integral(self, precision=64):
+        return lib().orc_env_integral(self.handle, _prec(precision))
+
     def light_selection(self):
         return np.array([lib().orc_light_selection(self.handle, i) for i in range(lib().orc_light_count(self.handle))])
 
@@ -398,4 +420,13 @@ def texture_level(image, level):
     _check(lib().orc_texture_level(im.shape[1], im.shape[0], _ptr(im), level, C.byref(w), C.byref(h), None))
     out = np.zeros((h.value, w.value, 3), np.float32)
     _check(lib().orc_texture_level(im.shape[1], im.shape[0], _ptr(im), level, C.byref(w), C.byref(h), _ptr(out)))
+    return out
+
+
+def atan2_turns(y, x):
+    """The FP32 twin's atan2(y, x) / (2 pi) (float32 in, float32 out)."""
+    y = np.ascontiguousarray(y, np.float32).reshape(-1)
+    x = np.ascontiguousarray(x, np.float32).reshape(-1)
+    out = np.zeros(len(y), np.float32)
+    lib().orc_atan2_turns(_ptr(y), _ptr(x), len(y), _ptr(out))
     return out

@@ -151,13 +151,16 @@ def read_flat_scene(path, name="flattened"):
         p += 8
         textures.append(np.frombuffer(b, "<f4", w * h * 3, p).reshape(h, w, 3).copy())
         p += w * h * 12
+    env_tex, env_strength, env_rot = struct.unpack_from("<Iff", b, p)
+    p += 12
     assert p == len(b)
     camera = S.Camera(tuple(map(float, cam["position"])), tuple(map(float, cam["target"])), tuple(map(float, cam["up"])),
                       float(cam["vertical_fov_degrees"]), float(cam["near_plane"]), float(cam["far_plane"]),
                       float(cam["aperture_radius"]), float(cam["focus_distance"]))
     return S.Scene(name=name, positions=pos.copy(), normals=nrm.copy(), instance=inst.copy(), triangle=tri.copy(),
                    materials=mats.copy(), environment=tuple(map(float, env)), directional=dirs.copy(), camera=camera,
-                   uvs=uvs.copy(), textures=textures)
+                   uvs=uvs.copy(), textures=textures, environment_texture=int(env_tex) - 1,
+                   environment_strength=env_strength, environment_rotation_degrees=env_rot)
 
 
 def read_obj_dump(path):

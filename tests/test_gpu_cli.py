@@ -50,8 +50,9 @@ QUAD_UV = "v -1 0 -1\nv 1 0 -1\nv 1 0 1\nv -1 0 1\nvt 0 0\nvt 2 0\nvt 2 2\nvt 0 
 
 def test_cli_render_textured_scene_matches_oracle(oracle, tmp_path):
     """A JSON scene whose materials use an sRGB PNG base-color texture (nearest and bilinear) and a
-    linear one, decoded by the command line and rendered through the C++ session, against the FP32
-    twin of the dumped flattened scene (textures included)."""
+    linear one, lit by an environment map (the sRGB texture, strength 2, rotated 45 degrees), decoded
+    by the command line and rendered through the C++ session, against the FP32 twin of the dumped
+    flattened scene (textures included)."""
     from test_textures import write_png
     rng = np.random.default_rng(8)
     write_png(tmp_path / "albedo.png", rng.integers(0, 256, (6, 9, 3)), 2, 8)
@@ -70,7 +71,8 @@ def test_cli_render_textured_scene_matches_oracle(oracle, tmp_path):
                        camera={"position": [0, 1.2, 3.0], "target": [0, 0.5, 0], "up": [0, 1, 0], "vertical_fov": 45,
                                "near": 0.03, "far": 100},
                        extra={"textures": [{"id": "albedo", "source": "albedo.png", "encoding": "srgb"},
-                                           {"id": "mask", "source": "mask.png", "encoding": "linear"}]})
+                                           {"id": "mask", "source": "mask.png", "encoding": "linear"}],
+                              "environment_map": {"texture": "albedo", "strength": 2.0, "rotation_degrees": 45}})
     w, h, spp, depth, seed = 32, 24, 6, 5, 12
     print(f"[cli textured render] {w}x{h} spp={spp} depth={depth}", flush=True)
     cli("--scene-info", path, tmp_path / "flat.bin")
@@ -80,6 +82,7 @@ def test_cli_render_textured_scene_matches_oracle(oracle, tmp_path):
     img = F.exr_rgb(tmp_path / "img.exr")
     scene = F.read_flat_scene(tmp_path / "flat.bin")
     assert len(scene.textures) == 2 and scene.textures[0].shape == (6, 9, 3)
+    assert scene.environment_texture == 0 and scene.environment_strength == 2 and scene.environment_rotation_degrees == 45
     twin, _ = oracle.OracleScene(scene).render(scene.camera, w, h, spp, seed=seed, max_depth=depth, precision=32)
     diff = np.abs(img.astype(np.float64) - twin.astype(np.float64))
     assert (diff <= 1e-4 * np.maximum(np.abs(twin.astype(np.float64)), 1e-6)).all()

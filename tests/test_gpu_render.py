@@ -105,3 +105,21 @@ def test_textures_change_the_image(device):
     plain.materials["base_color_texture"] = 0
     b, _ = api.render(device, plain, rs, ins)
     assert np.abs(a - b).max() > 1e-2
+
+
+@pytest.mark.parametrize("mode,size", [("mis", (64, 32)), ("light", (64, 32)), ("bsdf", (64, 32)), ("mis", (2100, 600))])
+def test_environment_map_matches_fp32_twin(device, oracle, mode, size):
+    """Importance-sampled HDR environment map (rotation, strength, constant term; a 2100 x 600 map
+    samples its 1050 x 300 mip level) against the FP32 twin within 1e-4."""
+    scene = scenegen.envmap_scene(map_width=size[0], map_height=size[1])
+    img, twin, prog, st = _render_pair(device, oracle, scene, 40, 30, 6, 5, mode=mode, seed=17)
+    _check(img, twin)
+    assert prog.statistics["shadow_rays"] == st["shadow_rays"]
+
+
+def test_environment_map_without_power_samples_uniformly(device, oracle):
+    """A map whose texels are all <= 0 keeps the constant term and samples the uniform sphere."""
+    scene = scenegen.envmap_scene()
+    scene.textures[0] = -np.abs(scene.textures[0])
+    img, twin, _, _ = _render_pair(device, oracle, scene, 24, 16, 4, 4, seed=2)
+    _check(img, twin)

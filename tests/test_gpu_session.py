@@ -149,10 +149,14 @@ def test_hbm_budget_admission():
 
 
 def test_results_identical_alone_and_in_the_mix():
-    """Each job's film is bit-identical when rendered alone (plain) and co-scheduled (batched)."""
+    """Each job's film is bit-identical when rendered alone (plain) and co-scheduled (batched). The
+    mix includes scenes with textures, an environment map and energy-compensated conductors, so the
+    plain scenes' paths are shaded by the extended kernel in the batches and by the plain one alone."""
     scenes = [scenegen.cornell_box(), scenegen.mixed_terrain(cells=48, spheres=8, sphere_level=2),
-              scenegen.icosphere(level=5)]
-    specs = [(0, 96, 64, 24, 8), (1, 128, 72, 16, 16), (2, 64, 64, 40, 10), (1, 33, 17, 7, 12), (0, 200, 150, 5, 8)]
+              scenegen.icosphere(level=5), scenegen.furnace_spheres(sphere_level=2), scenegen.textured_scene(sphere_level=2),
+              scenegen.envmap_scene(sphere_level=2)]
+    specs = [(0, 96, 64, 24, 8), (1, 128, 72, 16, 16), (2, 64, 64, 40, 10), (1, 33, 17, 7, 12), (0, 200, 150, 5, 8),
+             (3, 48, 40, 12, 6), (4, 56, 40, 9, 7), (5, 50, 30, 11, 6)]
     films = {}
     for mode in ("plain", "batched"):
         print(f"[mix] {mode}", flush=True)

@@ -166,7 +166,13 @@ def test_flatten_mirrored_instance_and_order(tmp_path):
 @pytest.mark.parametrize("extra,message", [
     ({"version": 2}, "Unsupported scene version"),
     ({"media": [{"id": "fog"}]}, "media are not supported"),
-    ({"environment_map": {"texture": "sky"}}, "Environment maps are not supported"),
+    ({"environment_map": {"texture": "sky"}}, "Environment references missing texture: sky"),
+    ({"materials": [{"id": "white", "kind": "diffuse", "base_color": [0.5, 0.5, 0.5], "base_color_texture": "t"}]},
+     "Material references missing texture: t"),
+    ({"materials": [{"id": "white", "kind": "diffuse", "base_color": [0.5, 0.5, 0.5], "normal_texture": "t"}]},
+     "normal maps are not supported"),
+    ({"materials": [{"id": "white", "kind": "diffuse", "base_color": [0.5, 0.5, 0.5], "texture_filter": "cubic"}]},
+     "Unknown texture filter"),
 ])
 def test_scene_rejections(tmp_path, extra, message):
     path = write_scene(tmp_path, [{"id": "a", "mesh": "m", "obj": "v 0 0 0\nv 1 0 0\nv 0 1 0\nf 1 2 3\n"}], extra=extra)
