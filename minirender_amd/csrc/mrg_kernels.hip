@@ -177,7 +177,6 @@ __device__ __forceinline__ StackView make_stack(uint32_t* lds_ref, float* lds_t,
     s.spill_ref = P.spill_ref;
     s.spill_t   = P.spill_t;
     s.lane      = threadIdx.x;
-    s.block     = blockDim.x;
     s.glane     = blockIdx.x * blockDim.x + threadIdx.x;
     s.lanes     = gridDim.x * blockDim.x;
     return s;
@@ -272,8 +271,8 @@ __global__ void __launch_bounds__(256) k_query(GpuScene sc, const float* __restr
                                                mrg_hit_raw* __restrict__ hits, float* __restrict__ bary,
                                                uint8_t* __restrict__ occluded, uint32_t* __restrict__ work, Pool P,
                                                uint64_t* __restrict__ stats) {
-    __shared__ uint32_t s_ref[kStackLds * 256];
-    __shared__ float    s_t[kStackLds * 256];
+    __shared__ uint32_t s_ref[kStackLds * kTraversalBlock];
+    __shared__ float    s_t[kStackLds * kTraversalBlock];
     const StackView stk = make_stack(s_ref, s_t, P);
     uint64_t        nb = 0, nt = 0;
     const TraceScene ts = trace_scene(sc);
@@ -414,8 +413,8 @@ __device__ __forceinline__ void shadow_one(const Pool& P, const GpuJob* __restri
 
 // Closest hit for every path in the extension queue of this bounce (split traversal kernels).
 __global__ void __launch_bounds__(256) k_extend(Pool P, const GpuJob* __restrict__ jobs, uint32_t iter) {
-    __shared__ uint32_t s_ref[kStackLds * 256];
-    __shared__ float    s_t[kStackLds * 256];
+    __shared__ uint32_t s_ref[kStackLds * kTraversalBlock];
+    __shared__ float    s_t[kStackLds * kTraversalBlock];
     const StackView stk   = make_stack(s_ref, s_t, P);
     const uint32_t  n     = P.counters[kCtrExt + iter];
     const uint32_t* queue = (iter & 1u) ? P.q_ext1 : P.q_ext0;
@@ -433,8 +432,8 @@ __global__ void __launch_bounds__(256) k_extend(Pool P, const GpuJob* __restrict
 // Radiance accumulation keeps its order: a path's shadow contribution of bounce iter - 1 lands
 // before the shading of bounce iter reads it.
 __global__ void __launch_bounds__(256) k_trace(Pool P, const GpuJob* __restrict__ jobs, uint32_t iter) {
-    __shared__ uint32_t s_ref[kStackLds * 256];
-    __shared__ float    s_t[kStackLds * 256];
+    __shared__ uint32_t s_ref[kStackLds * kTraversalBlock];
+    __shared__ float    s_t[kStackLds * kTraversalBlock];
     const StackView stk    = make_stack(s_ref, s_t, P);
     const uint32_t  n_ext  = P.counters[kCtrExt + iter];
     const uint32_t  n_sh   = iter ? P.counters[kCtrShadow + iter - 1] : 0u;
@@ -458,8 +457,8 @@ __global__ void __launch_bounds__(256) k_trace(Pool P, const GpuJob* __restrict_
 // bounce iter - 1 first, then the extension rays of bounce iter), but lanes that finish their ray
 // take the next queue entry at once, so a wave can trace shadow and extension rays side by side.
 __global__ void __launch_bounds__(256) k_trace_refill(Pool P, const GpuJob* __restrict__ jobs, uint32_t iter) {
-    __shared__ uint32_t s_ref[kStackLds * 256];
-    __shared__ float    s_t[kStackLds * 256];
+    __shared__ uint32_t s_ref[kStackLds * kTraversalBlock];
+    __shared__ float    s_t[kStackLds * kTraversalBlock];
     const StackView stk   = make_stack(s_ref, s_t, P);
     const uint32_t  n_ext = P.counters[kCtrExt + iter];
     const uint32_t  n_sh  = iter ? P.counters[kCtrShadow + iter - 1] : 0u;
@@ -732,8 +731,8 @@ __global__ void __launch_bounds__(kShadeThreads) __attribute__((amdgpu_waves_per
 
 // Any-hit for every shadow ray of this bounce (split traversal kernels).
 __global__ void __launch_bounds__(256) k_shadow(Pool P, const GpuJob* __restrict__ jobs, uint32_t iter) {
-    __shared__ uint32_t s_ref[kStackLds * 256];
-    __shared__ float    s_t[kStackLds * 256];
+    __shared__ uint32_t s_ref[kStackLds * kTraversalBlock];
+    __shared__ float    s_t[kStackLds * kTraversalBlock];
     const StackView stk  = make_stack(s_ref, s_t, P);
     const uint32_t  n    = P.counters[kCtrShadow + iter];
     uint32_t*       work = P.counters + kCtrWorkShadow + iter;

@@ -123,13 +123,16 @@ __device__ __forceinline__ bool tri_test(const RayPre& r, F3 v0, F3 v1, F3 v2, f
 constexpr uint32_t kTriSingleSided = 1u;
 constexpr uint32_t kTriMirrored    = 2u;
 
+// Every traversal kernel runs 256-thread blocks: the LDS stack row pitch is a compile-time
+// constant, so an entry's address is a shift and an add (no per-push integer multiply).
+constexpr uint32_t kTraversalBlock = 256;
+
 struct StackView {
-    uint32_t* lds_ref;   // [kStackLds][block]
+    uint32_t* lds_ref;   // [kStackLds][kTraversalBlock]
     float*    lds_t;
     uint32_t* spill_ref; // [kStackTotal - kStackLds][lanes]
     float*    spill_t;
     uint32_t  lane;      // threadIdx.x
-    uint32_t  block;     // blockDim.x
     uint32_t  glane;     // global lane id (spill column)
     uint32_t  lanes;     // total lanes of the grid (spill row pitch)
 };
@@ -137,8 +140,8 @@ struct StackView {
 __device__ __forceinline__ void stack_push(const StackView& s, uint32_t& sp, uint32_t ref, float t) {
     if (sp >= kStackTotal) return;  // unreachable: the builder bounds the worst-case stack by kStackTotal
     if (sp < kStackLds) {
-        s.lds_ref[sp * s.block + s.lane] = ref;
-        s.lds_t[sp * s.block + s.lane]   = t;
+        s.lds_ref[sp * kTraversalBlock + s.lane] = ref;
+        s.lds_t[sp * kTraversalBlock + s.lane]   = t;
     } else {
         const uint32_t k = (sp - kStackLds) * s.lanes + s.glane;
         gst(s.spill_ref + k, ref);
@@ -150,8 +153,8 @@ __device__ __forceinline__ void stack_push(const StackView& s, uint32_t& sp, uin
 __device__ __forceinline__ void stack_pop(const StackView& s, uint32_t& sp, uint32_t& ref, float& t) {
     --sp;
     if (sp < kStackLds) {
-        ref = s.lds_ref[sp * s.block + s.lane];
-        t   = s.lds_t[sp * s.block + s.lane];
+        ref = s.lds_ref[sp * kTraversalBlock + s.lane];
+        t   = s.lds_t[sp * kTraversalBlock + s.lane];
     } else {
         const uint32_t k = (sp - kStackLds) * s.lanes + s.glane;
         ref              = gld(s.spill_ref + k);
