@@ -72,7 +72,7 @@ typedef struct mrg_job    mrg_job;
 /* TextureFilter (texture.hpp:13). */
 enum { MRG_FILTER_NEAREST = 0, MRG_FILTER_BILINEAR = 1, MRG_FILTER_TRILINEAR = 2 };
 
-/* Material (scene.hpp:108-129) of one instance; 56 bytes. */
+/* Material (scene.hpp:108-129) of one instance; 68 bytes. */
 typedef struct mrg_material {
     uint32_t kind;
     float    base_color[3];
@@ -84,6 +84,9 @@ typedef struct mrg_material {
     uint32_t compensate_energy;  /* Kulla-Conty lobe of rough conductors (scene.hpp:125-126, bsdf.cpp:128) */
     uint32_t base_color_texture; /* 1 + index into mrg_scene_desc.textures; 0 = none (RenderInstance::base_color_texture) */
     uint32_t texture_filter;     /* MRG_FILTER_* */
+    uint32_t normal_texture;     /* tangent-space normal map: 1 + texture index (a linear texture), 0 = none */
+    float    normal_scale;       /* scales the map's x and y (scene.hpp:131-133) */
+    uint32_t normal_flip_y;      /* negate the map's y (glTF convention) */
 } mrg_material;
 
 /* Decoded linear-RGB texture, level 0 of the reference's mip chain (Texture, texture.hpp:22-40):

@@ -33,7 +33,8 @@ class NotAdmitted(MrgError):
 class Material(C.Structure):
     _fields_ = [("kind", C.c_uint32), ("base_color", C.c_float * 3), ("emission", C.c_float * 3),
                 ("roughness", C.c_float), ("ior", C.c_float), ("double_sided", C.c_uint32), ("mirrored", C.c_uint32),
-                ("compensate_energy", C.c_uint32), ("base_color_texture", C.c_uint32), ("texture_filter", C.c_uint32)]
+                ("compensate_energy", C.c_uint32), ("base_color_texture", C.c_uint32), ("texture_filter", C.c_uint32),
+                ("normal_texture", C.c_uint32), ("normal_scale", C.c_float), ("normal_flip_y", C.c_uint32)]
 
 
 class Texture(C.Structure):

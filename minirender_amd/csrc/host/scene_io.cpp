@@ -349,9 +349,10 @@ SceneDocument load_scene(const std::filesystem::path& path) {
             m.double_sided = bool_or(v, "double_sided", true) ? 1u : 0u;
             if (v.contains("cloth") || v.contains("subsurface"))
                 throw std::invalid_argument("Material " + id + ": cloth / subsurface data is not supported by the GPU integrator");
-            if (!string_or(v, "normal_texture").empty())
-                throw std::invalid_argument("Material " + id + ": normal maps are not supported by the GPU integrator");
             doc.material_textures.push_back(string_or(v, "base_color_texture"));
+            doc.material_normal_maps.push_back(string_or(v, "normal_texture"));
+            m.normal_scale  = number_or(v, "normal_scale", 1.0f);
+            m.normal_flip_y = bool_or(v, "normal_flip_y", false) ? 1u : 0u;
             m.compensate_energy = bool_or(v, "compensate_energy", false) ? 1u : 0u;
             m.texture_filter    = texture_filter(string_or(v, "texture_filter"));
             doc.material_ids.push_back(id);
@@ -371,16 +372,24 @@ SceneDocument load_scene(const std::filesystem::path& path) {
                 const std::string encoding = v.contains("encoding") ? v.at("encoding").string() : std::string("srgb");
                 if (encoding != "srgb" && encoding != "linear") throw std::invalid_argument("Unknown texture encoding");
                 t.image = read_texture(dir / std::filesystem::u8path(v.at("source").string()), encoding == "srgb");
+                t.srgb  = encoding == "srgb";
                 doc.textures.push_back(std::move(t));
             }
         }
         for (size_t i = 0; i < doc.materials.size(); ++i) {
-            const std::string& tex = doc.material_textures[i];
-            if (tex.empty()) continue;
-            size_t k = 0;
-            while (k < doc.textures.size() && doc.textures[k].id != tex) ++k;
-            if (k == doc.textures.size()) throw std::invalid_argument("Material references missing texture: " + tex);
-            doc.materials[i].base_color_texture = uint32_t(k + 1);
+            auto find = [&](const std::string& tex) {
+                size_t k = 0;
+                while (k < doc.textures.size() && doc.textures[k].id != tex) ++k;
+                if (k == doc.textures.size()) throw std::invalid_argument("Material references missing texture: " + tex);
+                return k;
+            };
+            if (!doc.material_textures[i].empty()) doc.materials[i].base_color_texture = uint32_t(find(doc.material_textures[i]) + 1);
+            if (!doc.material_normal_maps[i].empty()) {
+                // scene.cpp:295-300
+                const size_t k = find(doc.material_normal_maps[i]);
+                if (doc.textures[k].srgb) throw std::invalid_argument("Normal map must reference a linear texture");
+                doc.materials[i].normal_texture = uint32_t(k + 1);
+            }
         }
         for (const Json& v : json.at("objects").array()) {
             Object o;

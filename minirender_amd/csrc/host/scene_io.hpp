@@ -3,10 +3,10 @@
 // object transforms (src/scene/scene.cpp:119-139) and the snapshot + SceneBvh flattening
 // (scene.cpp:364-405, src/render/bvh.cpp:53-89) into the arrays of mrg_scene_desc.
 //
-// Base-color and environment textures are decoded from PNG (image_io.hpp). Features of the
-// document that the GPU integrator does not render (participating media, cloth, subsurface
-// materials and normal maps) are rejected with std::invalid_argument naming the feature, instead of
-// being dropped silently.
+// Base-color, normal-map and environment textures are decoded from PNG (image_io.hpp). Features of
+// the document that the GPU integrator does not render (participating media, cloth and subsurface
+// materials) are rejected with std::invalid_argument naming the feature, instead of being dropped
+// silently.
 #pragma once
 
 #include <array>
@@ -55,6 +55,7 @@ struct Object {
 struct TextureAsset {
     std::string id;
     RgbImage    image;
+    bool        srgb = true;  // ColorEncoding of the source
 };
 
 struct SceneDocument {
@@ -63,6 +64,7 @@ struct SceneDocument {
     std::vector<std::string>                material_ids;
     std::vector<mrg_material>               materials;  // base_color_texture resolved against `textures`
     std::vector<std::string>                material_textures;  // base_color_texture id per material ("" = none)
+    std::vector<std::string>                material_normal_maps;  // normal_texture id per material ("" = none)
     std::vector<TextureAsset>               textures;
     std::string                             environment_texture;  // EnvironmentMap (scene.hpp:188-193)
     float                                   environment_strength = 1.0f;

@@ -323,7 +323,10 @@ mrg_scene* build_scene(mrg_device* d, const mrg_scene_desc& s, const std::string
         if (!(m.ior > 0 && std::isfinite(m.ior))) throw std::invalid_argument("IOR must be positive");
         if (m.compensate_energy > 1) throw std::invalid_argument("compensate_energy must be 0 or 1");
         if (m.texture_filter > MRG_FILTER_TRILINEAR) throw std::invalid_argument("Unknown texture filter");
-        if (m.base_color_texture > s.texture_count) throw std::invalid_argument("Material references missing texture");
+        if (m.base_color_texture > s.texture_count || m.normal_texture > s.texture_count)
+            throw std::invalid_argument("Material references missing texture");
+        if (m.normal_texture && !(std::isfinite(m.normal_scale) && m.normal_scale >= 0))
+            throw std::invalid_argument("Normal scale must be finite and nonnegative");
     }
     if (s.texture_count && !s.textures) throw std::invalid_argument("textures are missing");
     for (uint32_t i = 0; i < s.texture_count; ++i) {
@@ -476,7 +479,10 @@ mrg_scene* build_scene(mrg_device* d, const mrg_scene_desc& s, const std::string
         g.compensate   = m.compensate_energy;
         g.texture      = int32_t(m.base_color_texture) - 1;
         g.filter       = m.texture_filter;
-        g.pad          = 0;
+        g.normal_map   = int32_t(m.normal_texture) - 1;
+        g.normal_scale = m.normal_texture ? m.normal_scale : 0.0f;
+        g.normal_flip  = m.normal_flip_y ? 1u : 0u;
+        g.pad[0] = g.pad[1] = 0;
     }
     check_hip(hipSetDevice(d->ordinal), "hipSetDevice");
     upload(sc->nodes, bvh.nodes);
@@ -525,7 +531,8 @@ mrg_scene* build_scene(mrg_device* d, const mrg_scene_desc& s, const std::string
     g.env_rows     = sc->env_rows.as<float>();
     sc->extended   = env_texture != nullptr || s.texture_count != 0;
     for (uint32_t i = 0; i < s.instance_count; ++i)
-        sc->extended = sc->extended || s.materials[i].compensate_energy != 0 || s.materials[i].base_color_texture != 0;
+        sc->extended = sc->extended || s.materials[i].compensate_energy != 0 || s.materials[i].base_color_texture != 0 ||
+                       s.materials[i].normal_texture != 0;
     mrg_scene_info& info = sc->info;
     info.triangles       = n;
     info.nodes           = bvh.nodes.size();

@@ -593,17 +593,22 @@ __global__ void __launch_bounds__(kShadeThreads) __attribute__((amdgpu_waves_per
                 }
                 // base color: the material's, times the clamped texel at the hit uv = sum of b_i uv_i
                 // (bvh.cpp:318-321, path_tracer.cpp:279-282, bsdf.cpp:122-125)
+                // and the tangent-space normal map of surface_hit (bvh.cpp:334-359)
                 F3 base_color = ld3(mat.color);
-                if (EXT && mat.texture >= 0) {
-                    float u = 0.0f, v = 0.0f;
+                if (EXT && (mat.texture >= 0 || mat.normal_map >= 0)) {
+                    float  u = 0.0f, v = 0.0f;
+                    float2 q[3] = {make_float2(0.0f, 0.0f), make_float2(0.0f, 0.0f), make_float2(0.0f, 0.0f)};
                     if (sc.uvs) {
-                        const float2* q  = reinterpret_cast<const float2*>(sc.uvs) + static_cast<uint64_t>(slot) * 3u;
-                        const float2  q0 = gld(q + 0), q1 = gld(q + 1), q2 = gld(q + 2);
-                        u = ((u + b0 * q0.x) + b1 * q1.x) + b2 * q2.x;
-                        v = ((v + b0 * q0.y) + b1 * q1.y) + b2 * q2.y;
+                        const float2* qp = reinterpret_cast<const float2*>(sc.uvs) + static_cast<uint64_t>(slot) * 3u;
+                        q[0] = gld(qp + 0), q[1] = gld(qp + 1), q[2] = gld(qp + 2);
+                        u = ((u + b0 * q[0].x) + b1 * q[1].x) + b2 * q[2].x;
+                        v = ((v + b0 * q[0].y) + b1 * q[1].y) + b2 * q[2].y;
                     }
-                    const F3 tc = texture_sample(sc, mat.texture, mat.filter, u, v);
-                    base_color  = base_color * mk(sclamp(tc.x, 0.0f, 1.0f), sclamp(tc.y, 0.0f, 1.0f), sclamp(tc.z, 0.0f, 1.0f));
+                    if (mat.normal_map >= 0 && mat.normal_scale > 0.0f) apply_normal_map(sc, mat, v0, v1, v2, q, u, v, geometric, shading);
+                    if (mat.texture >= 0) {
+                        const F3 tc = texture_sample(sc, mat.texture, mat.filter, u, v);
+                        base_color  = base_color * mk(sclamp(tc.x, 0.0f, 1.0f), sclamp(tc.y, 0.0f, 1.0f), sclamp(tc.z, 0.0f, 1.0f));
+                    }
                 }
                 const bool front_face = dot(ray_d, geometric) < 0.0f;
                 const F3   perr = absolute * (7.0f * kEpsF);

@@ -77,7 +77,10 @@ struct GpuMaterial {
     uint32_t compensate;  // Kulla-Conty lobe (rough conductors only; see BsdfParams)
     int32_t  texture;     // base-color texture index into GpuScene::textures, -1 = none
     uint32_t filter;      // 0 nearest, 1 bilinear, 2 trilinear (lod 0: the bilinear value)
-    uint32_t pad;
+    int32_t  normal_map;  // tangent-space normal map texture index, -1 = none
+    float    normal_scale;
+    uint32_t normal_flip; // negate the map's y
+    uint32_t pad[2];
 };
 
 // Level 0 of a base-color texture: texels[offset + y * width + x] (float4, rgb + 0), row 0 at the

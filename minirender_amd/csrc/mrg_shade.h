@@ -272,6 +272,37 @@ __device__ __forceinline__ F3 env_sample(const GpuScene& sc, float ux, float uy,
     return d;
 }
 
+// --- tangent-space normal map (bvh.cpp:334-359) on the uv tangent frame of the triangle
+// (uv_tangents, src/geometry/uv_tangents.cpp); `shading` is replaced when the mapped normal stays
+// above the geometric surface.
+__device__ __forceinline__ void apply_normal_map(const GpuScene& sc, const GpuMaterial& mat, F3 v0, F3 v1, F3 v2, const float2* uvc,
+                                                 float u, float v, F3 geometric, F3& shading) {
+    const F3    a  = v1 - v0, b = v2 - v0;
+    float       sx = uvc[1].x - uvc[0].x, sy = uvc[1].y - uvc[0].y;
+    float       tx = uvc[2].x - uvc[0].x, ty = uvc[2].y - uvc[0].y;
+    const float scale = smax(smax(smax(__builtin_fabsf(sx), __builtin_fabsf(sy)), __builtin_fabsf(tx)), __builtin_fabsf(ty));
+    if (!__builtin_isfinite(scale) || scale == 0.0f) return;
+    sx = sx / scale, sy = sy / scale, tx = tx / scale, ty = ty / scale;
+    const float det = sx * ty - sy * tx;
+    if (!__builtin_isfinite(det) || __builtin_fabsf(det) < 1e-12f) return;
+    const F3    tu = (ty * a - sy * b) / det;
+    const F3    tv = ((-tx) * a + sx * b) / det;
+    const float lu = length(tu), lv = length(tv);
+    if (!__builtin_isfinite(lu) || !__builtin_isfinite(lv) || lu == 0.0f || lv == 0.0f) return;
+    const F3 au = tu / lu, av = tv / lv;
+    const F3 t0 = au - shading * dot(au, shading);
+    if (!(dot(t0, t0) > 1e-12f)) return;
+    const F3 t = normalize(t0);
+    F3       bt = cross(shading, t);
+    if (dot(bt, av) < 0.0f) bt = -bt;
+    const F3 texel = texture_sample(sc, mat.normal_map, mat.filter, u, v);
+    F3       m     = texel * 2.0f - mk(1.0f, 1.0f, 1.0f);
+    m.x            = m.x * mat.normal_scale;
+    m.y            = m.y * (mat.normal_scale * (mat.normal_flip ? -1.0f : 1.0f));
+    const F3 mapped = (t * m.x + bt * m.y) + shading * m.z;
+    if (dot(mapped, geometric) > 1e-8f) shading = normalize(mapped);
+}
+
 // --- Kulla-Conty energy compensation (microfacet_energy.cpp:69-89, bsdf.cpp:128-189)
 // Bilinear lookup of the energy table ([64][64] float4 nodes (A, B, average, 0); x = cos(theta),
 // y = roughness): returns E = clamp(A + B, 0, 1) and the row average clamped to [0, 1].

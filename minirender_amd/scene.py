@@ -22,7 +22,7 @@ KIND_NAMES = {"diffuse": DIFFUSE, "conductor": CONDUCTOR, "mirror": MIRROR, "die
 # TextureFilter (texture.hpp:13).
 FILTERS = {"nearest": 0, "bilinear": 1, "trilinear": 2}
 
-# Layout shared with `mrg_material` (include/mrg.h): 56 bytes, 4-byte aligned.
+# Layout shared with `mrg_material` (include/mrg.h): 68 bytes, 4-byte aligned.
 MATERIAL_DTYPE = np.dtype(
     [
         ("kind", "<u4"),
@@ -35,6 +35,9 @@ MATERIAL_DTYPE = np.dtype(
         ("compensate_energy", "<u4"),
         ("base_color_texture", "<u4"),  # 1 + texture index, 0 = none
         ("texture_filter", "<u4"),
+        ("normal_texture", "<u4"),      # 1 + texture index, 0 = none
+        ("normal_scale", "<f4"),
+        ("normal_flip_y", "<u4"),
     ]
 )
 
@@ -58,9 +61,9 @@ CAMERA_DTYPE = np.dtype(
 
 def material(kind="diffuse", base_color=(0.7, 0.7, 0.7), emission=(0, 0, 0), roughness=0.35, ior=1.5,
              double_sided=True, mirrored=False, compensate_energy=False, base_color_texture=-1,
-             texture_filter="trilinear"):
+             texture_filter="trilinear", normal_texture=-1, normal_scale=1.0, normal_flip_y=False):
     """One material record with the reference defaults (Material, scene.hpp:108-129).
-    base_color_texture indexes Scene.textures (-1 = none)."""
+    base_color_texture and normal_texture index Scene.textures (-1 = none)."""
     m = np.zeros((), MATERIAL_DTYPE)
     m["kind"] = KIND_NAMES[kind] if isinstance(kind, str) else int(kind)
     m["base_color"] = base_color
@@ -72,6 +75,9 @@ def material(kind="diffuse", base_color=(0.7, 0.7, 0.7), emission=(0, 0, 0), rou
     m["compensate_energy"] = 1 if compensate_energy else 0
     m["base_color_texture"] = int(base_color_texture) + 1
     m["texture_filter"] = FILTERS[texture_filter] if isinstance(texture_filter, str) else int(texture_filter)
+    m["normal_texture"] = int(normal_texture) + 1
+    m["normal_scale"] = normal_scale
+    m["normal_flip_y"] = 1 if normal_flip_y else 0
     return m
 
 

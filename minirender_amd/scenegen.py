@@ -229,13 +229,17 @@ def mixed_terrain(cells=1024, spheres=64, sphere_level=3, seed=3, width=1920, he
 def textured_scene(sphere_level=3, seed=6) -> Scene:
     """Base-color textures on a floor (uv tiled 3x, so repeat addressing wraps), a back wall and three
     spheres (latitude-longitude uv) of diffuse, rough conductor and glass material, one per filter,
+    with a tangent-space normal map on the wall (scale 0.8) and two spheres (one with y flipped),
     under a sky and a sun."""
     r = np.random.default_rng(seed)
     checker = np.zeros((8, 8, 3), np.float32)
     checker[(np.arange(8)[:, None] + np.arange(8)[None, :]) % 2 == 0] = (0.9, 0.2, 0.1)
     checker[(np.arange(8)[:, None] + np.arange(8)[None, :]) % 2 == 1] = (0.1, 0.6, 0.9)
     noise = r.random((12, 20, 3)).astype(np.float32) * 1.2 - 0.1  # values outside [0, 1] exercise the clamp
-    textures = [checker, noise]
+    gy, gx = np.mgrid[0:16, 0:16] / 16.0
+    n = np.stack([0.6 * np.sin(2 * math.pi * 3 * gx), 0.6 * np.cos(2 * math.pi * 2 * gy), np.ones_like(gx)], -1)
+    bumps = (n / np.linalg.norm(n, axis=-1, keepdims=True) * 0.5 + 0.5).astype(np.float32)  # linear normal map
+    textures = [checker, noise, bumps]
     parts, mats, uvs = [], [], []
     floor = np.array([[(-4, 0, -4), (-4, 0, 4), (4, 0, 4)], [(-4, 0, -4), (4, 0, 4), (4, 0, -4)]], np.float64)
     parts.append((floor, np.tile(np.array([0, 1, 0], np.float64), (2, 3, 1))))
@@ -244,7 +248,8 @@ def textured_scene(sphere_level=3, seed=6) -> Scene:
     wall = np.array([[(-4, 0, -3), (4, 0, -3), (4, 4, -3)], [(-4, 0, -3), (4, 4, -3), (-4, 4, -3)]], np.float64)
     parts.append((wall, np.tile(np.array([0, 0, 1], np.float64), (2, 3, 1))))
     uvs.append((wall[..., [0, 1]] + (4, 0)) / 8)
-    mats.append(material("diffuse", (1.0, 1.0, 1.0), base_color_texture=1, texture_filter="bilinear"))
+    mats.append(material("diffuse", (1.0, 1.0, 1.0), base_color_texture=1, texture_filter="bilinear", normal_texture=2,
+                         normal_scale=0.8))
     unit = subdivided_sphere(sphere_level)
     lon = np.arctan2(unit[..., 2], unit[..., 0]) / (2 * math.pi) + 0.5
     lat = np.arccos(np.clip(unit[..., 1], -1, 1)) / math.pi
@@ -252,7 +257,8 @@ def textured_scene(sphere_level=3, seed=6) -> Scene:
                                             ("dielectric", 1, "nearest"))):
         parts.append((unit * 0.7 + (-1.6 + 1.6 * i, 0.7, 0.0), unit))
         uvs.append(np.stack([lon * 2, 1 - lat], -1))
-        mats.append(material(kind, (0.95, 0.9, 0.85), roughness=0.3, base_color_texture=tex, texture_filter=filt))
+        mats.append(material(kind, (0.95, 0.9, 0.85), roughness=0.3, base_color_texture=tex, texture_filter=filt,
+                             normal_texture=2 if i < 2 else -1, normal_flip_y=i == 1))
     cam = Camera(position=(0, 2.0, 5.5), target=(0, 0.7, 0), up=(0, 1, 0), vertical_fov_degrees=45)
     sun = directional_lights(((-0.4, -1.0, -0.6), (1.0, 0.95, 0.9), 2.5))
     sc = concat(parts, mats, environment=(0.3, 0.35, 0.45), directional=sun, camera=cam, name="textured")

@@ -398,6 +398,9 @@ struct Material {
     bool      compensate_energy = false;  // Kulla-Conty lobe for rough conductors (scene.hpp:125-126)
     int32_t   base_color_texture = -1;    // index into Scene::textures, -1 = none
     uint32_t  texture_filter = 2;         // 0 nearest, 1 bilinear, 2 trilinear (texture.hpp:13)
+    int32_t   normal_texture = -1;        // tangent-space normal map (a linear texture), -1 = none
+    float     normal_scale = 1;
+    bool      normal_flip_y = false;
 };
 
 struct Scene;
@@ -952,6 +955,26 @@ template <class R> struct SurfaceHit {
     V3<R>     position_error;
 };
 
+// uv_tangents (src/geometry/uv_tangents.cpp): unit directions of increasing u and v on a triangle
+// from its positions and corner uvs, or nothing for degenerate uvs or geometry.
+template <class R> std::optional<std::array<V3<R>, 2>> uv_tangents(const std::array<V3<float>, 3>& p, const std::array<V2<float>, 3>& c) {
+    const V3<R> a = widen<R>(p[1]) - widen<R>(p[0]);
+    const V3<R> b = widen<R>(p[2]) - widen<R>(p[0]);
+    V2<R>       su{R(c[1].x) - R(c[0].x), R(c[1].y) - R(c[0].y)};
+    V2<R>       tv{R(c[2].x) - R(c[0].x), R(c[2].y) - R(c[0].y)};
+    const R scale = std::max({std::abs(su.x), std::abs(su.y), std::abs(tv.x), std::abs(tv.y)});
+    if (!std::isfinite(scale) || scale == 0) return std::nullopt;
+    su = {su.x / scale, su.y / scale};
+    tv = {tv.x / scale, tv.y / scale};
+    const R det = su.x * tv.y - su.y * tv.x;
+    if (!std::isfinite(det) || std::abs(det) < R(1e-12)) return std::nullopt;
+    const V3<R> u  = (tv.y * a - su.y * b) / det;
+    const V3<R> v  = ((-tv.x) * a + su.x * b) / det;
+    const R     lu = length(u), lv = length(v);
+    if (!std::isfinite(lu) || !std::isfinite(lv) || lu == 0 || lv == 0) return std::nullopt;
+    return std::array<V3<R>, 2>{u / lu, v / lv};
+}
+
 template <class R> SurfaceHit<R> surface_hit(const Scene& s, const Ray<R>& ray, const PrimHit<R>& h) {
     const auto& P  = s.pos[h.prim];
     const auto& N  = s.nrm[h.prim];
@@ -977,6 +1000,27 @@ template <class R> SurfaceHit<R> surface_hit(const Scene& s, const Ray<R>& ray, 
     } else {
         shading = normalize(shading);
         if (dot(shading, geometric) < 0) shading = -shading;
+    }
+    // tangent-space normal map in the triangle's uv tangent frame (bvh.cpp:334-359)
+    const Material& mat = s.materials[s.inst[h.prim]];
+    if (mat.normal_texture >= 0 && mat.normal_scale > 0) {
+        std::array<V2<float>, 3> coords{};
+        if (!s.uv.empty()) coords = s.uv[h.prim];
+        if (const auto axes = uv_tangents<R>(P, coords)) {
+            const V3<R> t0 = (*axes)[0] - shading * dot((*axes)[0], shading);
+            if (dot(t0, t0) > R(1e-12)) {
+                const V3<R> t = normalize(t0);
+                V3<R>       b = cross(shading, t);
+                if (dot(b, (*axes)[1]) < 0) b = -b;
+                const V3<float> texel = s.textures.at(size_t(mat.normal_texture))
+                                            .sample(V2<float>{static_cast<float>(uv.x), static_cast<float>(uv.y)}, 0.0f, mat.texture_filter);
+                V3<R> smp = widen<R>(texel) * R(2) - V3<R>{1, 1, 1};
+                smp.x     = smp.x * R(mat.normal_scale);
+                smp.y     = smp.y * (R(mat.normal_scale) * (mat.normal_flip_y ? R(-1) : R(1)));
+                const V3<R> mapped = (t * smp.x + b * smp.y) + shading * smp.z;
+                if (dot(mapped, geometric) > R(1e-8)) shading = normalize(mapped);
+            }
+        }
     }
     SurfaceHit<R> out;
     out.t                = h.hit.t;
@@ -1556,7 +1600,8 @@ template <class F> static int guarded(F&& f) {
 static Material to_material(const OrcMaterial& m) {
     return Material{m.kind, {m.base_color[0], m.base_color[1], m.base_color[2]}, {m.emission[0], m.emission[1], m.emission[2]},
                     m.roughness, m.ior, m.double_sided != 0, m.mirrored != 0, m.compensate_energy != 0,
-                    static_cast<int32_t>(m.base_color_texture) - 1, m.texture_filter};
+                    static_cast<int32_t>(m.base_color_texture) - 1, m.texture_filter, static_cast<int32_t>(m.normal_texture) - 1,
+                    m.normal_scale, m.normal_flip_y != 0};
 }
 
 extern "C" int orc_texture_sample(uint32_t w, uint32_t h, const float* rgb, const float* uv, uint64_t n, float lod, uint32_t filter,
@@ -1580,6 +1625,27 @@ extern "C" int orc_texture_level(uint32_t w, uint32_t h, const float* rgb, uint3
         if (out)
             for (size_t i = 0; i < l.px.size(); ++i) { out[i * 3] = l.px[i].x; out[i * 3 + 1] = l.px[i].y; out[i * 3 + 2] = l.px[i].z; }
     });
+}
+
+template <class R> static int surface_query(const Scene& s, const double* ray, double* out) {
+    const Ray<R> r{{R(ray[0]), R(ray[1]), R(ray[2])}, {R(ray[3]), R(ray[4]), R(ray[5])}};
+    const auto   ph = s.trace<R>(r, R(0), std::numeric_limits<R>::infinity(), false, nullptr);
+    if (!ph) return 0;
+    const SurfaceHit<R> h = surface_hit(s, r, *ph);
+    const double v[11] = {double(h.position.x), double(h.position.y), double(h.position.z), h.geometric_normal.x,
+                          h.geometric_normal.y, h.geometric_normal.z, h.shading_normal.x, h.shading_normal.y,
+                          h.shading_normal.z, h.uv.x, h.uv.y};
+    std::copy(v, v + 11, out);
+    return 1;
+}
+
+extern "C" int orc_surface_hit(void* sp, int precision, const double* ray, double* out) {
+    int hit = 0;
+    const int rc = guarded([&] {
+        hit = precision == 32 ? surface_query<float>(*static_cast<Scene*>(sp), ray, out)
+                              : surface_query<double>(*static_cast<Scene*>(sp), ray, out);
+    });
+    return rc ? -1 : hit;
 }
 
 extern "C" void orc_atan2_turns(const float* y, const float* x, uint64_t n, float* out) {
@@ -1664,8 +1730,11 @@ extern "C" void* orc_scene_new(const OrcSceneDesc* d) {
             sc->materials.push_back(to_material(m));
         }
         for (uint32_t i = 0; i < d->texture_count; ++i) sc->textures.emplace_back(d->textures[i].width, d->textures[i].height, d->textures[i].rgb);
-        for (const auto& m : sc->materials)
-            if (m.base_color_texture >= int64_t(sc->textures.size())) throw std::invalid_argument("Material references missing texture");
+        for (const auto& m : sc->materials) {
+            if (m.base_color_texture >= int64_t(sc->textures.size()) || m.normal_texture >= int64_t(sc->textures.size()))
+                throw std::invalid_argument("Material references missing texture");
+            if (!(std::isfinite(m.normal_scale) && m.normal_scale >= 0)) throw std::invalid_argument("Normal scale must be finite and nonnegative");
+        }
         if (d->uvs) {
             sc->uv.resize(d->triangle_count);
             for (uint64_t p = 0; p < d->triangle_count; ++p)

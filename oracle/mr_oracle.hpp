@@ -40,6 +40,9 @@ struct OrcMaterial {
     uint32_t compensate_energy;   // Kulla-Conty multiple-scattering lobe (scene.hpp:125-126)
     uint32_t base_color_texture;  // 1 + texture index, 0 = none
     uint32_t texture_filter;      // 0 nearest, 1 bilinear, 2 trilinear
+    uint32_t normal_texture;      // tangent-space normal map: 1 + texture index, 0 = none
+    float    normal_scale;
+    uint32_t normal_flip_y;
 };
 
 struct OrcDirectional {
@@ -157,6 +160,9 @@ int    orc_env_eval(void* scene, int precision, const double* dir, uint64_t n, d
 int    orc_env_sample(void* scene, int precision, const double* u, uint64_t n, double* out);
 double orc_env_integral(void* scene, int precision);
 void   orc_atan2_turns(const float* y, const float* x, uint64_t n, float* out);
+// SceneBvh::intersect + surface_hit of one ray ([6]): out = position[3], geometric normal[3],
+// shading normal[3], uv[2]. Returns 1 on a hit, 0 on a miss, -1 on error.
+int    orc_surface_hit(void* scene, int precision, const double* ray, double* out);
 
 // Primitive pieces for known-answer tests.
 void   orc_pcg(uint64_t seed, uint64_t sequence, uint64_t n, uint32_t* out);

@@ -107,6 +107,7 @@ def lib():
         L.orc_env_integral.restype = C.c_double
         L.orc_env_integral.argtypes = [P, C.c_int]
         L.orc_atan2_turns.argtypes = [P, P, C.c_uint64, P]
+        L.orc_surface_hit.argtypes = [P, C.c_int, P, P]
         L.orc_texture_sample.argtypes = [C.c_uint32, C.c_uint32, P, P, C.c_uint64, C.c_float, C.c_uint32, P]
         L.orc_texture_level.argtypes = [C.c_uint32, C.c_uint32, P, C.c_uint32, P, P, P]
         L.orc_integrate_ibl_brdf.argtypes = [C.c_double, C.c_double, C.c_uint32, C.c_uint64, P]
@@ -262,6 +263,17 @@ class OracleScene:
             return None
         return dict(direction=out[0:3], radiance=out[3:6], distance=out[6], probability=out[7],
                     discrete=bool(out[8]), kind=int(out[9]), emitter=int(out[10]))
+
+    def surface_hit(self, origin, direction, precision=64):
+        """SceneBvh::intersect + surface_hit: dict(position, geometric_normal, shading_normal, uv) or None."""
+        ray = np.ascontiguousarray(np.concatenate([origin, direction]), np.float64)
+        out = np.zeros(11)
+        rc = lib().orc_surface_hit(self.handle, _prec(precision), _ptr(ray), _ptr(out))
+        if rc < 0:
+            raise ValueError(lib().orc_last_error().decode())
+        if rc == 0:
+            return None
+        return dict(position=out[0:3], geometric_normal=out[3:6], shading_normal=out[6:9], uv=out[9:11])
 
     def env_eval(self, directions, precision=64):
         """EnvironmentLight::evaluate and ::pdf of unit directions [n][3]."""

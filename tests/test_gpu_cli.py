@@ -61,7 +61,8 @@ def test_cli_render_textured_scene_matches_oracle(oracle, tmp_path):
                   "texture_filter": "nearest"},
                  {"id": "metal", "kind": "conductor", "base_color": [0.9, 0.8, 0.6], "roughness": 0.4,
                   "base_color_texture": "mask", "texture_filter": "bilinear"},
-                 {"id": "back", "kind": "diffuse", "base_color": [0.7, 0.7, 0.7], "base_color_texture": "albedo"}]
+                 {"id": "back", "kind": "diffuse", "base_color": [0.7, 0.7, 0.7], "base_color_texture": "albedo",
+                  "normal_texture": "mask", "normal_scale": 0.7, "normal_flip_y": True}]
     objects = [{"id": "floor", "mesh": "quad", "obj": QUAD_UV, "s": [2, 1, 2], "material": "floor"},
                {"id": "back", "mesh": "quad", "obj": QUAD_UV, "t": [0, 1, -1], "r": [90, 0, 0], "material": "back"},
                {"id": "box", "mesh": "cube", "obj": CUBE, "t": [0.2, 0.3, 0.1], "r": [0, 25, 0], "s": [0.3, 0.3, 0.3],

@@ -170,7 +170,7 @@ def test_flatten_mirrored_instance_and_order(tmp_path):
     ({"materials": [{"id": "white", "kind": "diffuse", "base_color": [0.5, 0.5, 0.5], "base_color_texture": "t"}]},
      "Material references missing texture: t"),
     ({"materials": [{"id": "white", "kind": "diffuse", "base_color": [0.5, 0.5, 0.5], "normal_texture": "t"}]},
-     "normal maps are not supported"),
+     "Material references missing texture: t"),
     ({"materials": [{"id": "white", "kind": "diffuse", "base_color": [0.5, 0.5, 0.5], "texture_filter": "cubic"}]},
      "Unknown texture filter"),
 ])
