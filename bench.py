@@ -161,7 +161,8 @@ def bench_render(args):
         b_ray = B_RAY_IN + B_HIT_OUT + B_BOX * ext_bounds + B_TRI * ext_tris
         b_shadow = B_RAY_IN + B_OCCLUDED_OUT + B_BOX * sh_bounds + B_TRI * sh_tris
         fused = kt["shadow_launches"] == 0  # one traversal kernel per bounce traces both kinds of ray
-        kernel = "k_trace" if fused else "k_extend"
+        refill = fused and kt["refill_launches"] == kt["extend_launches"]
+        kernel = ("k_trace_refill" if refill else "k_trace") if fused else "k_extend"
         launches = max(kt["extend_launches"], 1)
         rays = kt["extension_rays"] + (kt["shadow_rays"] if fused else 0)
         per_launch_bytes = (kt["extension_rays"] * b_ray + (kt["shadow_rays"] * b_shadow if fused else 0)) / launches

@@ -192,7 +192,8 @@ typedef struct mrg_device_options {
     uint32_t use_graph;          /* capture the bounce loop in a hipGraph */
     uint32_t block_size;         /* kernel block size (64/128/256); 0 = 256 */
     uint32_t persistent_blocks_per_cu; /* 0 = automatic */
-    uint32_t traversal;          /* MRG_TRAVERSAL_*; AUTO = fused */
+    uint32_t traversal;          /* MRG_TRAVERSAL_*; AUTO = refill for a batch with a scene of >= 2^20
+                                    triangles, fused otherwise */
 } mrg_device_options;
 
 /* Hit of the intersection entry points: closest t and scene-order primitive index (-1 = miss). */
@@ -266,7 +267,8 @@ int mrg_render(mrg_device* device, mrg_scene* scene, const mrg_job_desc* desc, f
 
 /* Device-wide kernel milliseconds by kernel class (accumulated since the last reset; measured only
  * when the bounce loop is not graph-captured). With fused traversal, extend_ms and extend_launches
- * cover the fused launches (shadow rays included) and shadow_ms / shadow_launches stay 0. */
+ * cover the fused launches (shadow rays included) and shadow_ms / shadow_launches stay 0;
+ * refill_launches counts those of the fused launches that ran the refill driver (k_trace_refill). */
 typedef struct mrg_kernel_times {
     double generate_ms;
     double extend_ms;
@@ -278,6 +280,7 @@ typedef struct mrg_kernel_times {
     uint64_t shadow_rays;
     uint64_t extend_launches;
     uint64_t shadow_launches;
+    uint64_t refill_launches;
 } mrg_kernel_times;
 int mrg_device_kernel_times(mrg_device* device, mrg_kernel_times* out, int reset);
 

@@ -96,7 +96,7 @@ class KernelTimes(C.Structure):
     _fields_ = [("generate_ms", C.c_double), ("extend_ms", C.c_double), ("shade_ms", C.c_double),
                 ("shadow_ms", C.c_double), ("accumulate_ms", C.c_double), ("batches", C.c_uint64),
                 ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("extend_launches", C.c_uint64),
-                ("shadow_launches", C.c_uint64)]
+                ("shadow_launches", C.c_uint64), ("refill_launches", C.c_uint64)]
 
 
 # Every entry point declared in include/mrg.h: name -> (restype, argtypes)

@@ -183,26 +183,37 @@ __device__ __forceinline__ StackView make_stack(uint32_t* lds_ref, float* lds_t,
 }
 
 // Persistent traversal with dynamic refill: every lane keeps its own ray; whenever at least
-// kRefillLanes lanes of the wave are idle, they take the next entries of the wave's claimed range
+// REFILL_LANES lanes of the wave are idle, they take the next entries of the wave's claimed range
 // (one atomic per kRefillClaim entries) and start new rays, so a wave does not wait for its slowest
 // ray. `start(i, st, any)` initialises item i and says whether it is an any-hit query (lanes of one
-// wave may trace both kinds); `finish(i, st)` consumes its result.
-constexpr int      kRefillLanes = 32;
+// wave may trace both kinds); `finish(i, st)` consumes its result at the lane's next refill (or
+// when the wave drains), so the lanes that finished since the last refill run it together.
+// Refill thresholds (measured): ray queries start a ray from one packed record, so they refill at 16
+// idle lanes; bounce traversal gathers each ray through the path queue, so it refills at 48.
+constexpr int      kQueryRefillLanes  = 16;
+constexpr int      kBounceRefillLanes = 48;
 constexpr uint32_t kRefillClaim = 128;  // entries per atomic claim (>= 64; 128 measured best, +61% over an atomic per refill)
 
-template <bool STATS, class Start, class Finish>
+template <int REFILL_LANES, bool STATS, class Start, class Finish>
 __device__ __forceinline__ void persistent_trace_mixed(const StackView& stk, uint32_t n, uint32_t* work, Start start, Finish finish,
                                                        uint64_t& nb, uint64_t& nt) {
     TraceState st;
     TraceScene ts{nullptr, nullptr, 0};
     bool       active  = false;
+    bool       pending = false;  // the lane's finished ray still awaits finish()
     bool       any     = false;
     bool       drained = false;  // wave-uniform: the queue has no entries left
     uint32_t   item    = 0;
     uint32_t   cur = 0, end = 0;  // wave-uniform: the unused part of the wave's claimed range
     for (;;) {
         const uint64_t idle = __ballot(!active);
-        if (!drained && __builtin_popcountll(idle) >= kRefillLanes) {
+        if (!drained && __builtin_popcountll(idle) >= REFILL_LANES) {
+            // finished rays are consumed here, together, rather than one lane at a time in the
+            // iteration each one ends
+            if (pending) {
+                finish(item, st);
+                pending = false;
+            }
             // idle lanes take the next entries of the wave's range; a new range of kRefillClaim
             // entries is claimed (one atomic) when the current one runs short
             const uint32_t need  = static_cast<uint32_t>(__builtin_popcountll(idle));
@@ -229,17 +240,18 @@ __device__ __forceinline__ void persistent_trace_mixed(const StackView& stk, uin
         }
         if (__ballot(active) == 0) break;
         if (active && trace_step_dyn<STATS>(ts, st, stk, nb, nt, any)) {
-            finish(item, st);
-            active = false;
+            active  = false;
+            pending = true;
         }
     }
+    if (pending) finish(item, st);
 }
 
 // Every item of one kind: closest hit (ANY = false) or any hit (ANY = true).
 template <bool ANY, bool STATS, class Start, class Finish>
 __device__ __forceinline__ void persistent_trace(const StackView& stk, uint32_t n, uint32_t* work, Start start, Finish finish,
                                                  uint64_t& nb, uint64_t& nt) {
-    persistent_trace_mixed<STATS>(
+    persistent_trace_mixed<kQueryRefillLanes, STATS>(
         stk, n, work,
         [&](uint32_t i, TraceState& st, bool& any) {
             any = ANY;
@@ -466,7 +478,7 @@ __global__ void __launch_bounds__(256) k_trace_refill(Pool P, const GpuJob* __re
     uint32_t*       work  = P.counters + kCtrWorkExt + iter;
     uint64_t        nb = 0, nt = 0;
     uint32_t        path = 0;
-    persistent_trace_mixed<false>(
+    persistent_trace_mixed<kBounceRefillLanes, false>(
         stk, n_sh + n_ext, work,
         [&](uint32_t i, TraceState& st, bool& any) {
             any = i < n_sh;
