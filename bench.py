@@ -196,6 +196,8 @@ def bench_serving():
            for m in ("plain", "plain_fill", "batched")}
     res["speedup"] = round(out["speedup"], 3)
     res["speedup_vs_plain_fill"] = round(out["speedup_vs_plain_fill"], 3)
+    res["last_arrival_s"] = round(out["last_arrival_s"], 4)
+    res["speedup_ceiling"] = round(out["speedup_ceiling"], 3)  # plain makespan / last arrival
     res["stream"] = "64 jobs, Poisson 40/s, side 128-1024, spp 16-1024, depth 8-16, 8 scenes (36-408k tris), 5% cancel"
     log(f"[bench] serving: plain {res['plain']['msamples_per_s']:.1f} Ms/s, batched {res['batched']['msamples_per_s']:.1f} Ms/s")
     return res

@@ -599,12 +599,12 @@ struct BatchPlan {
     bool                   refill   = false;  // bounce traversal with the refill driver (k_trace_refill)
 };
 
-// Traversal kernel of a batch under MRG_TRAVERSAL_AUTO: the refill driver when the batch holds a
-// scene of at least 2^20 triangles, the fused launch otherwise. Measured refill/fused throughput
-// (768 x 768 x 64 spp per scene, and the 1080p terrain job): 0.80-0.84 at 36-5k triangles,
-// 0.91-0.93 at 21k-141k, 0.98-0.99 at 328k-408k, 1.04 at 2.2M. A refill pays for every ray it
-// starts and gains where ray lengths vary most within a wave, which is in large scenes.
-constexpr uint32_t kRefillMinTriangles = 1u << 20;
+// Traversal kernel of a batch under MRG_TRAVERSAL_AUTO: the refill driver (with parked leaves) when
+// the batch holds a scene of at least 2^17 triangles, the fused launch otherwise. Measured
+// refill/fused throughput (768 x 768 x 64 spp per scene, and the 1080p terrain job): 0.85-0.88 at
+// 36-5k triangles, 0.99-1.00 at 21k-82k, 1.02-1.08 at 141k-408k, 1.10 at 2.2M. A refill pays for
+// every ray it starts and gains where ray lengths vary most within a wave, which is in large scenes.
+constexpr uint32_t kRefillMinTriangles = 1u << 17;
 
 bool refill_traversal(const mrg_device* d, const mrg_scene* s) {
     if (d->opt.traversal == MRG_TRAVERSAL_REFILL) return true;

@@ -188,13 +188,18 @@ __device__ __forceinline__ StackView make_stack(uint32_t* lds_ref, float* lds_t,
 // ray. `start(i, st, any)` initialises item i and says whether it is an any-hit query (lanes of one
 // wave may trace both kinds); `finish(i, st)` consumes its result at the lane's next refill (or
 // when the wave drains), so the lanes that finished since the last refill run it together.
-// Refill thresholds (measured): ray queries start a ray from one packed record, so they refill at 16
-// idle lanes; bounce traversal gathers each ray through the path queue, so it refills at 48.
+// Thresholds, measured on the 16.7M-ray icosphere microbench and the 1080p terrain job: ray queries
+// start a ray from one packed record and refill at 16 idle lanes; bounce traversal gathers each ray
+// through the path queue and refills at 40 (32: -2.5%, 48: -1.3%, 56: -16%). LEAF_BATCH is the number
+// of lanes with a parked leaf that starts a triangle step (trace_step_dyn): 24 for both (16: -3%
+// queries, -1% bounces; 32: -1.5% queries, bounces equal).
 constexpr int      kQueryRefillLanes  = 16;
-constexpr int      kBounceRefillLanes = 48;
+constexpr int      kBounceRefillLanes = 40;
+constexpr int      kQueryLeafBatch    = 24;
+constexpr int      kBounceLeafBatch   = 24;
 constexpr uint32_t kRefillClaim = 128;  // entries per atomic claim (>= 64; 128 measured best, +61% over an atomic per refill)
 
-template <int REFILL_LANES, bool STATS, class Start, class Finish>
+template <int REFILL_LANES, int LEAF_BATCH, bool STATS, class Start, class Finish>
 __device__ __forceinline__ void persistent_trace_mixed(const StackView& stk, uint32_t n, uint32_t* work, Start start, Finish finish,
                                                        uint64_t& nb, uint64_t& nt) {
     TraceState st;
@@ -239,7 +244,7 @@ __device__ __forceinline__ void persistent_trace_mixed(const StackView& stk, uin
             }
         }
         if (__ballot(active) == 0) break;
-        if (active && trace_step_dyn<STATS>(ts, st, stk, nb, nt, any)) {
+        if (active && trace_step_dyn<LEAF_BATCH, STATS>(ts, st, stk, nb, nt, any)) {
             active  = false;
             pending = true;
         }
@@ -251,7 +256,7 @@ __device__ __forceinline__ void persistent_trace_mixed(const StackView& stk, uin
 template <bool ANY, bool STATS, class Start, class Finish>
 __device__ __forceinline__ void persistent_trace(const StackView& stk, uint32_t n, uint32_t* work, Start start, Finish finish,
                                                  uint64_t& nb, uint64_t& nt) {
-    persistent_trace_mixed<kQueryRefillLanes, STATS>(
+    persistent_trace_mixed<kQueryRefillLanes, kQueryLeafBatch, STATS>(
         stk, n, work,
         [&](uint32_t i, TraceState& st, bool& any) {
             any = ANY;
@@ -478,7 +483,7 @@ __global__ void __launch_bounds__(256) k_trace_refill(Pool P, const GpuJob* __re
     uint32_t*       work  = P.counters + kCtrWorkExt + iter;
     uint64_t        nb = 0, nt = 0;
     uint32_t        path = 0;
-    persistent_trace_mixed<kBounceRefillLanes, false>(
+    persistent_trace_mixed<kBounceRefillLanes, kBounceLeafBatch, false>(
         stk, n_sh + n_ext, work,
         [&](uint32_t i, TraceState& st, bool& any) {
             any = i < n_sh;

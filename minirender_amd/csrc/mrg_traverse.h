@@ -195,13 +195,15 @@ struct TraceScene {
 
 __device__ __forceinline__ TraceScene trace_scene(const GpuScene& sc) { return TraceScene{sc.nodes, sc.tris, sc.node_count}; }
 
-// Per-lane traversal state: the ray, the best hit so far, the current node and the stack depth
-// (the stack itself lives in LDS / the spill area of this lane).
+// Per-lane traversal state of the step driver: the ray, the best hit so far, the next node to
+// visit, a parked leaf and the stack depth (the stack itself lives in LDS / the spill area of this
+// lane). `ref` is an inner node, kEmptyRef (nothing to visit), or a leaf waiting for the park slot.
 struct TraceState {
     RayPre      r;
     float       tmin;
     TraceResult res;
     uint32_t    ref;
+    uint32_t    leaf;  // parked leaf, kEmptyRef when none (kEmptyRef is never a leaf of a scene below 2^27 triangles)
     uint32_t    sp;
 };
 
@@ -213,20 +215,75 @@ __device__ __forceinline__ void trace_init(TraceState& st, F3 o, F3 d, float tmi
     st.res.prim = 0xffffffffu;
     st.res.b0 = st.res.b1 = st.res.b2 = 0.0f;
     st.res.det = 1.0f;
-    st.ref = 0;
-    st.sp  = 0;
+    st.ref  = 0;  // the root is always an inner node
+    st.leaf = kEmptyRef;
+    st.sp   = 0;
 }
 
-// One traversal step: an inner node (its four boxes) or a leaf (its triangles), then a pop if the
-// lane has no child to descend into. Returns true once the ray is finished: closest hit (any =
-// false) or any hit (any = true) in [tmin, tmax], result in st.res. `any` may differ between the
-// lanes of a wave (the mixed refill driver); trace_step<ANY> passes a constant.
-template <bool STATS>
+// Leaf batching of the step driver: a lane that reaches a leaf parks it (one parked leaf per lane)
+// and goes on with the inner nodes of its stack; the wave tests triangles when at least LEAF_BATCH
+// lanes hold a parked leaf, or when the lanes that can only wait for a triangle step (a parked leaf
+// and no inner node to visit) are at least as many as those with an inner node. Each step is then
+// either a box step or a triangle step for the whole wave, and triangle tests run with many lanes
+// at once instead of the few lanes that happen to reach a leaf in the same step. Neither result
+// depends on the order of the leaves: the closest hit is the nearest t with ties to the lower
+// scene-order index, and the any-hit result is only whether a hit exists; a box tested before a
+// parked leaf has lowered the closest distance is tested against a larger bound, which costs work,
+// never a hit.
+
+// One traversal step (all lanes of the wave that trace call it together): the parked leaves' triangles
+// or one inner node per lane, then the lane parks a leaf it reached and takes stack entries until it
+// has an inner node to visit. Returns true once the ray is finished: closest hit (any = false) or any
+// hit (any = true) in [tmin, tmax], result in st.res. `any` may differ between the lanes of a wave.
+template <int LEAF_BATCH, bool STATS>
 __device__ __forceinline__ bool trace_step_dyn(const TraceScene& sc, TraceState& st, const StackView& stk, uint64_t& bounds_tests,
                                                uint64_t& tri_tests, const bool any) {
     if (sc.node_count == 0) return true;
-    const RayPre& r = st.r;
-    if (!is_leaf(st.ref)) {
+    const RayPre& r          = st.r;
+    const bool    parked     = st.leaf != kEmptyRef;
+    const bool    node       = st.ref != kEmptyRef && !is_leaf(st.ref);
+    const int     n_parked   = __builtin_popcountll(__ballot(parked));
+    const int     n_node     = __builtin_popcountll(__ballot(node));
+    const int     n_stuck    = __builtin_popcountll(__ballot(parked && !node));
+    const bool    leaf_phase = n_parked >= LEAF_BATCH || n_stuck >= n_node;
+    if (leaf_phase) {
+        if (parked) {
+            const uint32_t first = leaf_first(st.leaf), count = leaf_count(st.leaf);
+            st.leaf = kEmptyRef;
+            for (uint32_t k = 0; k < count; ++k) {
+                const uint32_t slot = first + k;
+                const float4*  tp   = reinterpret_cast<const float4*>(sc.tris + slot);
+                const float4   a    = gld(tp + 0);
+                const float4   b    = gld(tp + 1);
+                const float4   c    = gld(tp + 2);
+                if (STATS) tri_tests += 1;
+                TriHit h;
+                const F3 v0 = mk(a.x, a.y, a.z), v1 = mk(b.x, b.y, b.z), v2 = mk(c.x, c.y, c.z);
+                if (!tri_test(r, v0, v1, v2, st.tmin, st.res.t, h)) continue;
+                const uint32_t flags = __float_as_uint(b.w);
+                if (flags & kTriSingleSided) {
+                    const F3 e1      = v1 - v0;
+                    const F3 e2      = v2 - v0;
+                    F3       outward = cross(e1, e2) * ((flags & kTriMirrored) ? -1.0f : 1.0f);
+                    if (dot(r.d, outward) >= 0.0f) continue;
+                }
+                const uint32_t prim = __float_as_uint(a.w);
+                if (st.res.slot == 0xffffffffu || h.t < st.res.t || (h.t == st.res.t && prim < st.res.prim)) {
+                    st.res.t    = h.t;
+                    st.res.b0   = h.a;
+                    st.res.b1   = h.b;
+                    st.res.b2   = h.c;
+                    st.res.det  = h.det;
+                    st.res.slot = slot;
+                    st.res.prim = prim;
+                    if (any) {
+                        finish_barycentrics(st.res);
+                        return true;
+                    }
+                }
+            }
+        }
+    } else if (node) {
         const float4* np = reinterpret_cast<const float4*>(sc.nodes + st.ref);
         const float4  lx = gld(np + 0), ly = gld(np + 1), lz = gld(np + 2), hx = gld(np + 3), hy = gld(np + 4), hz = gld(np + 5);
         const uint4   rf = gld(reinterpret_cast<const uint4*>(np) + 6);
@@ -245,64 +302,33 @@ __device__ __forceinline__ bool trace_step_dyn(const TraceScene& sc, TraceState&
         sort2(t0, r0, t2, r2);
         sort2(t1, r1, t3, r3);
         sort2(t1, r1, t2, r2);
+        st.ref = kEmptyRef;
         if (t0 != __builtin_inff()) {
             if (t3 != __builtin_inff()) stack_push(stk, st.sp, r3, t3);
             if (t2 != __builtin_inff()) stack_push(stk, st.sp, r2, t2);
             if (t1 != __builtin_inff()) stack_push(stk, st.sp, r1, t1);
             st.ref = r0;
-            return false;
-        }
-    } else {
-        const uint32_t first = leaf_first(st.ref), count = leaf_count(st.ref);
-        for (uint32_t k = 0; k < count; ++k) {
-            const uint32_t slot = first + k;
-            const float4*  tp   = reinterpret_cast<const float4*>(sc.tris + slot);
-            const float4   a    = gld(tp + 0);
-            const float4   b    = gld(tp + 1);
-            const float4   c    = gld(tp + 2);
-            if (STATS) tri_tests += 1;
-            TriHit h;
-            const F3 v0 = mk(a.x, a.y, a.z), v1 = mk(b.x, b.y, b.z), v2 = mk(c.x, c.y, c.z);
-            if (!tri_test(r, v0, v1, v2, st.tmin, st.res.t, h)) continue;
-            const uint32_t flags = __float_as_uint(b.w);
-            if (flags & kTriSingleSided) {
-                const F3 e1      = v1 - v0;
-                const F3 e2      = v2 - v0;
-                F3       outward = cross(e1, e2) * ((flags & kTriMirrored) ? -1.0f : 1.0f);
-                if (dot(r.d, outward) >= 0.0f) continue;
-            }
-            const uint32_t prim = __float_as_uint(a.w);
-            if (st.res.slot == 0xffffffffu || h.t < st.res.t || (h.t == st.res.t && prim < st.res.prim)) {
-                st.res.t    = h.t;
-                st.res.b0   = h.a;
-                st.res.b1   = h.b;
-                st.res.b2   = h.c;
-                st.res.det  = h.det;
-                st.res.slot = slot;
-                st.res.prim = prim;
-                if (any) {
-                    finish_barycentrics(st.res);
-                    return true;
-                }
-            }
         }
     }
-    // pop, skipping entries that start beyond the current closest hit
+    // park a leaf held in ref when the slot is free; take stack entries (skipping those that start
+    // beyond the current closest hit) until ref holds an inner node or a leaf that must wait
     for (;;) {
-        if (st.sp == 0) {
-            finish_barycentrics(st.res);
-            return true;
+        if (st.ref != kEmptyRef) {
+            if (!is_leaf(st.ref) || st.leaf != kEmptyRef) break;
+            st.leaf = st.ref;
+            st.ref  = kEmptyRef;
         }
-        float t;
-        stack_pop(stk, st.sp, st.ref, t);
-        if (!(t > st.res.t)) return false;
+        if (st.sp == 0) break;
+        float    t;
+        uint32_t ref;
+        stack_pop(stk, st.sp, ref, t);
+        if (!(t > st.res.t)) st.ref = ref;
     }
-}
-
-template <bool ANY, bool STATS>
-__device__ __forceinline__ bool trace_step(const TraceScene& sc, TraceState& st, const StackView& stk, uint64_t& bounds_tests,
-                                           uint64_t& tri_tests) {
-    return trace_step_dyn<STATS>(sc, st, stk, bounds_tests, tri_tests, ANY);
+    if (st.ref == kEmptyRef && st.leaf == kEmptyRef) {
+        finish_barycentrics(st.res);
+        return true;
+    }
+    return false;
 }
 
 // Closest hit (ANY = false) or any hit (ANY = true) in [tmin, tmax].

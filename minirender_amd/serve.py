@@ -138,4 +138,8 @@ def compare(requests=None, scenes=None, hbm_budget=64 << 30, max_jobs=64, use_gr
         dev.close()
     out["speedup"] = out["batched"]["msamples_per_s"] / out["plain"]["msamples_per_s"]
     out["speedup_vs_plain_fill"] = out["batched"]["msamples_per_s"] / out["plain_fill"]["msamples_per_s"]
+    # No schedule can finish before the last request arrives: at a fixed arrival rate the largest
+    # speedup over plain that this stream can show is plain's makespan over the last arrival time.
+    out["last_arrival_s"] = max(r.arrival for r in requests)
+    out["speedup_ceiling"] = out["plain"]["makespan_s"] / out["last_arrival_s"]
     return out, films
