@@ -223,16 +223,16 @@ __device__ __forceinline__ void persistent_trace_mixed(const StackView& stk, uin
             // entries is claimed (one atomic) when the current one runs short
             const uint32_t need  = static_cast<uint32_t>(__builtin_popcountll(idle));
             const uint32_t avail = end - cur;
-            uint32_t       nb    = 0;
+            uint32_t       claim = 0;
             if (avail < need) {
-                if (lane_id() == 0) nb = atomicAdd(work, kRefillClaim);
-                nb = __shfl(nb, 0, 64);
+                if (lane_id() == 0) claim = atomicAdd(work, kRefillClaim);
+                claim = __shfl(claim, 0, 64);
             }
             const uint32_t j    = prefix_in_mask(idle);
-            const uint32_t slot = j < avail ? cur + j : nb + (j - avail);
+            const uint32_t slot = j < avail ? cur + j : claim + (j - avail);
             if (avail < need) {
-                cur = nb + (need - avail);
-                end = nb + kRefillClaim;
+                cur = claim + (need - avail);
+                end = claim + kRefillClaim;
             } else {
                 cur += need;
             }
@@ -301,7 +301,7 @@ __global__ void __launch_bounds__(256) k_query(GpuScene sc, const float* __restr
             return ts;
         },
         [&](uint32_t i, const TraceState& st) {
-            const TraceResult& res = st.res;
+            TraceResult res = st.res;
             if (ANY) {
                 occluded[i] = res.slot != 0xffffffffu ? 1 : 0;
             } else {
@@ -310,6 +310,7 @@ __global__ void __launch_bounds__(256) k_query(GpuScene sc, const float* __restr
                 h.primitive = res.slot != 0xffffffffu ? static_cast<int32_t>(res.prim) : -1;
                 hits[i]     = h;
                 if (bary) {
+                    finish_barycentrics(res);
                     bary[static_cast<size_t>(i) * 3 + 0] = res.b0;
                     bary[static_cast<size_t>(i) * 3 + 1] = res.b1;
                     bary[static_cast<size_t>(i) * 3 + 2] = res.b2;
@@ -502,7 +503,7 @@ __global__ void __launch_bounds__(256) k_trace_refill(Pool P, const GpuJob* __re
             return trace_scene(jobs[job].scene);
         },
         [&](uint32_t i, const TraceState& st) {
-            const TraceResult& res = st.res;
+            TraceResult res = st.res;
             if (i < n_sh) {
                 if (res.slot == 0xffffffffu) {
                     const float4 c = P.sh_c[i];
@@ -513,6 +514,7 @@ __global__ void __launch_bounds__(256) k_trace_refill(Pool P, const GpuJob* __re
                     P.rad[path] = l;
                 }
             } else {
+                finish_barycentrics(res);
                 P.hit[path]      = make_float4(res.t, res.b0, res.b1, res.b2);
                 P.hit_slot[path] = res.slot;
             }

@@ -234,7 +234,9 @@ __device__ __forceinline__ void trace_init(TraceState& st, F3 o, F3 d, float tmi
 // One traversal step (all lanes of the wave that trace call it together): the parked leaves' triangles
 // or one inner node per lane, then the lane parks a leaf it reached and takes stack entries until it
 // has an inner node to visit. Returns true once the ray is finished: closest hit (any = false) or any
-// hit (any = true) in [tmin, tmax], result in st.res. `any` may differ between the lanes of a wave.
+// hit (any = true) in [tmin, tmax], result in st.res with the barycentrics still undivided (the
+// caller applies finish_barycentrics where it needs them). `any` may differ between the lanes of a
+// wave.
 template <int LEAF_BATCH, bool STATS>
 __device__ __forceinline__ bool trace_step_dyn(const TraceScene& sc, TraceState& st, const StackView& stk, uint64_t& bounds_tests,
                                                uint64_t& tri_tests, const bool any) {
@@ -276,10 +278,7 @@ __device__ __forceinline__ bool trace_step_dyn(const TraceScene& sc, TraceState&
                     st.res.det  = h.det;
                     st.res.slot = slot;
                     st.res.prim = prim;
-                    if (any) {
-                        finish_barycentrics(st.res);
-                        return true;
-                    }
+                    if (any) return true;
                 }
             }
         }
@@ -310,25 +309,24 @@ __device__ __forceinline__ bool trace_step_dyn(const TraceScene& sc, TraceState&
             st.ref = r0;
         }
     }
-    // park a leaf held in ref when the slot is free; take stack entries (skipping those that start
-    // beyond the current closest hit) until ref holds an inner node or a leaf that must wait
+    // Take stack entries (skipping those that start beyond the current closest hit) while ref holds
+    // nothing, or a leaf that the free park slot can take. kEmptyRef carries the leaf bit, so both
+    // cases read "ref is a leaf, and ref is empty or the slot is free"; parking is a select, since
+    // moving an empty ref into a free slot leaves it free.
     for (;;) {
-        if (st.ref != kEmptyRef) {
-            if (!is_leaf(st.ref) || st.leaf != kEmptyRef) break;
-            st.leaf = st.ref;
-            st.ref  = kEmptyRef;
-        }
-        if (st.sp == 0) break;
+        // non-short-circuit operators: one mask computation instead of nested branches
+        if (!((st.sp != 0) & is_leaf(st.ref) & ((st.ref == kEmptyRef) | (st.leaf == kEmptyRef)))) break;
+        st.leaf = st.leaf == kEmptyRef ? st.ref : st.leaf;
         float    t;
         uint32_t ref;
         stack_pop(stk, st.sp, ref, t);
-        if (!(t > st.res.t)) st.ref = ref;
+        st.ref = t > st.res.t ? kEmptyRef : ref;
     }
-    if (st.ref == kEmptyRef && st.leaf == kEmptyRef) {
-        finish_barycentrics(st.res);
-        return true;
+    if (is_leaf(st.ref) && st.leaf == kEmptyRef) {  // the stack ran out with a leaf in ref
+        st.leaf = st.ref;
+        st.ref  = kEmptyRef;
     }
-    return false;
+    return st.ref == kEmptyRef && st.leaf == kEmptyRef;
 }
 
 // Closest hit (ANY = false) or any hit (ANY = true) in [tmin, tmax].
