@@ -522,6 +522,203 @@ __global__ void __launch_bounds__(256) k_trace_refill(Pool P, const GpuJob* __re
         nb, nt);
 }
 
+// One path of the shading pass (the loop body of k_shade): reads the path state and its hit, adds
+// emission, samples a light and the BSDF, and leaves the continuation and shadow-ray outputs in the
+// references. `job` is the path's job; a caller whose wave holds a single job passes a reference
+// through a wave-uniform index, so the job's scene fields load through the scalar cache.
+template <bool EXT>
+__device__ __forceinline__ void shade_path(const Pool& P, const GpuJob& job, uint32_t path, float4 pv, uint32_t iter, bool& cont,
+                                           bool& emit, uint32_t& ext_key, uint32_t& sh_key, float4& so, float4& sd, float4& sc4,
+                                           uint32_t& n_scatter, uint32_t& n_rr, uint32_t& n_shadow) {
+        const uint32_t flags = __float_as_uint(pv.w);
+        const uint32_t depth = iter;
+        const GpuScene& sc  = job.scene;
+        const float4  ro = P.ray_o[path], rd = P.ray_d[path];
+        const F3      ray_d = mk(rd.x, rd.y, rd.z);
+        // camera paths (bounce 0) start with beta = 1, pdf 0, L = 0, eta scale 1: k_generate
+        // writes none of them and they are not read here
+        const float4  bt = iter == 0 ? make_float4(1.0f, 1.0f, 1.0f, 0.0f) : P.beta[path];
+        const float4  lr = iter == 0 ? make_float4(0.0f, 0.0f, 0.0f, 1.0f) : P.rad[path];
+        F3            beta = mk(bt.x, bt.y, bt.z);
+        float         prev_pdf  = bt.w;
+        F3            L         = mk(lr.x, lr.y, lr.z);
+        float         eta_scale = lr.w;
+        F3            prev_ref  = mk(pv.x, pv.y, pv.z);
+        bool          prev_delta = (flags & kPrevDelta) != 0u;
+        const uint4   rg = P.rng[path];
+        Pcg           rng{static_cast<uint64_t>(rg.x) | (static_cast<uint64_t>(rg.y) << 32),
+                          static_cast<uint64_t>(rg.z) | (static_cast<uint64_t>(rg.w) << 32)};
+        const float4  hv   = P.hit[path];
+        const uint32_t slot = P.hit_slot[path];
+        bool bad = false;
+        (void)ro;
+        do {
+            if (!finite3(beta) || beta.x < 0.0f || beta.y < 0.0f || beta.z < 0.0f) { bad = true; break; }
+            if (vmax(beta) == 0.0f) break;
+            const uint32_t mode = job.mode;
+            const bool unit_weight = depth == 0 || prev_delta || mode == 0u;
+            if (slot == 0xffffffffu) {
+                float w = 1.0f;
+                if (!unit_weight) w = mode == 2u ? power_heuristic(prev_pdf, EXT ? env_light_pdf(sc, ray_d) : sc.env_pdf, bad) : 0.0f;
+                L = L + (beta * (EXT ? env_evaluate(sc, ray_d) : ld3(sc.env))) * w;
+                break;
+            }
+            // surface_hit (bvh.cpp:302-370)
+            const float4* tp = reinterpret_cast<const float4*>(sc.tris + slot);
+            const float4  ta = gld(tp + 0), tb = gld(tp + 1), tc = gld(tp + 2);
+            const F3      v0 = mk(ta.x, ta.y, ta.z), v1 = mk(tb.x, tb.y, tb.z), v2 = mk(tc.x, tc.y, tc.z);
+            const uint32_t tflags = __float_as_uint(tb.w);
+            const float4* np = reinterpret_cast<const float4*>(sc.normals + slot);
+            const float4  na = gld(np + 0), nb = gld(np + 1), nc = gld(np + 2);
+            const uint32_t inst = gld(sc.slot_instance + slot);
+            const GpuMaterial& mat = sc.materials[inst];
+            F3 geometric = normalize(cross(v1 - v0, v2 - v0));
+            if (tflags & kTriMirrored) geometric = -geometric;
+            const float b0 = hv.y, b1 = hv.z, b2 = hv.w;
+            F3 shading  = mk(0.0f, 0.0f, 0.0f);
+            F3 position = mk(0.0f, 0.0f, 0.0f);
+            F3 absolute = mk(0.0f, 0.0f, 0.0f);
+            shading = shading + b0 * normalize(mk(na.x, na.y, na.z));
+            { const F3 term = b0 * v0; position = position + term; absolute = absolute + vabs(term); }
+            shading = shading + b1 * normalize(mk(nb.x, nb.y, nb.z));
+            { const F3 term = b1 * v1; position = position + term; absolute = absolute + vabs(term); }
+            shading = shading + b2 * normalize(mk(nc.x, nc.y, nc.z));
+            { const F3 term = b2 * v2; position = position + term; absolute = absolute + vabs(term); }
+            if (dot(shading, shading) < 1e-24f) {
+                shading = geometric;
+            } else {
+                shading = normalize(shading);
+                if (dot(shading, geometric) < 0.0f) shading = -shading;
+            }
+            // base color: the material's, times the clamped texel at the hit uv = sum of b_i uv_i
+            // (bvh.cpp:318-321, path_tracer.cpp:279-282, bsdf.cpp:122-125)
+            // and the tangent-space normal map of surface_hit (bvh.cpp:334-359)
+            F3 base_color = ld3(mat.color);
+            if (EXT && (mat.texture >= 0 || mat.normal_map >= 0)) {
+                float  u = 0.0f, v = 0.0f;
+                float2 q[3] = {make_float2(0.0f, 0.0f), make_float2(0.0f, 0.0f), make_float2(0.0f, 0.0f)};
+                if (sc.uvs) {
+                    const float2* qp = reinterpret_cast<const float2*>(sc.uvs) + static_cast<uint64_t>(slot) * 3u;
+                    q[0] = gld(qp + 0), q[1] = gld(qp + 1), q[2] = gld(qp + 2);
+                    u = ((u + b0 * q[0].x) + b1 * q[1].x) + b2 * q[2].x;
+                    v = ((v + b0 * q[0].y) + b1 * q[1].y) + b2 * q[2].y;
+                }
+                if (mat.normal_map >= 0 && mat.normal_scale > 0.0f) apply_normal_map(sc, mat, v0, v1, v2, q, u, v, geometric, shading);
+                if (mat.texture >= 0) {
+                    const F3 tc = texture_sample(sc, mat.texture, mat.filter, u, v);
+                    base_color  = base_color * mk(sclamp(tc.x, 0.0f, 1.0f), sclamp(tc.y, 0.0f, 1.0f), sclamp(tc.z, 0.0f, 1.0f));
+                }
+            }
+            const bool front_face = dot(ray_d, geometric) < 0.0f;
+            const F3   perr = absolute * (7.0f * kEpsF);
+            const F3   ng   = geometric;
+            // emission with its MIS weight
+            const F3 emission = ld3(mat.emission);
+            if (emission.x != 0.0f || emission.y != 0.0f || emission.z != 0.0f) {
+                float w = 1.0f;
+                if (!unit_weight) {
+                    w = mode == 2u ? power_heuristic(prev_pdf, area_pdf(sc, sc.slot_emitter[slot], prev_ref, position), bad) : 0.0f;
+                }
+                L = L + (beta * emission) * w;
+            }
+            if (depth == job.max_depth) break;
+            // shading frame and BSDF
+            F3 normal = mat.kind == 3u ? ng : shading;
+            const F3 wo_world = -ray_d;
+            if (dot(normal, wo_world) < 0.0f) normal = -normal;
+            const Frame f  = make_frame(normal);
+            const F3    wo = to_local(f, wo_world);
+            BsdfParams  bp;
+            bp.kind  = mat.kind;
+            bp.color = base_color;
+            bp.alpha = mat.alpha;
+            bp.eta_i = front_face ? 1.0f : mat.ior;
+            bp.eta_t = front_face ? mat.ior : 1.0f;
+            bp.roughness = mat.roughness;
+            bp.table     = P.energy;
+            bp.energy    = EXT && mat.kind == 1u && mat.compensate != 0u && !bp.is_delta();
+            const float light_u   = next_1d(rng);
+            const float su_x      = next_1d(rng);
+            const float su_y      = next_1d(rng);
+            const float component = next_1d(rng);
+            const float du_x      = next_1d(rng);
+            const float du_y      = next_1d(rng);
+            const F3    won       = normalize(wo);
+            bsdf_prepare(bp, won);
+            if (!bp.is_delta()) {
+                LightSample ls;
+                if (sample_light<EXT>(sc, position, light_u, su_x, su_y, ls) && (mode != 0u || ls.discrete)) {
+                    const F3   wi   = to_local(f, ls.dir);
+                    const bool side = dot(wo_world, ng) * dot(ls.dir, ng) > 0.0f;
+                    if (wi.z > 0.0f && side) {
+                        ++n_shadow;
+                        F3    origin = offset_origin(position, ng, perr, ls.dir);
+                        F3    sdir   = normalize(ls.dir);
+                        float limit  = __builtin_inff();
+                        bool  zero   = false;
+                        if (__builtin_isfinite(ls.distance)) {
+                            const F3 target = offset_origin(ls.position, ls.normal, ls.error, -ls.dir);
+                            const F3 diff   = target - origin;
+                            limit           = length(diff);
+                            if (limit == 0.0f) {
+                                zero = true;
+                            } else {
+                                sdir  = diff / limit;
+                                limit = next_after(limit, 0.0f);
+                            }
+                        }
+                        if (!zero) {
+                            const F3 win = normalize(wi);
+                            float weight = 1.0f;
+                            if (mode == 2u && !ls.discrete) weight = power_heuristic(ls.probability, bsdf_pdf_n(bp, won, win), bad);
+                            const F3 c = (((beta * bsdf_eval_n(bp, won, win)) * wi.z) * ls.radiance) * (weight / ls.probability);
+                            emit   = true;
+                            sh_key = octant(sdir.x, sdir.y, sdir.z);
+                            so   = make_float4(origin.x, origin.y, origin.z, limit);
+                            sd   = make_float4(sdir.x, sdir.y, sdir.z, 0.0f);
+                            sc4  = make_float4(c.x, c.y, c.z, 0.0f);
+                        }
+                    }
+                }
+            }
+            BsdfSample bs;
+            if (!bsdf_sample(bp, won, component, du_x, du_y, bs)) break;
+            if (vmax(bs.weight) == 0.0f) break;
+            const F3    wi_world = normalize(to_world(f, bs.dir));
+            const float sides    = dot(wo_world, ng) * dot(wi_world, ng);
+            if ((bs.transmission && sides >= 0.0f) || (!bs.transmission && sides <= 0.0f)) break;
+            beta = beta * bs.weight;
+            if (!finite3(beta) || beta.x < 0.0f || beta.y < 0.0f || beta.z < 0.0f) { bad = true; break; }
+            prev_ref   = position;
+            prev_pdf   = bs.probability;
+            prev_delta = bs.discrete;
+            eta_scale  = eta_scale * (bs.eta * bs.eta);
+            ++n_scatter;
+            const F3 no = offset_origin(position, ng, perr, wi_world);
+            const F3 nd = normalize(wi_world);
+            const float roulette_u = next_1d(rng);
+            if (job.rr && depth + 1 >= job.rr_depth) {
+                const float survival = sclamp(vmax(beta * eta_scale), 0.05f, 0.95f);
+                if (roulette_u >= survival) {
+                    ++n_rr;
+                    break;
+                }
+                beta = beta / survival;
+            }
+            cont    = true;
+            ext_key = octant(nd.x, nd.y, nd.z);
+            P.ray_o[path] = make_float4(no.x, no.y, no.z, 0.0f);
+            P.ray_d[path] = make_float4(nd.x, nd.y, nd.z, __builtin_inff());
+            P.beta[path]  = make_float4(beta.x, beta.y, beta.z, prev_pdf);
+            P.prev[path]  = make_float4(prev_ref.x, prev_ref.y, prev_ref.z,
+                                        __uint_as_float((flags & ~kPrevDelta) | (prev_delta ? kPrevDelta : 0u)));
+            P.rng[path]   = make_uint4(static_cast<uint32_t>(rng.state), static_cast<uint32_t>(rng.state >> 32),
+                                       static_cast<uint32_t>(rng.inc), static_cast<uint32_t>(rng.inc >> 32));
+        } while (false);
+        P.rad[path] = make_float4(L.x, L.y, L.z, eta_scale);
+        if (bad) atomicOr(job.error_flag, 1u);
+}
+
 // Shading of one bounce (PathTracer::trace loop body, path_tracer.cpp:164-363).
 // At most 128 VGPRs (4 waves per SIMD): the kernel is bound by the latency of its path-state
 // and scene loads, so occupancy pays more than the registers the compiler would otherwise use.
@@ -535,7 +732,6 @@ __global__ void __launch_bounds__(kShadeThreads) __attribute__((amdgpu_waves_per
     uint32_t*       next_n  = P.counters + kCtrExt + iter + 1;
     uint32_t*       sh_n    = P.counters + kCtrShadow + iter;
     const uint32_t  stride  = gridDim.x * blockDim.x;
-    const uint32_t  depth   = iter;
     uint32_t        n_scatter = 0, n_rr = 0, n_shadow = 0;
     __shared__ uint32_t s_off[16 * (kShadeThreads / 64)];
     __shared__ uint32_t s_base[2];
@@ -549,194 +745,13 @@ __global__ void __launch_bounds__(kShadeThreads) __attribute__((amdgpu_waves_per
         float4         so = make_float4(0, 0, 0, 0), sd = so, sc4 = so;
         if (i < n) {
             path = queue[i];
-            const float4  pv    = P.prev[path];
-            const uint32_t flags = __float_as_uint(pv.w);
-            const GpuJob& job   = jobs[flags >> 16];
-            const GpuScene& sc  = job.scene;
-            const float4  ro = P.ray_o[path], rd = P.ray_d[path];
-            const F3      ray_d = mk(rd.x, rd.y, rd.z);
-            // camera paths (bounce 0) start with beta = 1, pdf 0, L = 0, eta scale 1: k_generate
-            // writes none of them and they are not read here
-            const float4  bt = iter == 0 ? make_float4(1.0f, 1.0f, 1.0f, 0.0f) : P.beta[path];
-            const float4  lr = iter == 0 ? make_float4(0.0f, 0.0f, 0.0f, 1.0f) : P.rad[path];
-            F3            beta = mk(bt.x, bt.y, bt.z);
-            float         prev_pdf  = bt.w;
-            F3            L         = mk(lr.x, lr.y, lr.z);
-            float         eta_scale = lr.w;
-            F3            prev_ref  = mk(pv.x, pv.y, pv.z);
-            bool          prev_delta = (flags & kPrevDelta) != 0u;
-            const uint4   rg = P.rng[path];
-            Pcg           rng{static_cast<uint64_t>(rg.x) | (static_cast<uint64_t>(rg.y) << 32),
-                              static_cast<uint64_t>(rg.z) | (static_cast<uint64_t>(rg.w) << 32)};
-            const float4  hv   = P.hit[path];
-            const uint32_t slot = P.hit_slot[path];
-            bool bad = false;
-            (void)ro;
-            do {
-                if (!finite3(beta) || beta.x < 0.0f || beta.y < 0.0f || beta.z < 0.0f) { bad = true; break; }
-                if (vmax(beta) == 0.0f) break;
-                const uint32_t mode = job.mode;
-                const bool unit_weight = depth == 0 || prev_delta || mode == 0u;
-                if (slot == 0xffffffffu) {
-                    float w = 1.0f;
-                    if (!unit_weight) w = mode == 2u ? power_heuristic(prev_pdf, EXT ? env_light_pdf(sc, ray_d) : sc.env_pdf, bad) : 0.0f;
-                    L = L + (beta * (EXT ? env_evaluate(sc, ray_d) : ld3(sc.env))) * w;
-                    break;
-                }
-                // surface_hit (bvh.cpp:302-370)
-                const float4* tp = reinterpret_cast<const float4*>(sc.tris + slot);
-                const float4  ta = gld(tp + 0), tb = gld(tp + 1), tc = gld(tp + 2);
-                const F3      v0 = mk(ta.x, ta.y, ta.z), v1 = mk(tb.x, tb.y, tb.z), v2 = mk(tc.x, tc.y, tc.z);
-                const uint32_t tflags = __float_as_uint(tb.w);
-                const float4* np = reinterpret_cast<const float4*>(sc.normals + slot);
-                const float4  na = gld(np + 0), nb = gld(np + 1), nc = gld(np + 2);
-                const uint32_t inst = gld(sc.slot_instance + slot);
-                const GpuMaterial& mat = sc.materials[inst];
-                F3 geometric = normalize(cross(v1 - v0, v2 - v0));
-                if (tflags & kTriMirrored) geometric = -geometric;
-                const float b0 = hv.y, b1 = hv.z, b2 = hv.w;
-                F3 shading  = mk(0.0f, 0.0f, 0.0f);
-                F3 position = mk(0.0f, 0.0f, 0.0f);
-                F3 absolute = mk(0.0f, 0.0f, 0.0f);
-                shading = shading + b0 * normalize(mk(na.x, na.y, na.z));
-                { const F3 term = b0 * v0; position = position + term; absolute = absolute + vabs(term); }
-                shading = shading + b1 * normalize(mk(nb.x, nb.y, nb.z));
-                { const F3 term = b1 * v1; position = position + term; absolute = absolute + vabs(term); }
-                shading = shading + b2 * normalize(mk(nc.x, nc.y, nc.z));
-                { const F3 term = b2 * v2; position = position + term; absolute = absolute + vabs(term); }
-                if (dot(shading, shading) < 1e-24f) {
-                    shading = geometric;
-                } else {
-                    shading = normalize(shading);
-                    if (dot(shading, geometric) < 0.0f) shading = -shading;
-                }
-                // base color: the material's, times the clamped texel at the hit uv = sum of b_i uv_i
-                // (bvh.cpp:318-321, path_tracer.cpp:279-282, bsdf.cpp:122-125)
-                // and the tangent-space normal map of surface_hit (bvh.cpp:334-359)
-                F3 base_color = ld3(mat.color);
-                if (EXT && (mat.texture >= 0 || mat.normal_map >= 0)) {
-                    float  u = 0.0f, v = 0.0f;
-                    float2 q[3] = {make_float2(0.0f, 0.0f), make_float2(0.0f, 0.0f), make_float2(0.0f, 0.0f)};
-                    if (sc.uvs) {
-                        const float2* qp = reinterpret_cast<const float2*>(sc.uvs) + static_cast<uint64_t>(slot) * 3u;
-                        q[0] = gld(qp + 0), q[1] = gld(qp + 1), q[2] = gld(qp + 2);
-                        u = ((u + b0 * q[0].x) + b1 * q[1].x) + b2 * q[2].x;
-                        v = ((v + b0 * q[0].y) + b1 * q[1].y) + b2 * q[2].y;
-                    }
-                    if (mat.normal_map >= 0 && mat.normal_scale > 0.0f) apply_normal_map(sc, mat, v0, v1, v2, q, u, v, geometric, shading);
-                    if (mat.texture >= 0) {
-                        const F3 tc = texture_sample(sc, mat.texture, mat.filter, u, v);
-                        base_color  = base_color * mk(sclamp(tc.x, 0.0f, 1.0f), sclamp(tc.y, 0.0f, 1.0f), sclamp(tc.z, 0.0f, 1.0f));
-                    }
-                }
-                const bool front_face = dot(ray_d, geometric) < 0.0f;
-                const F3   perr = absolute * (7.0f * kEpsF);
-                const F3   ng   = geometric;
-                // emission with its MIS weight
-                const F3 emission = ld3(mat.emission);
-                if (emission.x != 0.0f || emission.y != 0.0f || emission.z != 0.0f) {
-                    float w = 1.0f;
-                    if (!unit_weight) {
-                        w = mode == 2u ? power_heuristic(prev_pdf, area_pdf(sc, sc.slot_emitter[slot], prev_ref, position), bad) : 0.0f;
-                    }
-                    L = L + (beta * emission) * w;
-                }
-                if (depth == job.max_depth) break;
-                // shading frame and BSDF
-                F3 normal = mat.kind == 3u ? ng : shading;
-                const F3 wo_world = -ray_d;
-                if (dot(normal, wo_world) < 0.0f) normal = -normal;
-                const Frame f  = make_frame(normal);
-                const F3    wo = to_local(f, wo_world);
-                BsdfParams  bp;
-                bp.kind  = mat.kind;
-                bp.color = base_color;
-                bp.alpha = mat.alpha;
-                bp.eta_i = front_face ? 1.0f : mat.ior;
-                bp.eta_t = front_face ? mat.ior : 1.0f;
-                bp.roughness = mat.roughness;
-                bp.table     = P.energy;
-                bp.energy    = EXT && mat.kind == 1u && mat.compensate != 0u && !bp.is_delta();
-                const float light_u   = next_1d(rng);
-                const float su_x      = next_1d(rng);
-                const float su_y      = next_1d(rng);
-                const float component = next_1d(rng);
-                const float du_x      = next_1d(rng);
-                const float du_y      = next_1d(rng);
-                const F3    won       = normalize(wo);
-                bsdf_prepare(bp, won);
-                if (!bp.is_delta()) {
-                    LightSample ls;
-                    if (sample_light<EXT>(sc, position, light_u, su_x, su_y, ls) && (mode != 0u || ls.discrete)) {
-                        const F3   wi   = to_local(f, ls.dir);
-                        const bool side = dot(wo_world, ng) * dot(ls.dir, ng) > 0.0f;
-                        if (wi.z > 0.0f && side) {
-                            ++n_shadow;
-                            F3    origin = offset_origin(position, ng, perr, ls.dir);
-                            F3    sdir   = normalize(ls.dir);
-                            float limit  = __builtin_inff();
-                            bool  zero   = false;
-                            if (__builtin_isfinite(ls.distance)) {
-                                const F3 target = offset_origin(ls.position, ls.normal, ls.error, -ls.dir);
-                                const F3 diff   = target - origin;
-                                limit           = length(diff);
-                                if (limit == 0.0f) {
-                                    zero = true;
-                                } else {
-                                    sdir  = diff / limit;
-                                    limit = next_after(limit, 0.0f);
-                                }
-                            }
-                            if (!zero) {
-                                const F3 win = normalize(wi);
-                                float weight = 1.0f;
-                                if (mode == 2u && !ls.discrete) weight = power_heuristic(ls.probability, bsdf_pdf_n(bp, won, win), bad);
-                                const F3 c = (((beta * bsdf_eval_n(bp, won, win)) * wi.z) * ls.radiance) * (weight / ls.probability);
-                                emit   = true;
-                                sh_key = octant(sdir.x, sdir.y, sdir.z);
-                                so   = make_float4(origin.x, origin.y, origin.z, limit);
-                                sd   = make_float4(sdir.x, sdir.y, sdir.z, 0.0f);
-                                sc4  = make_float4(c.x, c.y, c.z, 0.0f);
-                            }
-                        }
-                    }
-                }
-                BsdfSample bs;
-                if (!bsdf_sample(bp, won, component, du_x, du_y, bs)) break;
-                if (vmax(bs.weight) == 0.0f) break;
-                const F3    wi_world = normalize(to_world(f, bs.dir));
-                const float sides    = dot(wo_world, ng) * dot(wi_world, ng);
-                if ((bs.transmission && sides >= 0.0f) || (!bs.transmission && sides <= 0.0f)) break;
-                beta = beta * bs.weight;
-                if (!finite3(beta) || beta.x < 0.0f || beta.y < 0.0f || beta.z < 0.0f) { bad = true; break; }
-                prev_ref   = position;
-                prev_pdf   = bs.probability;
-                prev_delta = bs.discrete;
-                eta_scale  = eta_scale * (bs.eta * bs.eta);
-                ++n_scatter;
-                const F3 no = offset_origin(position, ng, perr, wi_world);
-                const F3 nd = normalize(wi_world);
-                const float roulette_u = next_1d(rng);
-                if (job.rr && depth + 1 >= job.rr_depth) {
-                    const float survival = sclamp(vmax(beta * eta_scale), 0.05f, 0.95f);
-                    if (roulette_u >= survival) {
-                        ++n_rr;
-                        break;
-                    }
-                    beta = beta / survival;
-                }
-                cont    = true;
-                ext_key = octant(nd.x, nd.y, nd.z);
-                P.ray_o[path] = make_float4(no.x, no.y, no.z, 0.0f);
-                P.ray_d[path] = make_float4(nd.x, nd.y, nd.z, __builtin_inff());
-                P.beta[path]  = make_float4(beta.x, beta.y, beta.z, prev_pdf);
-                P.prev[path]  = make_float4(prev_ref.x, prev_ref.y, prev_ref.z,
-                                            __uint_as_float((flags & ~kPrevDelta) | (prev_delta ? kPrevDelta : 0u)));
-                P.rng[path]   = make_uint4(static_cast<uint32_t>(rng.state), static_cast<uint32_t>(rng.state >> 32),
-                                           static_cast<uint32_t>(rng.inc), static_cast<uint32_t>(rng.inc >> 32));
-            } while (false);
-            P.rad[path] = make_float4(L.x, L.y, L.z, eta_scale);
-            if (bad) atomicOr(job.error_flag, 1u);
+            const float4   pv  = P.prev[path];
+            const uint32_t jid = __float_as_uint(pv.w) >> 16;
+            const uint32_t j0  = __builtin_amdgcn_readfirstlane(jid);
+            if (__ballot(jid != j0) == 0)
+                shade_path<EXT>(P, jobs[j0], path, pv, iter, cont, emit, ext_key, sh_key, so, sd, sc4, n_scatter, n_rr, n_shadow);
+            else
+                shade_path<EXT>(P, jobs[jid], path, pv, iter, cont, emit, ext_key, sh_key, so, sd, sc4, n_scatter, n_rr, n_shadow);
         }
         const GroupedSlots g = block_reserve_grouped<kShadeThreads / 64>(next_n, cont, ext_key, sh_n, emit, sh_key, s_off, s_base);
         const uint32_t cpos = g.ext, spos = g.shadow;
