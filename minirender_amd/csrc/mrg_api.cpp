@@ -203,7 +203,7 @@ struct mrg_device {
     mrg_kernel_times                   times{};
     std::vector<hipEvent_t>            events;
     hipEvent_t                         query_events[2] = {nullptr, nullptr};
-    std::map<uint32_t, hipGraphExec_t> graphs;  // by bounce count, extended shading and refill traversal
+    std::map<uint32_t, hipGraphExec_t> graphs;  // by bounce count, extended shading, refill traversal, one scene
 };
 
 namespace {
@@ -597,6 +597,7 @@ struct BatchPlan {
     uint32_t               slots = 0, pixels = 0, max_depth = 0;
     bool                   extended = false;  // some job's scene needs the extended shading kernel
     bool                   refill   = false;  // bounce traversal with the refill driver (k_trace_refill)
+    bool                   one_scene = true;  // every job of the batch renders the same scene
 };
 
 // Traversal kernel of a batch under MRG_TRAVERSAL_AUTO: the refill driver (with parked leaves) when
@@ -700,6 +701,7 @@ BatchPlan plan_batch_locked(mrg_device* d) {
         b.gjobs.push_back(gpu_job(j));
         b.extended = b.extended || j->scene->extended;
         b.refill   = b.refill || refill_traversal(d, j->scene);
+        b.one_scene = b.one_scene && j->scene == b.jobs.front()->scene;
         b.max_depth = std::max(b.max_depth, j->desc.maximum_depth);
     }
     if (batched && !b.jobs.empty() && b.slots < cap) {
@@ -753,11 +755,11 @@ BatchPlan plan_batch_locked(mrg_device* d) {
 
 bool fused_traversal(const mrg_device* d) { return d->opt.traversal != MRG_TRAVERSAL_SPLIT; }
 
-using TraceLauncher = hipError_t (*)(const Pool&, const GpuJob*, uint32_t, const LaunchShape&, hipStream_t);
-
-void launch_iteration_kernels(mrg_device* d, uint32_t iters, bool extended, bool refill, bool timed,
+void launch_iteration_kernels(mrg_device* d, uint32_t iters, bool extended, bool refill, bool one_scene, bool timed,
                               std::vector<std::pair<int, int>>& marks) {
-    const TraceLauncher trace = refill ? launch_trace_refill : launch_trace;
+    auto trace = [&](const Pool& P, const GpuJob* jobs, uint32_t it, const LaunchShape& shape, hipStream_t stream) {
+        return refill ? launch_trace_refill(P, jobs, it, one_scene, shape, stream) : launch_trace(P, jobs, it, shape, stream);
+    };
     const GpuJob*   jobs   = d->jobs_buf.as<GpuJob>();
     const GpuChunk* chunks = d->chunks_buf.as<GpuChunk>();
     int             ev     = 0;
@@ -806,12 +808,12 @@ void run_batch(mrg_device* d, BatchPlan& b) {
     std::vector<std::pair<int, int>> marks;
     const bool timed = d->opt.use_graph == 0;
     if (d->opt.use_graph) {
-        const uint32_t key = (iters * 2u + (b.extended ? 1u : 0u)) * 2u + (b.refill ? 1u : 0u);
+        const uint32_t key = ((iters * 2u + (b.extended ? 1u : 0u)) * 2u + (b.refill ? 1u : 0u)) * 2u + (b.one_scene ? 1u : 0u);
         auto           it  = d->graphs.find(key);
         if (it == d->graphs.end()) {
             hipGraph_t graph;
             check_hip(hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-            launch_iteration_kernels(d, iters, b.extended, b.refill, false, marks);
+            launch_iteration_kernels(d, iters, b.extended, b.refill, b.one_scene, false, marks);
             check_hip(hipStreamEndCapture(d->stream, &graph), "hipStreamEndCapture");
             hipGraphExec_t exec;
             check_hip(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
@@ -820,7 +822,7 @@ void run_batch(mrg_device* d, BatchPlan& b) {
         }
         check_hip(hipGraphLaunch(it->second, d->stream), "hipGraphLaunch");
     } else {
-        launch_iteration_kernels(d, iters, b.extended, b.refill, timed, marks);
+        launch_iteration_kernels(d, iters, b.extended, b.refill, b.one_scene, timed, marks);
     }
     std::vector<uint32_t> ctr(kCounterWords);
     uint64_t              st[kStatWords];

@@ -83,7 +83,9 @@ hipError_t launch_shadow(const Pool& P, const GpuJob* jobs, uint32_t iter, const
 // Fused traversal: shadow rays of bounce iter - 1 and extension rays of bounce iter in one launch.
 hipError_t launch_trace(const Pool& P, const GpuJob* jobs, uint32_t iter, const LaunchShape& shape, hipStream_t stream);
 // The same with the refill driver (finished lanes take the next queue entry at once).
-hipError_t launch_trace_refill(const Pool& P, const GpuJob* jobs, uint32_t iter, const LaunchShape& shape, hipStream_t stream);
+// one_scene: every job of the batch renders jobs[0]'s scene (the kernel reads it once per wave).
+hipError_t launch_trace_refill(const Pool& P, const GpuJob* jobs, uint32_t iter, bool one_scene, const LaunchShape& shape,
+                               hipStream_t stream);
 hipError_t launch_validate(const Pool& P, const GpuJob* jobs, const GpuChunk* chunks, const LaunchShape& shape, hipStream_t stream);
 hipError_t launch_accumulate(const Pool& P, const GpuJob* jobs, const GpuChunk* chunks, const LaunchShape& shape, hipStream_t stream);
 

@@ -185,9 +185,11 @@ __device__ __forceinline__ StackView make_stack(uint32_t* lds_ref, float* lds_t,
 // Persistent traversal with dynamic refill: every lane keeps its own ray; whenever at least
 // REFILL_LANES lanes of the wave are idle, they take the next entries of the wave's claimed range
 // (one atomic per kRefillClaim entries) and start new rays, so a wave does not wait for its slowest
-// ray. `start(i, st, any)` initialises item i and says whether it is an any-hit query (lanes of one
-// wave may trace both kinds); `finish(i, st)` consumes its result at the lane's next refill (or
-// when the wave drains), so the lanes that finished since the last refill run it together.
+// ray. `start(i, st, any)` initialises item i, says whether it is an any-hit query (lanes of one
+// wave may trace both kinds) and returns its scene; `finish(i, st)` consumes its result at the
+// lane's next refill (or when the wave drains), so the lanes that finished since the last refill run
+// it together. UNIFORM: every item traces `uscene` (kept in scalar registers, so node and triangle
+// addresses are a scalar base plus a lane offset), and start's scene is ignored.
 // Thresholds, measured on the 16.7M-ray icosphere microbench and the 1080p terrain job: ray queries
 // start a ray from one packed record and refill at 16 idle lanes; bounce traversal gathers each ray
 // through the path queue and refills at 40 (32: -2.5%, 48: -1.3%, 56: -16%). LEAF_BATCH is the number
@@ -199,9 +201,9 @@ constexpr int      kQueryLeafBatch    = 24;
 constexpr int      kBounceLeafBatch   = 24;
 constexpr uint32_t kRefillClaim = 128;  // entries per atomic claim (>= 64; 128 measured best, +61% over an atomic per refill)
 
-template <int REFILL_LANES, int LEAF_BATCH, bool STATS, class Start, class Finish>
-__device__ __forceinline__ void persistent_trace_mixed(const StackView& stk, uint32_t n, uint32_t* work, Start start, Finish finish,
-                                                       uint64_t& nb, uint64_t& nt) {
+template <int REFILL_LANES, int LEAF_BATCH, bool STATS, bool UNIFORM, class Start, class Finish>
+__device__ __forceinline__ void persistent_trace_mixed(const StackView& stk, uint32_t n, uint32_t* work, const TraceScene& uscene,
+                                                       Start start, Finish finish, uint64_t& nb, uint64_t& nt) {
     TraceState st;
     TraceScene ts{nullptr, nullptr, 0};
     bool       active  = false;
@@ -238,13 +240,14 @@ __device__ __forceinline__ void persistent_trace_mixed(const StackView& stk, uin
             }
             drained = __ballot(!active && slot >= n) != 0;
             if (!active && slot < n) {
-                item   = slot;
-                ts     = start(item, st, any);
+                item                = slot;
+                const TraceScene sc = start(item, st, any);
+                if (!UNIFORM) ts = sc;
                 active = true;
             }
         }
         if (__ballot(active) == 0) break;
-        if (active && trace_step_dyn<LEAF_BATCH, STATS>(ts, st, stk, nb, nt, any)) {
+        if (active && trace_step_dyn<LEAF_BATCH, STATS>(UNIFORM ? uscene : ts, st, stk, nb, nt, any)) {
             active  = false;
             pending = true;
         }
@@ -252,15 +255,16 @@ __device__ __forceinline__ void persistent_trace_mixed(const StackView& stk, uin
     if (pending) finish(item, st);
 }
 
-// Every item of one kind: closest hit (ANY = false) or any hit (ANY = true).
+// Every item of one kind against one scene: closest hit (ANY = false) or any hit (ANY = true).
 template <bool ANY, bool STATS, class Start, class Finish>
-__device__ __forceinline__ void persistent_trace(const StackView& stk, uint32_t n, uint32_t* work, Start start, Finish finish,
-                                                 uint64_t& nb, uint64_t& nt) {
-    persistent_trace_mixed<kQueryRefillLanes, kQueryLeafBatch, STATS>(
-        stk, n, work,
+__device__ __forceinline__ void persistent_trace(const StackView& stk, uint32_t n, uint32_t* work, const TraceScene& scene,
+                                                 Start start, Finish finish, uint64_t& nb, uint64_t& nt) {
+    persistent_trace_mixed<kQueryRefillLanes, kQueryLeafBatch, STATS, true>(
+        stk, n, work, scene,
         [&](uint32_t i, TraceState& st, bool& any) {
             any = ANY;
-            return start(i, st);
+            start(i, st);
+            return scene;
         },
         finish, nb, nt);
 }
@@ -294,11 +298,10 @@ __global__ void __launch_bounds__(256) k_query(GpuScene sc, const float* __restr
     uint64_t        nb = 0, nt = 0;
     const TraceScene ts = trace_scene(sc);
     persistent_trace<ANY, STATS>(
-        stk, n, work,
+        stk, n, work, ts,
         [&](uint32_t i, TraceState& st) {
             const float* r = rays + static_cast<size_t>(i) * 6;
             trace_init(st, mk(r[0], r[1], r[2]), mk(r[3], r[4], r[5]), tmin, tmax);
-            return ts;
         },
         [&](uint32_t i, const TraceState& st) {
             TraceResult res = st.res;
@@ -474,6 +477,8 @@ __global__ void __launch_bounds__(256) k_trace(Pool P, const GpuJob* __restrict_
 // Fused traversal of one bounce with the refill driver: the same work as k_trace (shadow rays of
 // bounce iter - 1 first, then the extension rays of bounce iter), but lanes that finish their ray
 // take the next queue entry at once, so a wave can trace shadow and extension rays side by side.
+// ONE_SCENE: every job of the batch renders the same scene (jobs[0]'s), read once per wave.
+template <bool ONE_SCENE>
 __global__ void __launch_bounds__(256) k_trace_refill(Pool P, const GpuJob* __restrict__ jobs, uint32_t iter) {
     __shared__ uint32_t s_ref[kStackLds * kTraversalBlock];
     __shared__ float    s_t[kStackLds * kTraversalBlock];
@@ -484,8 +489,9 @@ __global__ void __launch_bounds__(256) k_trace_refill(Pool P, const GpuJob* __re
     uint32_t*       work  = P.counters + kCtrWorkExt + iter;
     uint64_t        nb = 0, nt = 0;
     uint32_t        path = 0;
-    persistent_trace_mixed<kBounceRefillLanes, kBounceLeafBatch, false>(
-        stk, n_sh + n_ext, work,
+    const TraceScene one = ONE_SCENE ? trace_scene(jobs[0].scene) : TraceScene{nullptr, nullptr, 0};
+    persistent_trace_mixed<kBounceRefillLanes, kBounceLeafBatch, false, ONE_SCENE>(
+        stk, n_sh + n_ext, work, one,
         [&](uint32_t i, TraceState& st, bool& any) {
             any = i < n_sh;
             if (any) {
@@ -499,6 +505,7 @@ __global__ void __launch_bounds__(256) k_trace_refill(Pool P, const GpuJob* __re
                 const float4 rd = P.ray_d[path];
                 trace_init(st, mk(ro.x, ro.y, ro.z), mk(rd.x, rd.y, rd.z), ro.w, rd.w);
             }
+            if (ONE_SCENE) return one;
             const uint32_t job = __float_as_uint(P.prev[path].w) >> 16;
             return trace_scene(jobs[job].scene);
         },
@@ -886,8 +893,10 @@ hipError_t launch_trace(const Pool& P, const GpuJob* jobs, uint32_t iter, const 
     return hipGetLastError();
 }
 
-hipError_t launch_trace_refill(const Pool& P, const GpuJob* jobs, uint32_t iter, const LaunchShape& shape, hipStream_t stream) {
-    hipLaunchKernelGGL(k_trace_refill, dim3(shape.traversal_blocks), dim3(256), 0, stream, P, jobs, iter);
+hipError_t launch_trace_refill(const Pool& P, const GpuJob* jobs, uint32_t iter, bool one_scene, const LaunchShape& shape,
+                               hipStream_t stream) {
+    if (one_scene) hipLaunchKernelGGL(k_trace_refill<true>, dim3(shape.traversal_blocks), dim3(256), 0, stream, P, jobs, iter);
+    else hipLaunchKernelGGL(k_trace_refill<false>, dim3(shape.traversal_blocks), dim3(256), 0, stream, P, jobs, iter);
     return hipGetLastError();
 }
 
