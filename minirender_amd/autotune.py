@@ -3,11 +3,12 @@
 The knobs are the launch and queue choices of the wavefront integrator:
 
 * ``path_capacity``             paths in flight per batch (the extension / shadow queue capacity),
-* ``persistent_blocks_per_cu``  resident 256-thread blocks per CU of the persistent traversal grid,
+* ``persistent_blocks_per_cu``  resident 256-thread blocks per CU of the persistent traversal grids
+                                (0: the library's choice, 4 for fused / split and 6 for refill),
 * ``traversal``                 ``split`` (closest-hit and any-hit kernels per bounce), ``fused``
                                 (one traversal launch per bounce for both kinds of ray), ``refill``
                                 (the fused launch where lanes that finish take the next ray at once)
-                                or ``auto`` (refill for batches holding a scene of >= 2^17
+                                or ``auto`` (refill for batches holding a scene of >= 2^14
                                 triangles, fused otherwise),
 * ``use_graph``                 the bounce loop captured once in a hipGraph and replayed,
 * ``max_jobs_in_flight``        (batched schedule) jobs co-scheduled in one batch.
@@ -27,13 +28,13 @@ from . import api
 
 KNOBS = {
     "path_capacity": [1 << 22, 1 << 23, 1 << 24, 1 << 25, 1 << 26, 1 << 27],
-    "persistent_blocks_per_cu": [2, 3, 4, 5],
+    "persistent_blocks_per_cu": [0, 2, 3, 4, 5, 6],
     "traversal": ["split", "fused", "refill", "auto"],
     "use_graph": [False, True],
     "max_jobs_in_flight": [8, 16, 32, 64],
 }
 
-DEFAULTS = {"path_capacity": 1 << 26, "persistent_blocks_per_cu": 4, "traversal": "auto", "use_graph": False,
+DEFAULTS = {"path_capacity": 1 << 26, "persistent_blocks_per_cu": 0, "traversal": "auto", "use_graph": False,
             "max_jobs_in_flight": 64}
 
 

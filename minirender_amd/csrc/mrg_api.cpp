@@ -236,7 +236,8 @@ void alloc_pool(mrg_device* d) {
     pl.q_ext0  = reinterpret_cast<uint32_t*>(take(P * 4));
     pl.q_ext1  = reinterpret_cast<uint32_t*>(take(P * 4));
     pl.q_sh    = reinterpret_cast<uint32_t*>(take(P * 4));
-    const uint64_t lanes = uint64_t(std::max(d->shape.traversal_blocks, 1u)) * 256;
+    // spill columns for the larger of the two persistent grids (a lane's column is its global id)
+    const uint64_t lanes = uint64_t(std::max({d->shape.traversal_blocks, d->shape.refill_blocks, 1u})) * 256;
     d->spill_ref.alloc(lanes * (kStackTotal - kStackLds) * 4);
     d->spill_t.alloc(lanes * (kStackTotal - kStackLds) * 4);
     d->counters.alloc(kCounterWords * 4);
@@ -600,12 +601,13 @@ struct BatchPlan {
     bool                   one_scene = true;  // every job of the batch renders the same scene
 };
 
-// Traversal kernel of a batch under MRG_TRAVERSAL_AUTO: the refill driver (with parked leaves) when
-// the batch holds a scene of at least 2^17 triangles, the fused launch otherwise. Measured
-// refill/fused throughput (768 x 768 x 64 spp per scene, and the 1080p terrain job): 0.85-0.88 at
-// 36-5k triangles, 0.99-1.00 at 21k-82k, 1.02-1.08 at 141k-408k, 1.10 at 2.2M. A refill pays for
-// every ray it starts and gains where ray lengths vary most within a wave, which is in large scenes.
-constexpr uint32_t kRefillMinTriangles = 1u << 17;
+// Traversal kernel of a batch under MRG_TRAVERSAL_AUTO: the refill driver (parked leaves, six waves
+// per SIMD) when the batch holds a scene of at least 2^14 triangles, the fused launch otherwise.
+// Measured refill/fused throughput (768 x 768 x 64 spp per scene, and the 1080p terrain job):
+// 0.85-0.90 at 36-5k triangles, 1.03-1.10 at 21k-82k, 1.09-1.21 at 141k-408k, 1.31 at 2.2M. A refill
+// pays for every ray it starts and gains where ray lengths vary most within a wave, which is in
+// larger scenes.
+constexpr uint32_t kRefillMinTriangles = 1u << 14;
 
 bool refill_traversal(const mrg_device* d, const mrg_scene* s) {
     if (d->opt.traversal == MRG_TRAVERSAL_REFILL) return true;
@@ -1004,8 +1006,13 @@ int mrg_device_open(int ordinal, const mrg_device_options* options, mrg_device**
         check_hip(hipStreamCreateWithFlags(&d->copy_stream, hipStreamNonBlocking), "hipStreamCreate");
         check_hip(hipStreamCreateWithFlags(&d->query_stream, hipStreamNonBlocking), "hipStreamCreate");
         const uint32_t cus = uint32_t(std::max(1, d->props.multiProcessorCount));
-        const uint32_t per = d->opt.persistent_blocks_per_cu ? d->opt.persistent_blocks_per_cu : 4u;
-        d->shape           = LaunchShape{cus, cus * per, cus * 8};
+        // Resident 256-thread blocks per CU of the persistent grids, unless the options fix one count
+        // for both: 4 for the fused / split traversal kernels (at most 5 waves per SIMD by their
+        // registers; 4 measured best), 6 for the refill-driver kernels (80 VGPRs and 26 KB of LDS:
+        // six blocks fill the CU; 6 vs 4 measured +8% on the 1080p terrain job).
+        const uint32_t per    = d->opt.persistent_blocks_per_cu ? d->opt.persistent_blocks_per_cu : 4u;
+        const uint32_t per_rf = d->opt.persistent_blocks_per_cu ? d->opt.persistent_blocks_per_cu : 6u;
+        d->shape              = LaunchShape{cus, cus * per, cus * 8, cus * per_rf};
         // Automatic pool: 2^26 paths (about 12 GB of path state) on a large-HBM device. Measured on the
         // 1080p terrain scene, 2^23 -> 2^26 paths per batch raised samples/s by 30% (fewer, fuller
         // bounce launches); 2^27 gave no more.

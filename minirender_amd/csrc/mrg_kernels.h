@@ -66,8 +66,9 @@ struct Pool {
 
 struct LaunchShape {
     uint32_t compute_units;
-    uint32_t traversal_blocks;  // persistent grid of the traversal kernels (256-thread blocks)
+    uint32_t traversal_blocks;  // persistent grid of k_trace, k_extend and k_shadow (256-thread blocks)
     uint32_t shade_blocks;
+    uint32_t refill_blocks;     // persistent grid of the refill-driver kernels (k_trace_refill, k_query)
 };
 
 hipError_t launch_query(const GpuScene& sc, const float* rays, uint32_t n, float tmin, float tmax, mrg_hit_raw* hits, float* bary,

@@ -287,8 +287,9 @@ __device__ __forceinline__ float refine_distance(const GpuScene& sc, uint32_t sl
 
 // ---------------------------------------------------------------------------------------------
 // Intersection microbenchmark / SceneBvh::intersect and ::occluded over a packed ray array.
+// Six waves per SIMD, as k_trace_refill.
 template <bool ANY, bool STATS>
-__global__ void __launch_bounds__(256) k_query(GpuScene sc, const float* __restrict__ rays, uint32_t n, float tmin, float tmax,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k_query(GpuScene sc, const float* __restrict__ rays, uint32_t n, float tmin, float tmax,
                                                mrg_hit_raw* __restrict__ hits, float* __restrict__ bary,
                                                uint8_t* __restrict__ occluded, uint32_t* __restrict__ work, Pool P,
                                                uint64_t* __restrict__ stats) {
@@ -478,8 +479,9 @@ __global__ void __launch_bounds__(256) k_trace(Pool P, const GpuJob* __restrict_
 // bounce iter - 1 first, then the extension rays of bounce iter), but lanes that finish their ray
 // take the next queue entry at once, so a wave can trace shadow and extension rays side by side.
 // ONE_SCENE: every job of the batch renders the same scene (jobs[0]'s), read once per wave.
+// Six waves per SIMD (at most 80 VGPRs, 26 KB of LDS stack per block): six resident blocks per CU.
 template <bool ONE_SCENE>
-__global__ void __launch_bounds__(256) k_trace_refill(Pool P, const GpuJob* __restrict__ jobs, uint32_t iter) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k_trace_refill(Pool P, const GpuJob* __restrict__ jobs, uint32_t iter) {
     __shared__ uint32_t s_ref[kStackLds * kTraversalBlock];
     __shared__ float    s_t[kStackLds * kTraversalBlock];
     const StackView stk   = make_stack(s_ref, s_t, P);
@@ -855,7 +857,7 @@ static uint32_t grid_for(const LaunchShape& s, uint32_t per_cu) { return s.compu
 hipError_t launch_query(const GpuScene& sc, const float* rays, uint32_t n, float tmin, float tmax, mrg_hit_raw* hits, float* bary,
                         uint8_t* occluded, uint32_t* work, const Pool& P, uint64_t* stats, bool any, bool with_stats,
                         const LaunchShape& shape, hipStream_t stream) {
-    const dim3 grid(shape.traversal_blocks), block(256);
+    const dim3 grid(shape.refill_blocks), block(256);
     if (any) {
         if (with_stats) hipLaunchKernelGGL((k_query<true, true>), grid, block, 0, stream, sc, rays, n, tmin, tmax, hits, bary, occluded, work, P, stats);
         else hipLaunchKernelGGL((k_query<true, false>), grid, block, 0, stream, sc, rays, n, tmin, tmax, hits, bary, occluded, work, P, stats);
@@ -895,8 +897,8 @@ hipError_t launch_trace(const Pool& P, const GpuJob* jobs, uint32_t iter, const 
 
 hipError_t launch_trace_refill(const Pool& P, const GpuJob* jobs, uint32_t iter, bool one_scene, const LaunchShape& shape,
                                hipStream_t stream) {
-    if (one_scene) hipLaunchKernelGGL(k_trace_refill<true>, dim3(shape.traversal_blocks), dim3(256), 0, stream, P, jobs, iter);
-    else hipLaunchKernelGGL(k_trace_refill<false>, dim3(shape.traversal_blocks), dim3(256), 0, stream, P, jobs, iter);
+    if (one_scene) hipLaunchKernelGGL(k_trace_refill<true>, dim3(shape.refill_blocks), dim3(256), 0, stream, P, jobs, iter);
+    else hipLaunchKernelGGL(k_trace_refill<false>, dim3(shape.refill_blocks), dim3(256), 0, stream, P, jobs, iter);
     return hipGetLastError();
 }
 
