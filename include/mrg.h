@@ -192,8 +192,8 @@ typedef struct mrg_device_options {
     uint32_t use_graph;          /* capture the bounce loop in a hipGraph */
     uint32_t block_size;         /* kernel block size (64/128/256); 0 = 256 */
     uint32_t persistent_blocks_per_cu; /* 0 = automatic */
-    uint32_t traversal;          /* MRG_TRAVERSAL_*; AUTO = refill for a batch with a scene of >= 2^14
-                                    triangles, fused otherwise */
+    uint32_t traversal;          /* MRG_TRAVERSAL_*; AUTO = refill for a batch of >= 2^23 paths with a
+                                    scene of >= 2^14 triangles, fused otherwise */
 } mrg_device_options;
 
 /* Hit of the intersection entry points: closest t and scene-order primitive index (-1 = miss). */

@@ -8,8 +8,8 @@ The knobs are the launch and queue choices of the wavefront integrator:
 * ``traversal``                 ``split`` (closest-hit and any-hit kernels per bounce), ``fused``
                                 (one traversal launch per bounce for both kinds of ray), ``refill``
                                 (the fused launch where lanes that finish take the next ray at once)
-                                or ``auto`` (refill for batches holding a scene of >= 2^14
-                                triangles, fused otherwise),
+                                or ``auto`` (refill for batches of >= 2^23 paths holding a
+                                scene of >= 2^14 triangles, fused otherwise),
 * ``use_graph``                 the bounce loop captured once in a hipGraph and replayed,
 * ``max_jobs_in_flight``        (batched schedule) jobs co-scheduled in one batch.
 

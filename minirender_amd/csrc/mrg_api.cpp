@@ -602,14 +602,17 @@ struct BatchPlan {
 };
 
 // Traversal kernel of a batch under MRG_TRAVERSAL_AUTO: the refill driver (parked leaves, six waves
-// per SIMD) when the batch holds a scene of at least 2^14 triangles, the fused launch otherwise.
-// Measured refill/fused throughput (768 x 768 x 64 spp per scene, and the 1080p terrain job):
-// 0.85-0.90 at 36-5k triangles, 1.03-1.10 at 21k-82k, 1.09-1.21 at 141k-408k, 1.31 at 2.2M. A refill
-// pays for every ray it starts and gains where ray lengths vary most within a wave, which is in
-// larger scenes.
+// per SIMD) when the batch holds a scene of at least 2^14 triangles and at least 2^23 paths, the
+// fused launch otherwise. Measured refill/fused throughput: by scene (768 x 768 x 64 spp batches,
+// and the 1080p terrain job) 0.85-0.90 at 36-5k triangles, 1.03-1.10 at 21k-82k, 1.09-1.21 at
+// 141k-408k, 1.31 at 2.2M; by batch (16-spp passes, 21k-408k triangles) 0.46-0.67 at 128 x 128,
+// 0.60-0.75 at 256 x 256, 0.79-0.91 at 512 x 512 (4.2M paths), 1.03-1.13 at 1024 x 1024 (16.8M).
+// A refill pays for every ray it starts and per step of its finer loop; it gains where ray lengths
+// vary most within a wave, in larger scenes, and where a launch has rays to refill with.
 constexpr uint32_t kRefillMinTriangles = 1u << 14;
+constexpr uint32_t kRefillMinPaths     = 1u << 23;
 
-bool refill_traversal(const mrg_device* d, const mrg_scene* s) {
+bool refill_scene(const mrg_device* d, const mrg_scene* s) {
     if (d->opt.traversal == MRG_TRAVERSAL_REFILL) return true;
     return d->opt.traversal == MRG_TRAVERSAL_AUTO && s->gpu.tri_count >= kRefillMinTriangles;
 }
@@ -702,7 +705,7 @@ BatchPlan plan_batch_locked(mrg_device* d) {
         b.jobs.push_back(j);
         b.gjobs.push_back(gpu_job(j));
         b.extended = b.extended || j->scene->extended;
-        b.refill   = b.refill || refill_traversal(d, j->scene);
+        b.refill   = b.refill || refill_scene(d, j->scene);
         b.one_scene = b.one_scene && j->scene == b.jobs.front()->scene;
         b.max_depth = std::max(b.max_depth, j->desc.maximum_depth);
     }
@@ -748,6 +751,7 @@ BatchPlan plan_batch_locked(mrg_device* d) {
             px += c.pixel_count;
         }
     }
+    if (d->opt.traversal == MRG_TRAVERSAL_AUTO && b.slots < kRefillMinPaths) b.refill = false;
     for (mrg_job* j : b.jobs) {
         j->in_flight = true;
         j->refs++;

@@ -184,8 +184,10 @@ __device__ __forceinline__ StackView make_stack(uint32_t* lds_ref, float* lds_t,
 
 // Persistent traversal with dynamic refill: every lane keeps its own ray; whenever at least
 // REFILL_LANES lanes of the wave are idle, they take the next entries of the wave's claimed range
-// (one atomic per kRefillClaim entries) and start new rays, so a wave does not wait for its slowest
-// ray. `start(i, st, any)` initialises item i, says whether it is an any-hit query (lanes of one
+// and start new rays, so a wave does not wait for its slowest ray. A claim is one atomic for
+// kRefillClaim entries, or for 64 when the launch has fewer than two items per lane of the grid:
+// a small launch then spreads one item per lane over as many waves as it can fill, instead of two
+// per lane over half as many. `start(i, st, any)` initialises item i, says whether it is an any-hit query (lanes of one
 // wave may trace both kinds) and returns its scene; `finish(i, st)` consumes its result at the
 // lane's next refill (or when the wave drains), so the lanes that finished since the last refill run
 // it together. UNIFORM: every item traces `uscene` (kept in scalar registers, so node and triangle
@@ -212,6 +214,7 @@ __device__ __forceinline__ void persistent_trace_mixed(const StackView& stk, uin
     bool       drained = false;  // wave-uniform: the queue has no entries left
     uint32_t   item    = 0;
     uint32_t   cur = 0, end = 0;  // wave-uniform: the unused part of the wave's claimed range
+    const uint32_t claim_size = n >= 2u * gridDim.x * blockDim.x ? kRefillClaim : 64u;
     for (;;) {
         const uint64_t idle = __ballot(!active);
         if (!drained && __builtin_popcountll(idle) >= REFILL_LANES) {
@@ -221,20 +224,20 @@ __device__ __forceinline__ void persistent_trace_mixed(const StackView& stk, uin
                 finish(item, st);
                 pending = false;
             }
-            // idle lanes take the next entries of the wave's range; a new range of kRefillClaim
+            // idle lanes take the next entries of the wave's range; a new range of claim_size
             // entries is claimed (one atomic) when the current one runs short
             const uint32_t need  = static_cast<uint32_t>(__builtin_popcountll(idle));
             const uint32_t avail = end - cur;
             uint32_t       claim = 0;
             if (avail < need) {
-                if (lane_id() == 0) claim = atomicAdd(work, kRefillClaim);
+                if (lane_id() == 0) claim = atomicAdd(work, claim_size);
                 claim = __shfl(claim, 0, 64);
             }
             const uint32_t j    = prefix_in_mask(idle);
             const uint32_t slot = j < avail ? cur + j : claim + (j - avail);
             if (avail < need) {
                 cur = claim + (need - avail);
-                end = claim + kRefillClaim;
+                end = claim + claim_size;
             } else {
                 cur += need;
             }
