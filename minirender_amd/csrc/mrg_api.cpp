@@ -764,7 +764,7 @@ bool fused_traversal(const mrg_device* d) { return d->opt.traversal != MRG_TRAVE
 void launch_iteration_kernels(mrg_device* d, uint32_t iters, bool extended, bool refill, bool one_scene, bool timed,
                               std::vector<std::pair<int, int>>& marks) {
     auto trace = [&](const Pool& P, const GpuJob* jobs, uint32_t it, const LaunchShape& shape, hipStream_t stream) {
-        return refill ? launch_trace_refill(P, jobs, it, one_scene, shape, stream) : launch_trace(P, jobs, it, shape, stream);
+        return refill ? launch_trace_refill(P, jobs, it, one_scene, shape, stream) : launch_trace(P, jobs, it, one_scene, shape, stream);
     };
     const GpuJob*   jobs   = d->jobs_buf.as<GpuJob>();
     const GpuChunk* chunks = d->chunks_buf.as<GpuChunk>();

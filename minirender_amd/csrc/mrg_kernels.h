@@ -82,7 +82,8 @@ hipError_t launch_extend(const Pool& P, const GpuJob* jobs, uint32_t iter, const
 hipError_t launch_shade(const Pool& P, const GpuJob* jobs, uint32_t iter, bool extended, const LaunchShape& shape, hipStream_t stream);
 hipError_t launch_shadow(const Pool& P, const GpuJob* jobs, uint32_t iter, const LaunchShape& shape, hipStream_t stream);
 // Fused traversal: shadow rays of bounce iter - 1 and extension rays of bounce iter in one launch.
-hipError_t launch_trace(const Pool& P, const GpuJob* jobs, uint32_t iter, const LaunchShape& shape, hipStream_t stream);
+hipError_t launch_trace(const Pool& P, const GpuJob* jobs, uint32_t iter, bool one_scene, const LaunchShape& shape,
+                        hipStream_t stream);
 // The same with the refill driver (finished lanes take the next queue entry at once).
 // one_scene: every job of the batch renders jobs[0]'s scene (the kernel reads it once per wave).
 hipError_t launch_trace_refill(const Pool& P, const GpuJob* jobs, uint32_t iter, bool one_scene, const LaunchShape& shape,

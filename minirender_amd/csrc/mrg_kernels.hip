@@ -403,14 +403,16 @@ __global__ void __launch_bounds__(256) k_generate(Pool P, const GpuJob* __restri
     }
 }
 
-// One extension ray: closest hit of path queue[i].
+// One extension ray: closest hit of path queue[i]. ONE_SCENE: every job of the batch renders
+// jobs[0]'s scene, read through a uniform index (scalar loads) without the path's job lookup.
+template <bool ONE_SCENE>
 __device__ __forceinline__ void extend_one(const Pool& P, const GpuJob* __restrict__ jobs, const uint32_t* queue, uint32_t i,
                                            const StackView& stk) {
     uint64_t       dummy = 0;
     const uint32_t path  = queue[i];
     const float4   ro    = P.ray_o[path];
     const float4   rd    = P.ray_d[path];
-    const uint32_t job   = __float_as_uint(P.prev[path].w) >> 16;
+    const uint32_t job   = ONE_SCENE ? 0u : __float_as_uint(P.prev[path].w) >> 16;
     const RayPre   rp    = ray_setup(mk(ro.x, ro.y, ro.z), mk(rd.x, rd.y, rd.z));
     const TraceResult res = trace_ray<false, false>(jobs[job].scene, rp, ro.w, rd.w, stk, &dummy, &dummy);
     P.hit[path]      = make_float4(res.t, res.b0, res.b1, res.b2);
@@ -418,12 +420,13 @@ __device__ __forceinline__ void extend_one(const Pool& P, const GpuJob* __restri
 }
 
 // One shadow ray (entry i of the shadow queue): any hit; unoccluded rays add their contribution.
+template <bool ONE_SCENE>
 __device__ __forceinline__ void shadow_one(const Pool& P, const GpuJob* __restrict__ jobs, uint32_t i, const StackView& stk) {
     uint64_t       dummy = 0;
     const uint32_t path  = P.q_sh[i];
     const float4   o     = P.sh_o[i];
     const float4   d     = P.sh_d[i];
-    const uint32_t job   = __float_as_uint(P.prev[path].w) >> 16;
+    const uint32_t job   = ONE_SCENE ? 0u : __float_as_uint(P.prev[path].w) >> 16;
     const RayPre   rp    = ray_setup(mk(o.x, o.y, o.z), mk(d.x, d.y, d.z));
     const TraceResult res = trace_ray<true, false>(jobs[job].scene, rp, 0.0f, o.w, stk, &dummy, &dummy);
     if (res.slot == 0xffffffffu) {
@@ -447,7 +450,7 @@ __global__ void __launch_bounds__(256) k_extend(Pool P, const GpuJob* __restrict
     WaveWork<kExtendClaim> w;
     for (w.begin(); w.base < n; w.next(work, n)) {
         const uint32_t i = w.base + lane_id();
-        if (i < n) extend_one(P, jobs, queue, i, stk);
+        if (i < n) extend_one<false>(P, jobs, queue, i, stk);
     }
 }
 
@@ -455,7 +458,8 @@ __global__ void __launch_bounds__(256) k_extend(Pool P, const GpuJob* __restrict
 // bounce iter (both produced by the previous shading pass) share one persistent launch. The two
 // ranges are padded to whole 64-item chunks, so every wave traces one kind of ray at a time.
 // Radiance accumulation keeps its order: a path's shadow contribution of bounce iter - 1 lands
-// before the shading of bounce iter reads it.
+// before the shading of bounce iter reads it. ONE_SCENE: the batch renders one scene (extend_one).
+template <bool ONE_SCENE>
 __global__ void __launch_bounds__(256) k_trace(Pool P, const GpuJob* __restrict__ jobs, uint32_t iter) {
     __shared__ uint32_t s_ref[kStackLds * kTraversalBlock];
     __shared__ float    s_t[kStackLds * kTraversalBlock];
@@ -470,10 +474,10 @@ __global__ void __launch_bounds__(256) k_trace(Pool P, const GpuJob* __restrict_
     for (w.begin(); w.base < n; w.next(work, n)) {
         if (w.base < sh_end) {
             const uint32_t i = w.base + lane_id();
-            if (i < n_sh) shadow_one(P, jobs, i, stk);
+            if (i < n_sh) shadow_one<ONE_SCENE>(P, jobs, i, stk);
         } else {
             const uint32_t i = w.base - sh_end + lane_id();
-            if (i < n_ext) extend_one(P, jobs, queue, i, stk);
+            if (i < n_ext) extend_one<ONE_SCENE>(P, jobs, queue, i, stk);
         }
     }
 }
@@ -790,7 +794,7 @@ __global__ void __launch_bounds__(256) k_shadow(Pool P, const GpuJob* __restrict
     WaveWork<kShadowClaim> w;
     for (w.begin(); w.base < n; w.next(work, n)) {
         const uint32_t i = w.base + lane_id();
-        if (i < n) shadow_one(P, jobs, i, stk);
+        if (i < n) shadow_one<false>(P, jobs, i, stk);
     }
 }
 
@@ -893,8 +897,10 @@ hipError_t launch_shade(const Pool& P, const GpuJob* jobs, uint32_t iter, bool e
     return hipGetLastError();
 }
 
-hipError_t launch_trace(const Pool& P, const GpuJob* jobs, uint32_t iter, const LaunchShape& shape, hipStream_t stream) {
-    hipLaunchKernelGGL(k_trace, dim3(shape.traversal_blocks), dim3(256), 0, stream, P, jobs, iter);
+hipError_t launch_trace(const Pool& P, const GpuJob* jobs, uint32_t iter, bool one_scene, const LaunchShape& shape,
+                        hipStream_t stream) {
+    if (one_scene) hipLaunchKernelGGL(k_trace<true>, dim3(shape.traversal_blocks), dim3(256), 0, stream, P, jobs, iter);
+    else hipLaunchKernelGGL(k_trace<false>, dim3(shape.traversal_blocks), dim3(256), 0, stream, P, jobs, iter);
     return hipGetLastError();
 }
 
