@@ -76,6 +76,25 @@ def test_traversal_kernels_agree_with_fp32_twin(oracle, traversal):
     assert prog.statistics["extension_rays"] == st["extension_rays"]
 
 
+def test_default_refill_batch_matches_fp32_twin(oracle):
+    """The library's default for a large batch: AUTO traversal picks the refill launch (a scene of at
+    least 2^14 triangles in one batch of 1024 x 1024 x 8 = 2^23 paths), at the library's block counts
+    and with the one-scene kernel. A band of rows against the FP32 twin within 1e-4, and the kernel
+    times confirm that every traversal launch was a refill launch."""
+    scene = scenegen.mixed_terrain(cells=128, spheres=8, sphere_level=2, seed=5)
+    w, h, spp, depth, seed, rows = 1024, 1024, 8, 10, 17, (480, 496)
+    assert scene.triangle_count >= 1 << 14
+    print(f"[render parity] default options, mixed {w}x{h} spp={spp} depth={depth}, rows {rows}", flush=True)
+    with api.Device(0) as dev:
+        dev.kernel_times(reset=True)
+        img, prog = api.render(dev, scene, api.RenderSettings(w, h, spp, seed), api.IntegratorSettings("mis", depth, True, 3))
+        kt = dev.kernel_times()
+    assert kt["batches"] == 1 and kt["extend_launches"] > 0 and kt["refill_launches"] == kt["extend_launches"]
+    twin, _ = oracle.OracleScene(scene).render(scene.camera, w, h, spp, seed=seed, mode="mis", max_depth=depth, rr=True,
+                                               precision=32, crop=(0, rows[0], w, rows[1]))
+    _check(img[rows[0]:rows[1]], twin)
+
+
 @pytest.mark.parametrize("mode", ["mis", "bsdf", "light"])
 def test_energy_compensated_conductors_match_fp32_twin(device, oracle, mode):
     """Kulla-Conty compensated rough conductors (bsdf.cpp:128-189): mixed lobe sampling, mixed pdf

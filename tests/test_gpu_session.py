@@ -158,18 +158,21 @@ def test_results_identical_alone_and_in_the_mix():
     specs = [(0, 96, 64, 24, 8), (1, 128, 72, 16, 16), (2, 64, 64, 40, 10), (1, 33, 17, 7, 12), (0, 200, 150, 5, 8),
              (3, 48, 40, 12, 6), (4, 56, 40, 9, 7), (5, 50, 30, 11, 6)]
     films = {}
-    for mode in ("plain", "batched"):
-        print(f"[mix] {mode}", flush=True)
-        with api.Device(0, schedule=mode, path_capacity=1 << 17, use_graph=(mode == "batched")) as d:
+    # the third run traces the co-scheduled batches with the refill driver, whose lanes then hold
+    # rays of different scenes side by side
+    for mode, traversal in (("plain", "auto"), ("batched", "auto"), ("batched", "refill")):
+        print(f"[mix] {mode} traversal={traversal}", flush=True)
+        with api.Device(0, schedule=mode, path_capacity=1 << 17, use_graph=(mode == "batched"), traversal=traversal) as d:
             bvhs = [d.upload(s) for s in scenes]
             jobs = [api.Job(bvhs[k], scenes[k].camera, api.RenderSettings(w, h, n, 7 + i), api.IntegratorSettings("mis", dep),
                             pass_samples=4) for i, (k, w, h, n, dep) in enumerate(specs)]
             for j in jobs:
                 j.wait()
                 assert j.progress().state == "completed"
-            films[mode] = [j.film() for j in jobs]
+            films[mode, traversal] = [j.film() for j in jobs]
             for j in jobs:
                 j.release()
-    for (s0, c0), (s1, c1) in zip(films["plain"], films["batched"]):
-        np.testing.assert_array_equal(c0, c1)
-        np.testing.assert_array_equal(s0, s1)
+    for other in (("batched", "auto"), ("batched", "refill")):
+        for (s0, c0), (s1, c1) in zip(films["plain", "auto"], films[other]):
+            np.testing.assert_array_equal(c0, c1)
+            np.testing.assert_array_equal(s0, s1)
