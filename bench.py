@@ -323,7 +323,7 @@ def bench_batch(args):
     scenes = serve.scene_pool(seed=4)
     jobs = serve.batch_jobs(seed=4)
     calib = serve.batch_jobs(seed=4, spp_divisor=8)
-    knobs = {"path_capacity": [1 << 23, 1 << 24, 1 << 25, 1 << 26, 1 << 27], "persistent_blocks_per_cu": [0, 4, 6],
+    knobs = {"path_capacity": [1 << 23, 1 << 24, 1 << 25, 1 << 26, 1 << 27, 1 << 28], "persistent_blocks_per_cu": [0, 4, 6],
              "traversal": ["split", "fused", "refill", "auto"], "use_graph": [False, True], "max_jobs_in_flight": [16, 64]}
     tuned = {}
     for mode in ("plain", "plain_fill", "batched"):

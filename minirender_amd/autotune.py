@@ -27,7 +27,7 @@ import time
 from . import api
 
 KNOBS = {
-    "path_capacity": [1 << 22, 1 << 23, 1 << 24, 1 << 25, 1 << 26, 1 << 27],
+    "path_capacity": [1 << 22, 1 << 23, 1 << 24, 1 << 25, 1 << 26, 1 << 27, 1 << 28],
     "persistent_blocks_per_cu": [0, 2, 3, 4, 5, 6],
     "traversal": ["split", "fused", "refill", "auto"],
     "use_graph": [False, True],

@@ -1017,11 +1017,18 @@ int mrg_device_open(int ordinal, const mrg_device_options* options, mrg_device**
         const uint32_t per    = d->opt.persistent_blocks_per_cu ? d->opt.persistent_blocks_per_cu : 4u;
         const uint32_t per_rf = d->opt.persistent_blocks_per_cu ? d->opt.persistent_blocks_per_cu : 6u;
         d->shape              = LaunchShape{cus, cus * per, cus * 8, cus * per_rf};
-        // Automatic pool: 2^26 paths (about 12 GB of path state) on a large-HBM device. Measured on the
-        // 1080p terrain scene, 2^23 -> 2^26 paths per batch raised samples/s by 30% (fewer, fuller
-        // bounce launches); 2^27 gave no more.
-        if (d->opt.path_capacity == 0)
-            d->opt.path_capacity = d->props.totalGlobalMem >= (uint64_t(64) << 30) ? uint64_t(1) << 26 : uint64_t(1) << 23;
+        // Automatic pool: 2^28 paths (47 GB of path state, within the default 64 GiB budget) on a
+        // device of at least 192 GiB, 2^26 (12 GB) from 64 GiB, else 2^23. Fewer, fuller batches pay
+        // each batch's latency-bound tail of late bounces (a few ms, whatever the batch size) less
+        // often. Measured on the 1080p terrain job (530.8M samples): 2^23 -> 2^26 paths +30%
+        // (per-ray traversal); with the refill traversal 2^26: 2,303, 2^27: 2,486, 2^28: 2,602
+        // Msamples/s. Latency-oriented callers (progressive serving) pass a smaller pool.
+        if (d->opt.path_capacity == 0) {
+            const uint64_t mem = d->props.totalGlobalMem;
+            d->opt.path_capacity = mem >= (uint64_t(192) << 30)  ? uint64_t(1) << 28
+                                   : mem >= (uint64_t(64) << 30) ? uint64_t(1) << 26
+                                                                 : uint64_t(1) << 23;
+        }
         if (d->opt.path_capacity > (uint64_t(1) << 30)) throw std::invalid_argument("path capacity too large");
         if (d->opt.traversal > MRG_TRAVERSAL_REFILL) throw std::invalid_argument("unknown traversal kernel mode");
         if (d->opt.max_jobs_in_flight == 0) d->opt.max_jobs_in_flight = 64;

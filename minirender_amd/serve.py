@@ -120,15 +120,18 @@ def run_stream(device: api.Device, bvhs, scenes, requests, pass_samples=16, poll
     return summary, records, films
 
 
-def compare(requests=None, scenes=None, hbm_budget=64 << 30, max_jobs=64, use_graph=True, keep_films=False):
-    """Replays one request stream on the plain and on the batched schedule."""
+def compare(requests=None, scenes=None, hbm_budget=64 << 30, max_jobs=64, use_graph=True, keep_films=False,
+            path_capacity=1 << 26):
+    """Replays one request stream on the plain and on the batched schedule. The serving pool is 2^26
+    paths (12 GB): a batch is the unit of progress and pause latency, and smaller batches keep the
+    time to first pass of newly arrived jobs short."""
     scenes = scenes or scene_pool()
     requests = requests or request_stream()
     out = {}
     films = {}
     for mode in ("plain", "plain_fill", "batched"):
         dev = api.Device(0, schedule=mode, hbm_budget_bytes=hbm_budget, max_jobs_in_flight=max_jobs,
-                         use_graph=use_graph)
+                         use_graph=use_graph, path_capacity=path_capacity)
         bvhs = [dev.upload(s) for s in scenes]
         summary, records, f = run_stream(dev, bvhs, scenes, requests, keep_films=keep_films)
         out[mode] = summary
