@@ -1,11 +1,14 @@
 // HIP kernels for gfx950: ray queries and the wavefront path integrator.
 //
 // One batch renders a list of chunks (job, pixel range, sample range) for one or many jobs:
-//   reset -> generate -> { extend -> shade -> shadow } x (max depth + 1) -> accumulate
+//   reset -> generate -> { trace -> shade } x (max depth + 1) -> trace -> validate -> accumulate
+// where `trace` of bounce i traces the shadow rays of bounce i - 1 and the extension rays of bounce
+// i in one persistent launch (k_trace: a per-ray loop; k_trace_refill: the step driver with lane
+// refill and parked leaves), or, in split mode, { extend -> shade -> shadow } per bounce.
 // Queues: the extension queue (path ids, ping-pong by bounce) and the shadow queue (path id, shadow
-// ray, contribution) are compacted in `shade` with a wave ballot + mbcnt prefix and one atomic per
-// wave. All bounce kernels are persistent (fixed grid, counts read from device memory), so the
-// bounce loop can be captured once in a hipGraph.
+// ray, contribution) are compacted in `shade` block-wide, grouped by direction octant, with one
+// atomic per queue per block. All bounce kernels are persistent (fixed grid, counts read from device
+// memory), so the bounce loop can be captured once in a hipGraph.
 //
 // Reference: src/render/path_tracer.cpp:153-386 (trace, sample_pixel), scene.cpp:171-199 (camera),
 // render_session.cpp:39-74 (film commit order: per pixel, samples in index order, FP64 sums).

@@ -2,6 +2,9 @@
 // stack of (child reference, entry distance) pairs in LDS (kStackLds per lane) that spills to a
 // per-lane global region for deeper entries (the builder bounds the worst-case stack by kStackTotal).
 // Children of a node are visited nearest first (a five-exchange sorting network on entry distance).
+// Two drivers: trace_ray runs one ray to its end per call (per-ray loop); trace_step_dyn advances
+// a ray by one step (one node, or a wave-wide triangle step over parked leaves) for drivers that
+// refill lanes as their rays finish.
 //
 // Box test: conservative slab test (each slab bound computed as (bound - origin) * (1/d), then the
 // entry/exit distances widened by 2^-20 relative), so a box is never culled when the watertight
