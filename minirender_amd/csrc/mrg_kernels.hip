@@ -402,17 +402,23 @@ __global__ void __launch_bounds__(256) k_generate(Pool P, const GpuJob* __restri
         P.prev[slot]  = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(kPrevDelta | (ch.job << 16)));
         P.rng[slot]   = make_uint4(static_cast<uint32_t>(rng.state), static_cast<uint32_t>(rng.state >> 32),
                                    static_cast<uint32_t>(rng.inc), static_cast<uint32_t>(rng.inc >> 32));
-        P.q_ext0[slot] = slot;
     }
 }
 
-// One extension ray: closest hit of path queue[i]. ONE_SCENE: every job of the batch renders
+// Extension queue of bounce `iter` (ping-pong between two arrays); null at bounce 0, where every
+// path slot of the batch continues in slot order (k_generate writes no queue).
+__device__ __forceinline__ const uint32_t* ext_queue(const Pool& P, uint32_t iter) {
+    return iter == 0 ? nullptr : (iter & 1u) ? P.q_ext1 : P.q_ext0;
+}
+
+// One extension ray: closest hit of path queue[i] (path i at bounce 0, whose queue is the identity:
+// a null queue). ONE_SCENE: every job of the batch renders
 // jobs[0]'s scene, read through a uniform index (scalar loads) without the path's job lookup.
 template <bool ONE_SCENE>
 __device__ __forceinline__ void extend_one(const Pool& P, const GpuJob* __restrict__ jobs, const uint32_t* queue, uint32_t i,
                                            const StackView& stk) {
     uint64_t       dummy = 0;
-    const uint32_t path  = queue[i];
+    const uint32_t path  = queue ? queue[i] : i;
     const float4   ro    = P.ray_o[path];
     const float4   rd    = P.ray_d[path];
     const uint32_t job   = ONE_SCENE ? 0u : __float_as_uint(P.prev[path].w) >> 16;
@@ -448,7 +454,7 @@ __global__ void __launch_bounds__(256) k_extend(Pool P, const GpuJob* __restrict
     __shared__ float    s_t[kStackLds * kTraversalBlock];
     const StackView stk   = make_stack(s_ref, s_t, P);
     const uint32_t  n     = P.counters[kCtrExt + iter];
-    const uint32_t* queue = (iter & 1u) ? P.q_ext1 : P.q_ext0;
+    const uint32_t* queue = ext_queue(P, iter);
     uint32_t*       work  = P.counters + kCtrWorkExt + iter;
     WaveWork<kExtendClaim> w;
     for (w.begin(); w.base < n; w.next(work, n)) {
@@ -471,7 +477,7 @@ __global__ void __launch_bounds__(256) k_trace(Pool P, const GpuJob* __restrict_
     const uint32_t  n_sh   = iter ? P.counters[kCtrShadow + iter - 1] : 0u;
     const uint32_t  sh_end = (n_sh + 63u) & ~63u;  // shadow rays first: the longer, more divergent work
     const uint32_t  n      = sh_end + n_ext;
-    const uint32_t* queue  = (iter & 1u) ? P.q_ext1 : P.q_ext0;
+    const uint32_t* queue  = ext_queue(P, iter);
     uint32_t*       work   = P.counters + kCtrWorkExt + iter;
     WaveWork<kTraceClaim> w;
     for (w.begin(); w.base < n; w.next(work, n)) {
@@ -497,7 +503,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k
     const StackView stk   = make_stack(s_ref, s_t, P);
     const uint32_t  n_ext = P.counters[kCtrExt + iter];
     const uint32_t  n_sh  = iter ? P.counters[kCtrShadow + iter - 1] : 0u;
-    const uint32_t* queue = (iter & 1u) ? P.q_ext1 : P.q_ext0;
+    const uint32_t* queue = ext_queue(P, iter);
     uint32_t*       work  = P.counters + kCtrWorkExt + iter;
     uint64_t        nb = 0, nt = 0;
     uint32_t        path = 0;
@@ -512,7 +518,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k
                 const float4 d = P.sh_d[i];
                 trace_init(st, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), 0.0f, o.w);
             } else {
-                path            = queue[i - n_sh];
+                path            = queue ? queue[i - n_sh] : i - n_sh;
                 const float4 ro = P.ray_o[path];
                 const float4 rd = P.ray_d[path];
                 trace_init(st, mk(ro.x, ro.y, ro.z), mk(rd.x, rd.y, rd.z), ro.w, rd.w);
@@ -746,7 +752,7 @@ __device__ __forceinline__ void shade_path(const Pool& P, const GpuJob& job, uin
 template <bool EXT>
 __global__ void __launch_bounds__(kShadeThreads) __attribute__((amdgpu_waves_per_eu(4))) k_shade(Pool P, const GpuJob* __restrict__ jobs, uint32_t iter) {
     const uint32_t  n       = P.counters[kCtrExt + iter];
-    const uint32_t* queue   = (iter & 1u) ? P.q_ext1 : P.q_ext0;
+    const uint32_t* queue   = ext_queue(P, iter);
     uint32_t*       next_q  = (iter & 1u) ? P.q_ext0 : P.q_ext1;
     uint32_t*       next_n  = P.counters + kCtrExt + iter + 1;
     uint32_t*       sh_n    = P.counters + kCtrShadow + iter;
@@ -763,7 +769,7 @@ __global__ void __launch_bounds__(kShadeThreads) __attribute__((amdgpu_waves_per
         uint32_t       path     = 0;
         float4         so = make_float4(0, 0, 0, 0), sd = so, sc4 = so;
         if (i < n) {
-            path = queue[i];
+            path = queue ? queue[i] : i;
             const float4   pv  = P.prev[path];
             const uint32_t jid = __float_as_uint(pv.w) >> 16;
             const uint32_t j0  = __builtin_amdgcn_readfirstlane(jid);
