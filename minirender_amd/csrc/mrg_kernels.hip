@@ -190,10 +190,10 @@ __device__ __forceinline__ StackView make_stack(uint32_t* lds_ref, float* lds_t,
 // and start new rays, so a wave does not wait for its slowest ray. A claim is one atomic for
 // kRefillClaim entries, or for 64 when the launch has fewer than two items per lane of the grid:
 // a small launch then spreads one item per lane over as many waves as it can fill, instead of two
-// per lane over half as many. `start(i, st, any)` initialises item i, says whether it is an any-hit query (lanes of one
-// wave may trace both kinds) and returns its scene; `finish(i, st)` consumes its result at the
-// lane's next refill (or when the wave drains), so the lanes that finished since the last refill run
-// it together. UNIFORM: every item traces `uscene` (kept in scalar registers, so node and triangle
+// per lane over half as many. `start(i, st, any)` initialises item i, says whether it is an any-hit
+// query (lanes of one wave may trace both kinds) and returns its scene; `finish(i, st)` consumes its
+// result at the lane's next refill (or when the wave drains), so the lanes that finished since the
+// last refill run it together. UNIFORM: every item traces `uscene` (kept in scalar registers, so node and triangle
 // addresses are a scalar base plus a lane offset), and start's scene is ignored.
 // Thresholds, measured on the 16.7M-ray icosphere microbench and the 1080p terrain job: ray queries
 // start a ray from one packed record and refill at 16 idle lanes; bounce traversal gathers each ray
