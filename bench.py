@@ -49,15 +49,20 @@ def make_scene(name):
     raise SystemExit(f"unknown workload {name}")
 
 
-def read_traffic(kernel, workload):
-    """HBM bytes per launch measured by rocprofv3 --pmc (see profiles/), or None."""
-    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_traffic.txt")
+def read_profile_value(name, key):
+    """A `key = value` line of a profiles/ summary (figures measured by rocprofv3 or the cache probe), or None."""
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch"):
+        if line.startswith(key):
             return float(line.split("=")[1])
     return None
+
+
+def read_traffic(kernel, workload):
+    """HBM bytes per launch measured by rocprofv3 --pmc (see profiles/), or None."""
+    return read_profile_value(f"{kernel}_{workload}_traffic.txt", "traffic_bytes_per_launch")
 
 
 def cpu_baseline(scene, w, threads):
@@ -178,6 +183,14 @@ def bench_render(args):
                               "triangle_tests_per_shadow_ray": round(sh_tris, 2),
                               "avg_launch_ms": round(avg_s * 1e3, 4), "rays_per_launch": round(rays / launches, 1),
                               "algorithmic_bytes_per_launch": round(per_launch_bytes)}
+        # the same kernel against the L2: bytes through the L2 per launch (rocprofv3 TCC_REQ x 128 B) over the
+        # measured L2 read bandwidth (profiles/memory_hierarchy_probe.txt)
+        l2_bytes = read_profile_value(f"{kernel}_{args.workload}_traffic.txt", "l2_bytes_per_launch")
+        l2_peak = read_profile_value("memory_hierarchy_probe.txt", "l2_read_gb_per_s")
+        if l2_bytes and l2_peak and avg_s > 0:
+            l2_gbps = l2_bytes / avg_s / 1e9
+            result["roofline"].update({"l2_bytes_per_launch": round(l2_bytes), "l2_achieved": round(l2_gbps, 1),
+                                       "l2_peak_measured": l2_peak, "l2_frac": round(l2_gbps / l2_peak, 4)})
     result["cpu_baseline"] = cpu
     return result
 
@@ -364,12 +377,40 @@ def bench_copy(args):
             "bytes_read_per_dispatch": a.numel() * 4, "bytes_written_per_dispatch": a.numel() * 4}
 
 
+def bench_cache(args):
+    """Read bandwidth of each level of the memory hierarchy (libmrg_probe.so): a grid of 8 256-thread
+    blocks per CU streams a resident buffer whose size picks the level serving the reads (a few MB:
+    the per-XCD L2; tens of MB: the Infinity Cache; GBs: HBM). The roofline of the traversal kernel
+    is quoted against these measured figures as well as against the 8 TB/s HBM peak."""
+    import ctypes as C
+    from minirender_amd import build as B
+    lib = C.CDLL(B.PROBE_LIB)
+    lib.mrg_probe_read_bandwidth.restype = C.c_int
+    lib.mrg_probe_read_bandwidth.argtypes = [C.c_int, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32,
+                                             C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_uint64)]
+    per_cu, iters = 8, 1024  # 8.6 GB read per launch on 256 CUs
+    levels = []
+    for label, size in (("l2", 2 << 20), ("l2", 3 << 20), ("infinity_cache", 64 << 20), ("infinity_cache", 128 << 20),
+                        ("hbm", 4 << 30)):
+        gbps, ms, nbytes = C.c_double(), C.c_double(), C.c_uint64()
+        rc = lib.mrg_probe_read_bandwidth(0, size, per_cu, iters, max(args.steps, 3), C.byref(gbps), C.byref(ms),
+                                          C.byref(nbytes))
+        if rc != 0:
+            raise SystemExit(f"probe failed with HIP error {rc}")
+        log(f"[bench] cache probe {size >> 20} MiB: {gbps.value:.0f} GB/s ({ms.value:.3f} ms per launch)")
+        levels.append({"level": label, "bytes": size, "gb_per_s": round(gbps.value, 1), "ms_per_launch": round(ms.value, 4),
+                       "bytes_read_per_launch": nbytes.value})
+    return {"metric": "read bandwidth by memory level (GB/s)", "value": max(l["gb_per_s"] for l in levels), "unit": "GB/s",
+            "blocks_per_cu": per_cu, "threads_per_block": 256, "float4_loads_per_thread": iters, "levels": levels}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="mixed", choices=["mixed", "cornell", "intersect", "convergence", "copy", "batch"])
+    ap.add_argument("--workload", default="mixed", choices=["mixed", "cornell", "intersect", "convergence", "copy", "batch",
+                                                                 "cache"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--blocks-per-cu", type=int, default=0)
@@ -392,6 +433,8 @@ def main():
         res = bench_batch(args)
     elif args.workload == "copy":
         res = bench_copy(args)
+    elif args.workload == "cache":
+        res = bench_cache(args)
     elif args.workload == "convergence":
         res = bench_convergence(args)
     else:

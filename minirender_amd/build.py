@@ -65,6 +65,22 @@ def build_session_check(verbose=False):
     return SESSION_CHECK
 
 
+PROBE_LIB = os.path.join(OUT_DIR, "libmrg_probe.so")
+
+
+def build_probe(verbose=False):
+    """libmrg_probe.so: the cache-hierarchy read-bandwidth probe behind bench.py --workload cache
+    (measurement only; the render path does not use it)."""
+    src = os.path.join(CSRC, "probe", "mrg_probe.hip")
+    cmd = [HIPCC] + COMMON + DEVICE + ["-shared", "-o", PROBE_LIB, "-x", "hip", src]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"probe build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print("built", PROBE_LIB)
+    return PROBE_LIB
+
+
 RENDER_CLI = os.path.join(OUT_DIR, "mrg_render")
 
 
@@ -87,4 +103,5 @@ if __name__ == "__main__":
     build(verbose=True)
     build_session_check(verbose=True)
     build_cli(verbose=True)
+    build_probe(verbose=True)
     sys.exit(0)
