@@ -808,6 +808,10 @@ void launch_iteration_kernels(mrg_device* d, uint32_t iters, bool extended, bool
 void run_batch(mrg_device* d, BatchPlan& b) {
     const uint32_t iters = b.max_depth + 1;
     uint32_t       hdr[kHdrWords] = {uint32_t(b.chunks.size()), b.slots, b.pixels, iters};
+    for (GpuChunk& c : b.chunks) {
+        const uint32_t w = b.gjobs[c.job].width;
+        c.tile_rows      = c.pixel_begin % w == 0 && c.pixel_count % w == 0 ? c.pixel_count / w : 0u;
+    }
     check_hip(hipMemcpyAsync(d->jobs_buf.ptr, b.gjobs.data(), b.gjobs.size() * sizeof(GpuJob), hipMemcpyHostToDevice, d->stream), "upload jobs");
     check_hip(hipMemcpyAsync(d->chunks_buf.ptr, b.chunks.data(), b.chunks.size() * sizeof(GpuChunk), hipMemcpyHostToDevice, d->stream), "upload chunks");
     check_hip(hipMemcpyAsync(d->header.ptr, hdr, sizeof hdr, hipMemcpyHostToDevice, d->stream), "upload header");

@@ -342,6 +342,37 @@ __global__ void k_reset(Pool P) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < kStatWords; i += gridDim.x * blockDim.x) P.stats[i] = 0;
 }
 
+// Pixel (x, y) of position q of one sample of a chunk. A chunk of whole image rows (tile_rows != 0)
+// lays its pixels out in 8x8 tiles, tile rows top to bottom and tiles left to right (the last tile
+// row and column hold the remainder rows and columns), so the 64 paths of a wave start from one
+// compact patch of the image instead of a 64-pixel strip. Other chunks keep scanline order.
+// k_generate, k_validate and k_accumulate use the same map, so every sample lands on its own pixel.
+__device__ __forceinline__ void chunk_xy(const GpuChunk& ch, uint32_t width, uint32_t q, uint32_t& x, uint32_t& y) {
+    if (ch.tile_rows == 0u) {
+        const uint32_t pixel = ch.pixel_begin + q;
+        y = pixel / width;
+        x = pixel - y * width;
+        return;
+    }
+    const uint32_t band = q / (width * 8u);
+    const uint32_t y0   = band * 8u;
+    const uint32_t h    = min(8u, ch.tile_rows - y0);
+    const uint32_t in   = q - band * (width * 8u);
+    const uint32_t full = width & ~7u;
+    if (in < full * h) {
+        const uint32_t t = h == 8u ? in >> 6 : in / (8u * h);
+        const uint32_t w = in - t * (8u * h);
+        y = w >> 3;
+        x = t * 8u + (w & 7u);
+    } else {
+        const uint32_t rest = width - full;
+        const uint32_t w    = in - full * h;
+        y = w / rest;
+        x = full + (w - y * rest);
+    }
+    y += y0 + ch.pixel_begin / width;
+}
+
 __device__ __forceinline__ uint32_t find_chunk_by_slot(const GpuChunk* chunks, uint32_t count, uint32_t slot) {
     uint32_t lo = 0, hi = count;
     while (hi - lo > 1) {
@@ -370,10 +401,9 @@ __global__ void __launch_bounds__(256) k_generate(Pool P, const GpuJob* __restri
         const uint32_t  local  = slot - ch.slot_begin;
         const uint32_t  s      = local / ch.pixel_count;
         const uint32_t  p      = local - s * ch.pixel_count;
-        const uint32_t  pixel  = ch.pixel_begin + p;
         const GpuJob&   job    = jobs[ch.job];
-        const uint32_t  x      = pixel % job.width;
-        const uint32_t  y      = pixel / job.width;
+        uint32_t        x, y;
+        chunk_xy(ch, job.width, p, x, y);
         Pcg             rng    = for_sample(job.seed, static_cast<uint64_t>(y) * job.width + x, ch.sample_begin + s);
         const float     fx     = next_1d(rng);
         const float     fy     = next_1d(rng);
@@ -818,7 +848,9 @@ __global__ void __launch_bounds__(256) k_validate(Pool P, const GpuJob* __restri
         const GpuChunk ch = chunks[ci];
         const uint32_t p  = g - ch.pixel_prefix;
         const GpuJob&  job = jobs[ch.job];
-        const uint32_t pixel = ch.pixel_begin + p;
+        uint32_t       px, py;
+        chunk_xy(ch, job.width, p, px, py);
+        const uint32_t pixel = py * job.width + px;
         double sx = job.film_sums[static_cast<size_t>(pixel) * 3 + 0];
         double sy = job.film_sums[static_cast<size_t>(pixel) * 3 + 1];
         double sz = job.film_sums[static_cast<size_t>(pixel) * 3 + 2];
@@ -848,7 +880,9 @@ __global__ void __launch_bounds__(256) k_accumulate(Pool P, const GpuJob* __rest
         const uint32_t p  = g - ch.pixel_prefix;
         const GpuJob&  job = jobs[ch.job];
         if (*job.error_flag != 0u) continue;  // a failed batch publishes nothing
-        const uint32_t pixel = ch.pixel_begin + p;
+        uint32_t       px, py;
+        chunk_xy(ch, job.width, p, px, py);
+        const uint32_t pixel = py * job.width + px;
         double sx = job.film_sums[static_cast<size_t>(pixel) * 3 + 0];
         double sy = job.film_sums[static_cast<size_t>(pixel) * 3 + 1];
         double sz = job.film_sums[static_cast<size_t>(pixel) * 3 + 2];

@@ -177,7 +177,7 @@ struct GpuChunk {
     uint32_t sample_count;
     uint32_t slot_begin;    // first path slot of this chunk in the pool
     uint32_t pixel_prefix;  // pixels of all earlier chunks of the batch
-    uint32_t pad;
+    uint32_t tile_rows;     // image rows of a chunk of whole rows (pixels in 8x8 tiles), else 0 (scanline order)
 };
 
 }  // namespace mrg
