@@ -342,6 +342,10 @@ __global__ void k_reset(Pool P) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < kStatWords; i += gridDim.x * blockDim.x) P.stats[i] = 0;
 }
 
+// Pixel tile of the slot order (a power of two wide; kTileW * kTileH = one wave).
+constexpr uint32_t kTileW = 8, kTileH = 8;
+static_assert((kTileW & (kTileW - 1u)) == 0 && kTileW * kTileH == 64, "tile: power-of-two width, one wave");
+
 // Pixel (x, y) of position q of one sample of a chunk. A chunk of whole image rows (tile_rows != 0)
 // lays its pixels out in 8x8 tiles, tile rows top to bottom and tiles left to right (the last tile
 // row and column hold the remainder rows and columns), so the 64 paths of a wave start from one
@@ -354,16 +358,17 @@ __device__ __forceinline__ void chunk_xy(const GpuChunk& ch, uint32_t width, uin
         x = pixel - y * width;
         return;
     }
-    const uint32_t band = q / (width * 8u);
-    const uint32_t y0   = band * 8u;
-    const uint32_t h    = min(8u, ch.tile_rows - y0);
-    const uint32_t in   = q - band * (width * 8u);
-    const uint32_t full = width & ~7u;
+    constexpr uint32_t TW = kTileW, TH = kTileH;
+    const uint32_t band = q / (width * TH);
+    const uint32_t y0   = band * TH;
+    const uint32_t h    = min(TH, ch.tile_rows - y0);
+    const uint32_t in   = q - band * (width * TH);
+    const uint32_t full = width & ~(TW - 1u);
     if (in < full * h) {
-        const uint32_t t = h == 8u ? in >> 6 : in / (8u * h);
-        const uint32_t w = in - t * (8u * h);
-        y = w >> 3;
-        x = t * 8u + (w & 7u);
+        const uint32_t t = h == TH ? in / (TW * TH) : in / (TW * h);
+        const uint32_t w = in - t * (TW * h);
+        y = w / TW;
+        x = t * TW + (w & (TW - 1u));
     } else {
         const uint32_t rest = width - full;
         const uint32_t w    = in - full * h;
