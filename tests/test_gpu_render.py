@@ -76,6 +76,25 @@ def test_traversal_kernels_agree_with_fp32_twin(oracle, traversal):
     assert prog.statistics["extension_rays"] == st["extension_rays"]
 
 
+@pytest.mark.parametrize("width", [64, 60])
+def test_pixel_split_batches_match_fp32_twin(oracle, width):
+    """An image of more pixels than the path pool holds renders in pixel ranges of 4,096 / 4 spp =
+    1,024 pixels. At 64 pixels wide a range is 16 whole rows and its slots take the 8x8-tiled order
+    from the range's first row; at 60 wide ranges end mid-row and keep scanline order. Both give the
+    FP32 twin's image."""
+    scene = scenegen.cornell_box()
+    height, spp, depth, seed = 80, 4, 6, 5
+    print(f"[render parity] pixel-split cornell {width}x{height} spp={spp} pool=4096", flush=True)
+    with api.Device(0, path_capacity=1 << 12) as dev:
+        dev.kernel_times(reset=True)
+        img, prog = api.render(dev, scene, api.RenderSettings(width, height, spp, seed), api.IntegratorSettings("mis", depth, True, 3))
+        assert dev.kernel_times()["batches"] == 5
+    twin, _, st = oracle.OracleScene(scene).render(scene.camera, width, height, spp, seed=seed, mode="mis", max_depth=depth,
+                                                   rr=True, precision=32, sums=True)
+    _check(img, twin)
+    assert prog.statistics["extension_rays"] == st["extension_rays"]
+
+
 def test_default_refill_batch_matches_fp32_twin(oracle):
     """The library's default for a large batch: AUTO traversal picks the refill launch (a scene of at
     least 2^14 triangles in one batch of 1024 x 1024 x 8 = 2^23 paths), at the library's block counts
