@@ -343,14 +343,51 @@ __global__ void k_reset(Pool P) {
 }
 
 // Pixel tile of the slot order (a power of two wide; kTileW * kTileH = one wave).
-constexpr uint32_t kTileW = 8, kTileH = 8;
-static_assert((kTileW & (kTileW - 1u)) == 0 && kTileW * kTileH == 64, "tile: power-of-two width, one wave");
+constexpr uint32_t kTileW = 4, kTileH = 4;
+// Samples of one pixel held side by side in a tiled chunk (a power of two): a wave then holds
+// kSampleGroup samples of each pixel of one kTileW x kTileH tile.
+constexpr uint32_t kSampleGroup = 4;
+static_assert((kTileW & (kTileW - 1u)) == 0 && (kSampleGroup & (kSampleGroup - 1u)) == 0 && kTileW * kTileH * kSampleGroup == 64,
+              "tile: power-of-two width, one wave");
 
-// Pixel (x, y) of position q of one sample of a chunk. A chunk of whole image rows (tile_rows != 0)
-// lays its pixels out in 8x8 tiles, tile rows top to bottom and tiles left to right (the last tile
-// row and column hold the remainder rows and columns), so the 64 paths of a wave start from one
-// compact patch of the image instead of a 64-pixel strip. Other chunks keep scanline order.
-// k_generate, k_validate and k_accumulate use the same map, so every sample lands on its own pixel.
+// Slot layout of a chunk: samples [0, grouped) in groups of kSampleGroup, each group pixel-major
+// (slot = group base + q * kSampleGroup + sample within the group), then the remaining samples one
+// after the other, each over all pixel positions q. Chunks in scanline order group no samples.
+__device__ __forceinline__ uint32_t grouped_samples(const GpuChunk& ch) {
+    return ch.tile_rows ? ch.sample_count & ~(kSampleGroup - 1u) : 0u;
+}
+
+// Sample s and pixel position q of slot `local` (counted from the chunk's first slot).
+__device__ __forceinline__ void chunk_sample(const GpuChunk& ch, uint32_t local, uint32_t& s, uint32_t& q) {
+    const uint32_t n       = ch.pixel_count;
+    const uint32_t grouped = grouped_samples(ch);
+    if (local < grouped * n) {
+        const uint32_t g   = local / (n * kSampleGroup);
+        const uint32_t rem = local - g * (n * kSampleGroup);
+        q = rem / kSampleGroup;
+        s = g * kSampleGroup + (rem & (kSampleGroup - 1u));
+    } else {
+        const uint32_t rest = local - grouped * n;
+        const uint32_t k    = rest / n;
+        s = grouped + k;
+        q = rest - k * n;
+    }
+}
+
+// Slot of sample s at pixel position q (the inverse of chunk_sample).
+__device__ __forceinline__ uint32_t chunk_slot(const GpuChunk& ch, uint32_t s, uint32_t q) {
+    const uint32_t n       = ch.pixel_count;
+    const uint32_t grouped = grouped_samples(ch);
+    if (s < grouped) return ch.slot_begin + (s / kSampleGroup) * (n * kSampleGroup) + q * kSampleGroup + (s & (kSampleGroup - 1u));
+    return ch.slot_begin + grouped * n + (s - grouped) * n + q;
+}
+
+// Pixel (x, y) of pixel position q of a chunk. A chunk of whole image rows (tile_rows != 0) lays its
+// pixels out in kTileW x kTileH tiles, tile rows top to bottom and tiles left to right (the last tile
+// row and column hold the remainder rows and columns); with kSampleGroup samples of each pixel side
+// by side (chunk_sample), the 64 paths of a wave start from one compact patch of the image instead
+// of a 64-pixel strip. Other chunks keep scanline order. k_generate, k_validate and k_accumulate
+// use the same maps, so every sample lands on its own pixel.
 __device__ __forceinline__ void chunk_xy(const GpuChunk& ch, uint32_t width, uint32_t q, uint32_t& x, uint32_t& y) {
     if (ch.tile_rows == 0u) {
         const uint32_t pixel = ch.pixel_begin + q;
@@ -403,9 +440,8 @@ __global__ void __launch_bounds__(256) k_generate(Pool P, const GpuJob* __restri
     for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < total; slot += gridDim.x * blockDim.x) {
         const uint32_t  ci = find_chunk_by_slot(chunks, chunk_count, slot);
         const GpuChunk  ch = chunks[ci];
-        const uint32_t  local  = slot - ch.slot_begin;
-        const uint32_t  s      = local / ch.pixel_count;
-        const uint32_t  p      = local - s * ch.pixel_count;
+        uint32_t        s, p;
+        chunk_sample(ch, slot - ch.slot_begin, s, p);
         const GpuJob&   job    = jobs[ch.job];
         uint32_t        x, y;
         chunk_xy(ch, job.width, p, x, y);
@@ -861,7 +897,7 @@ __global__ void __launch_bounds__(256) k_validate(Pool P, const GpuJob* __restri
         double sz = job.film_sums[static_cast<size_t>(pixel) * 3 + 2];
         uint32_t bad = 0;
         for (uint32_t s = 0; s < ch.sample_count; ++s) {
-            const float4 v = P.rad[ch.slot_begin + s * ch.pixel_count + p];
+            const float4 v = P.rad[chunk_slot(ch, s, p)];
             if (!(__builtin_isfinite(v.x) && __builtin_isfinite(v.y) && __builtin_isfinite(v.z)) || v.x < 0.0f || v.y < 0.0f || v.z < 0.0f)
                 bad |= 1u;
             sx = sx + static_cast<double>(v.x);
@@ -892,7 +928,7 @@ __global__ void __launch_bounds__(256) k_accumulate(Pool P, const GpuJob* __rest
         double sy = job.film_sums[static_cast<size_t>(pixel) * 3 + 1];
         double sz = job.film_sums[static_cast<size_t>(pixel) * 3 + 2];
         for (uint32_t s = 0; s < ch.sample_count; ++s) {
-            const float4 v = P.rad[ch.slot_begin + s * ch.pixel_count + p];
+            const float4 v = P.rad[chunk_slot(ch, s, p)];
             sx = sx + static_cast<double>(v.x);
             sy = sy + static_cast<double>(v.y);
             sz = sz + static_cast<double>(v.z);

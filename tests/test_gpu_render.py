@@ -79,9 +79,9 @@ def test_traversal_kernels_agree_with_fp32_twin(oracle, traversal):
 @pytest.mark.parametrize("width", [64, 60])
 def test_pixel_split_batches_match_fp32_twin(oracle, width):
     """An image of more pixels than the path pool holds renders in pixel ranges of 4,096 / 4 spp =
-    1,024 pixels. At 64 pixels wide a range is 16 whole rows and its slots take the 8x8-tiled order
-    from the range's first row; at 60 wide ranges end mid-row and keep scanline order. Both give the
-    FP32 twin's image."""
+    1,024 pixels. At 64 pixels wide a range is 16 whole rows and its slots take the tiled order (4x4
+    pixel tiles, 4 samples of a pixel side by side) from the range's first row; at 60 wide ranges end
+    mid-row and keep scanline order. Both give the FP32 twin's image."""
     scene = scenegen.cornell_box()
     height, spp, depth, seed = 80, 4, 6, 5
     print(f"[render parity] pixel-split cornell {width}x{height} spp={spp} pool=4096", flush=True)
