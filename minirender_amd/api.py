@@ -141,7 +141,9 @@ class SceneBvh:
         self.handle = h
 
     def release(self):
-        if getattr(self, "handle", None) and getattr(self.device, "handle", None):
+        # valid after Device.close too: the library keeps a closed device's bookkeeping until its last
+        # scene and job are released
+        if getattr(self, "handle", None):
             N.check(N.lib().mrg_scene_release(self.handle))
         self.handle = None
 

@@ -68,20 +68,23 @@ def run_workload(options, schedule, scenes, requests, repeats=2, pass_samples=16
     return best[0], [r[1] for r in runs]
 
 
-def tune(scenes, requests, schedule="batched", knobs=None, rounds=1, repeats=2, log=None):
+def tune(scenes, requests, schedule="batched", knobs=None, rounds=1, repeats=2, log=None, scorer=None):
     """Coordinate descent over `knobs` (name -> candidate values; default KNOBS, without
-    max_jobs_in_flight for the plain schedules). Returns the best options and every measurement."""
+    max_jobs_in_flight for the plain schedules). Returns the best options and every measurement.
+    `scorer(options) -> (samples/s, per-run seconds)` replaces the device measurement (run_workload on
+    `scenes` and `requests`) when given."""
     knobs = dict(knobs or KNOBS)
     if schedule != "batched":
         knobs.pop("max_jobs_in_flight", None)
     best = {k: DEFAULTS[k] for k in knobs}
     cache = {}
     trials = []
+    measure = scorer or (lambda opts: run_workload(opts, schedule, scenes, requests, repeats))
 
     def score(opts):
         key = json.dumps(opts, sort_keys=True)
         if key not in cache:
-            value, secs = run_workload(opts, schedule, scenes, requests, repeats)
+            value, secs = measure(opts)
             cache[key] = value
             trials.append({"options": dict(opts), "samples_per_s": value, "seconds": secs})
             if log:

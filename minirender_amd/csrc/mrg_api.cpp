@@ -169,9 +169,14 @@ struct mrg_job {
     Clock::time_point submitted, admitted_at, first_pass_at, finished_at;
     bool          first_pass_done = false, finished = false;
     uint64_t      film_bytes = 0;
+    bool          film_counted = false;  // film_bytes are part of device->resident_bytes
     int           refs = 1;  // caller + scheduler
     DevBuf        film_sums, film_counts, error_flag;
-    std::mutex    film_mutex;  // held while a batch that writes this film is in flight
+    // A finished job's film, copied out of HBM (the device buffers are freed at that point).
+    std::vector<double>   host_sums;
+    std::vector<uint32_t> host_counts;
+    bool                  film_on_host = false;
+    std::mutex    film_mutex;  // held while a batch that writes this film is in flight, and by film reads
 };
 
 struct mrg_device {
@@ -198,6 +203,12 @@ struct mrg_device {
     std::deque<mrg_job*>               jobs;  // admission order
     uint64_t                           next_job_id = 1;
     bool                               shutdown = false;
+    // Set once mrg_device_close has stopped the scheduler and freed the device's own buffers and
+    // streams. The object itself lives on while scenes or jobs still reference it (`live` counts
+    // them), so their entry points keep working on host state (progress, films) or fail cleanly.
+    std::atomic<bool>                  closed{false};
+    bool                               released = false;  // mrg_device_close has returned (guarded by mutex)
+    uint64_t                           live = 0;
     std::thread                        worker;
     uint64_t                           resident_bytes = 0;  // scenes + admitted films
     mrg_kernel_times                   times{};
@@ -392,6 +403,7 @@ mrg_scene* build_scene(mrg_device* d, const mrg_scene_desc& s, const std::string
         st(a.radiance, radiance);
         a.area         = area;
         a.double_sided = m.double_sided ? 1u : 0u;
+        a.prim         = uint32_t(p);
         areas.push_back(a);
         weights.push_back(float(m.double_sided ? 2 : 1) * kPiF * area * luminance(radiance));
     }
@@ -556,14 +568,49 @@ bool terminal(uint32_t s) {
 
 uint64_t film_bytes_of(const mrg_job_desc& d) { return uint64_t(d.width) * d.height * (24 + 4); }
 
-void release_job_locked(mrg_job* j);
+bool release_job_locked(mrg_job* j);
 
-void finish_job_locked(mrg_device* d, mrg_job* j, uint32_t state) {
+void uncount_film_locked(mrg_device* d, mrg_job* j) {
+    if (!j->film_counted) return;
+    d->resident_bytes -= std::min(d->resident_bytes, j->film_bytes);
+    j->film_counted = false;
+}
+
+// Copies a finished job's film into host memory and frees its HBM, so admission counts only the films
+// of jobs that can still render while every film stays readable. The caller holds the job's
+// film_mutex, and no batch that writes the film is in flight. On a copy failure the film stays on the
+// device (and in the budget) until the job is released.
+void retire_film(mrg_device* d, mrg_job* j) {
+    if (!j->film_sums.ptr || j->film_on_host) return;
+    const uint64_t px = uint64_t(j->desc.width) * j->desc.height;
+    j->host_sums.resize(px * 3);
+    j->host_counts.resize(px);
+    if (hipMemcpyAsync(j->host_sums.data(), j->film_sums.ptr, px * 24, hipMemcpyDeviceToHost, d->copy_stream) != hipSuccess ||
+        hipMemcpyAsync(j->host_counts.data(), j->film_counts.ptr, px * 4, hipMemcpyDeviceToHost, d->copy_stream) != hipSuccess ||
+        hipStreamSynchronize(d->copy_stream) != hipSuccess) {
+        j->host_sums.clear();
+        j->host_counts.clear();
+        return;
+    }
+    j->film_on_host = true;
+    j->film_sums.release();
+    j->film_counts.release();
+    j->error_flag.release();
+    uncount_film_locked(d, j);
+}
+
+// Terminal state; `film_held`: the caller already holds the job's film_mutex (a batch just landed).
+void finish_job_locked(mrg_device* d, mrg_job* j, uint32_t state, bool film_held = false) {
     j->state       = state;
     j->finished    = true;
     j->finished_at = Clock::now();
     if (j->admitted) {
-        d->resident_bytes -= std::min(d->resident_bytes, j->film_bytes);
+        if (film_held) {
+            retire_film(d, j);
+        } else {
+            std::lock_guard<std::mutex> fl(j->film_mutex);
+            retire_film(d, j);
+        }
         j->admitted = false;
     }
     d->cv.notify_all();
@@ -584,8 +631,12 @@ void admit_locked(mrg_device* d) {
         check_hip(hipMemsetAsync(j->film_sums.ptr, 0, j->film_sums.bytes, d->stream), "hipMemsetAsync");
         check_hip(hipMemsetAsync(j->film_counts.ptr, 0, j->film_counts.bytes, d->stream), "hipMemsetAsync");
         check_hip(hipMemsetAsync(j->error_flag.ptr, 0, 16, d->stream), "hipMemsetAsync");
-        j->admitted    = true;
-        j->admitted_at = Clock::now();
+        // film reads run on the copy stream: the zeroed film is complete before the job is visible as
+        // admitted (the scheduler stream is idle between batches, so this waits for the fills only)
+        check_hip(hipStreamSynchronize(d->stream), "hipStreamSynchronize");
+        j->admitted     = true;
+        j->admitted_at  = Clock::now();
+        j->film_counted = true;
         d->resident_bytes += j->film_bytes;
         if (j->state == MRG_STATE_PREPARING) j->state = j->pause_requested ? MRG_STATE_PAUSED : MRG_STATE_RUNNING;
     }
@@ -893,13 +944,13 @@ void run_batch(mrg_device* d, BatchPlan& b) {
         if (flags[i * 4] != 0) {
             j->status = MRG_NONFINITE;
             j->error  = (flags[i * 4] & 1u) ? "Path contribution is nonfinite or negative" : "Render sample exceeds the float image range";
-            finish_job_locked(d, j, MRG_STATE_FAILED);
+            finish_job_locked(d, j, MRG_STATE_FAILED, true);
         } else if (j->cancel_requested) {
             j->status = MRG_CANCELLED;
-            finish_job_locked(d, j, MRG_STATE_CANCELLED);
+            finish_job_locked(d, j, MRG_STATE_CANCELLED, true);
         } else if (j->next_sample >= j->desc.samples_per_pixel && j->next_pixel == 0) {
             j->state = MRG_STATE_FINALIZING;
-            finish_job_locked(d, j, MRG_STATE_COMPLETED);
+            finish_job_locked(d, j, MRG_STATE_COMPLETED, true);
         } else if (j->pause_requested) {
             j->state = MRG_STATE_PAUSED;
         }
@@ -916,7 +967,20 @@ void scheduler(mrg_device* d) {
         {
             std::unique_lock<std::mutex> lock(d->mutex);
             for (;;) {
-                if (d->shutdown) return;
+                if (d->shutdown) {
+                    // every job still queued ends cancelled (its film, if admitted, retired to host memory
+                    // so it stays readable), waiters wake, and the queue drops its references
+                    for (mrg_job* j : d->jobs) {
+                        if (!j->finished) {
+                            j->status = MRG_CANCELLED;
+                            finish_job_locked(d, j, MRG_STATE_CANCELLED);
+                        }
+                        release_job_locked(j);
+                    }
+                    d->jobs.clear();
+                    d->cv.notify_all();
+                    return;
+                }
                 try {
                     admit_locked(d);
                 } catch (const std::exception& e) {
@@ -963,7 +1027,7 @@ void scheduler(mrg_device* d) {
                 j->in_flight = false;
                 j->status    = MRG_DEVICE_ERROR;
                 j->error     = e.what();
-                if (!j->finished) finish_job_locked(d, j, MRG_STATE_FAILED);
+                if (!j->finished) finish_job_locked(d, j, MRG_STATE_FAILED, true);
                 j->film_mutex.unlock();
                 release_job_locked(j);
             }
@@ -971,16 +1035,24 @@ void scheduler(mrg_device* d) {
     }
 }
 
-void release_job_locked(mrg_job* j) {
+// Drops one reference of a job; deletes the job (and its scene with the scene's last reference).
+// Returns true when the device is closed and nothing references it any more: the caller deletes it
+// once it has let go of the device mutex.
+bool release_job_locked(mrg_job* j) {
+    mrg_device* d = j->device;
     if (--j->refs == 0) {
         mrg_scene* s = j->scene;
+        uncount_film_locked(d, j);
         delete j;
+        d->live--;
         if (s && --s->refs == 0) {
-            s->device->scenes.erase(s->hash);
-            s->device->resident_bytes -= std::min(s->device->resident_bytes, s->info.device_bytes);
+            d->scenes.erase(s->hash);
+            d->resident_bytes -= std::min(d->resident_bytes, s->info.device_bytes);
             delete s;
+            d->live--;
         }
     }
+    return d->released && d->live == 0;
 }
 
 void validate_job(const mrg_job_desc& d) {
@@ -1056,20 +1128,40 @@ int mrg_device_close(mrg_device* d) {
         if (!d) return;
         {
             std::lock_guard<std::mutex> lock(d->mutex);
+            if (d->shutdown) throw std::invalid_argument("device is already closed");
             for (mrg_job* j : d->jobs) j->cancel_requested = true;
             d->shutdown = true;
             d->cv.notify_all();
         }
+        // the scheduler cancels every unfinished job (films retired to host memory) before it returns
         if (d->worker.joinable()) d->worker.join();
         (void)hipSetDevice(d->ordinal);
-        (void)hipStreamSynchronize(d->stream);
-        for (auto& g : d->graphs) (void)hipGraphExecDestroy(g.second);
-        for (auto& e : d->events) (void)hipEventDestroy(e);
-        for (auto& e : d->query_events) (void)hipEventDestroy(e);
-        (void)hipStreamDestroy(d->stream);
-        (void)hipStreamDestroy(d->copy_stream);
-        (void)hipStreamDestroy(d->query_stream);
-        delete d;  // remaining scenes/jobs owned by callers are leaked intentionally only if unreleased
+        {
+            std::lock_guard<std::mutex> q(d->query_mutex);  // no ray query is using the streams
+            (void)hipStreamSynchronize(d->stream);
+            (void)hipStreamSynchronize(d->query_stream);
+            for (auto& g : d->graphs) (void)hipGraphExecDestroy(g.second);
+            d->graphs.clear();
+            for (auto& e : d->events) (void)hipEventDestroy(e);
+            d->events.clear();
+            for (auto& e : d->query_events) (void)hipEventDestroy(e);
+            (void)hipStreamDestroy(d->stream);
+            (void)hipStreamDestroy(d->copy_stream);
+            (void)hipStreamDestroy(d->query_stream);
+            d->stream = d->copy_stream = d->query_stream = nullptr;
+            for (DevBuf* b : {&d->pool_mem, &d->spill_ref, &d->spill_t, &d->counters, &d->stats, &d->header, &d->jobs_buf,
+                              &d->chunks_buf, &d->query_work, &d->query_spill_ref, &d->query_spill_t, &d->energy})
+                b->release();
+            d->closed = true;
+        }
+        bool last;
+        {
+            std::lock_guard<std::mutex> lock(d->mutex);
+            d->released = true;
+            last        = d->live == 0;
+        }
+        // with scenes or jobs still held by the caller, the last of their releases deletes the device
+        if (last) delete d;
     });
 }
 
@@ -1095,6 +1187,7 @@ int mrg_conductor_energy_table(float* out) {
 int mrg_scene_upload(mrg_device* d, const mrg_scene_desc* s, mrg_scene** out) {
     return guarded([&] {
         if (!d || !s || !out) throw std::invalid_argument("null argument");
+        if (d->closed) throw std::invalid_argument("device is closed");
         Hasher h;
         const uint64_t n = s->triangle_count;
         h.bytes(&n, 8);
@@ -1143,6 +1236,7 @@ int mrg_scene_upload(mrg_device* d, const mrg_scene_desc* s, mrg_scene** out) {
         }
         d->scenes[key] = sc;
         d->resident_bytes += sc->info.device_bytes;
+        d->live++;
         *out = sc;
     });
 }
@@ -1150,13 +1244,19 @@ int mrg_scene_upload(mrg_device* d, const mrg_scene_desc* s, mrg_scene** out) {
 int mrg_scene_release(mrg_scene* s) {
     return guarded([&] {
         if (!s) return;
-        mrg_device*                 d = s->device;
-        std::lock_guard<std::mutex> lock(d->mutex);
-        if (--s->refs == 0) {
-            d->scenes.erase(s->hash);
-            d->resident_bytes -= std::min(d->resident_bytes, s->info.device_bytes);
-            delete s;
+        mrg_device* d    = s->device;
+        bool        last = false;
+        {
+            std::lock_guard<std::mutex> lock(d->mutex);
+            if (--s->refs == 0) {
+                d->scenes.erase(s->hash);
+                d->resident_bytes -= std::min(d->resident_bytes, s->info.device_bytes);
+                delete s;
+                d->live--;
+                last = d->released && d->live == 0;
+            }
         }
+        if (last) delete d;
     });
 }
 
@@ -1176,6 +1276,7 @@ static void run_query(mrg_scene* s, const void* d_rays, uint64_t n, float tmin, 
     if (n >= (uint64_t(1) << 32) - 64) throw std::invalid_argument("too many rays for one call");
     mrg_device* d = s->device;
     std::lock_guard<std::mutex> lock(d->query_mutex);  // one query at a time per device (shared spill area)
+    if (d->closed) throw std::invalid_argument("device is closed");
     check_hip(hipSetDevice(d->ordinal), "hipSetDevice");
     DevBuf st(16);
     check_hip(hipMemsetAsync(st.ptr, 0, 16, d->query_stream), "hipMemsetAsync");
@@ -1200,6 +1301,7 @@ int mrg_intersect(mrg_scene* s, const float* rays, uint64_t n, float tmin, float
     return guarded([&] {
         if (!s || (n && (!rays || !hits))) throw std::invalid_argument("null argument");
         if (n == 0) return;
+        if (s->device->closed) throw std::invalid_argument("device is closed");
         check_hip(hipSetDevice(s->device->ordinal), "hipSetDevice");
         DevBuf dr(n * 24), dh(n * 8), db(bary ? n * 12 : 0);
         check_hip(hipMemcpy(dr.ptr, rays, n * 24, hipMemcpyHostToDevice), "hipMemcpy");
@@ -1213,6 +1315,7 @@ int mrg_occluded(mrg_scene* s, const float* rays, uint64_t n, float tmin, float 
     return guarded([&] {
         if (!s || (n && (!rays || !occ))) throw std::invalid_argument("null argument");
         if (n == 0) return;
+        if (s->device->closed) throw std::invalid_argument("device is closed");
         check_hip(hipSetDevice(s->device->ordinal), "hipSetDevice");
         DevBuf dr(n * 24), dout(n);
         check_hip(hipMemcpy(dr.ptr, rays, n * 24, hipMemcpyHostToDevice), "hipMemcpy");
@@ -1244,9 +1347,11 @@ int mrg_job_submit(mrg_device* d, mrg_scene* s, const mrg_job_desc* desc, mrg_jo
         j->submitted  = Clock::now();
         j->refs       = 2;  // caller + scheduler queue
         std::lock_guard<std::mutex> lock(d->mutex);
+        if (d->shutdown) throw std::invalid_argument("device is closed");
         if (j->film_bytes + d->pool_bytes + s->info.device_bytes > budget(d))
             throw Error(MRG_NOT_ADMITTED, "job exceeds the device HBM budget");
         s->refs++;
+        d->live++;
         j->id = d->next_job_id++;
         d->jobs.push_back(j.get());
         *out = j.release();
@@ -1339,6 +1444,11 @@ static void read_film_raw(mrg_job* j, std::vector<double>& sums, std::vector<uin
     sums.assign(px * 3, 0.0);
     counts.assign(px, 0);
     std::lock_guard<std::mutex> fl(j->film_mutex);
+    if (j->film_on_host) {  // finished: the film was copied out of HBM
+        sums   = j->host_sums;
+        counts = j->host_counts;
+        return;
+    }
     if (!j->film_sums.ptr) return;  // not admitted yet
     check_hip(hipSetDevice(j->device->ordinal), "hipSetDevice");
     check_hip(hipMemcpyAsync(sums.data(), j->film_sums.ptr, px * 24, hipMemcpyDeviceToHost, j->device->copy_stream), "read film");
@@ -1401,14 +1511,18 @@ int mrg_job_preview(mrg_job* j, uint32_t max_dim, float* rgb, uint32_t* width, u
 int mrg_job_release(mrg_job* j) {
     return guarded([&] {
         if (!j) return;
-        mrg_device* d = j->device;
-        std::lock_guard<std::mutex> lock(d->mutex);
-        if (!terminal(j->state)) {
-            j->cancel_requested = true;
-            j->state            = MRG_STATE_CANCELLING;
-            d->cv.notify_all();
+        mrg_device* d    = j->device;
+        bool        last = false;
+        {
+            std::lock_guard<std::mutex> lock(d->mutex);
+            if (!terminal(j->state)) {
+                j->cancel_requested = true;
+                j->state            = MRG_STATE_CANCELLING;
+                d->cv.notify_all();
+            }
+            last = release_job_locked(j);
         }
-        release_job_locked(j);
+        if (last) delete d;
     });
 }
 

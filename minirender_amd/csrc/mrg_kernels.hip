@@ -508,7 +508,7 @@ __device__ __forceinline__ void shadow_one(const Pool& P, const GpuJob* __restri
     const float4   d     = P.sh_d[i];
     const uint32_t job   = ONE_SCENE ? 0u : __float_as_uint(P.prev[path].w) >> 16;
     const RayPre   rp    = ray_setup(mk(o.x, o.y, o.z), mk(d.x, d.y, d.z));
-    const TraceResult res = trace_ray<true, false>(jobs[job].scene, rp, 0.0f, o.w, stk, &dummy, &dummy);
+    const TraceResult res = trace_ray<true, false>(jobs[job].scene, rp, 0.0f, o.w, stk, &dummy, &dummy, __float_as_uint(d.w));
     if (res.slot == 0xffffffffu) {
         const float4 c = P.sh_c[i];
         float4       l = P.rad[path];
@@ -587,7 +587,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k
                 path           = P.q_sh[i];
                 const float4 o = P.sh_o[i];
                 const float4 d = P.sh_d[i];
-                trace_init(st, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), 0.0f, o.w);
+                trace_init(st, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), 0.0f, o.w, __float_as_uint(d.w));
             } else {
                 path            = queue ? queue[i - n_sh] : i - n_sh;
                 const float4 ro = P.ray_o[path];
@@ -771,7 +771,8 @@ __device__ __forceinline__ void shade_path(const Pool& P, const GpuJob& job, uin
                             emit   = true;
                             sh_key = octant(sdir.x, sdir.y, sdir.z);
                             so   = make_float4(origin.x, origin.y, origin.z, limit);
-                            sd   = make_float4(sdir.x, sdir.y, sdir.z, 0.0f);
+                            // w: the emitter triangle the segment ends on, which its any-hit skips
+                            sd   = make_float4(sdir.x, sdir.y, sdir.z, __uint_as_float(ls.prim));
                             sc4  = make_float4(c.x, c.y, c.z, 0.0f);
                         }
                     }

@@ -96,7 +96,7 @@ struct GpuAreaLight {
     float    radiance[3];
     float    area;
     uint32_t double_sided;
-    uint32_t pad;
+    uint32_t prim;  // scene-order primitive index of the emitter triangle (skipped by its shadow rays)
 };
 
 struct GpuLightChoice {

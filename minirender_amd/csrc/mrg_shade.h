@@ -443,10 +443,11 @@ __device__ __forceinline__ float area_density(F3 ref, F3 point, F3 normal, float
 }
 
 struct LightSample {
-    F3    dir, radiance;
-    float distance, probability;
-    bool  discrete;
-    F3    position, normal, error;
+    F3       dir, radiance;
+    float    distance, probability;
+    bool     discrete;
+    F3       position, normal, error;
+    uint32_t prim;  // area emitter's scene-order primitive (its shadow ray skips it), else 0xffffffff
 };
 
 // DiscreteDistribution::sample index (upper_bound over the CDF) and mass.
@@ -474,6 +475,7 @@ __device__ __forceinline__ bool sample_light(const GpuScene& sc, F3 ref, float l
         out.distance    = __builtin_inff();
         out.probability = mass;
         out.discrete    = true;
+        out.prim        = 0xffffffffu;
         return true;
     }
     if (ch.kind == 2u) {
@@ -487,6 +489,7 @@ __device__ __forceinline__ bool sample_light(const GpuScene& sc, F3 ref, float l
         out.distance    = __builtin_inff();
         out.probability = mass * pdf;
         out.discrete    = false;
+        out.prim        = 0xffffffffu;
         return true;
     }
     const GpuAreaLight& a  = sc.areas[ch.index];
@@ -506,6 +509,7 @@ __device__ __forceinline__ bool sample_light(const GpuScene& sc, F3 ref, float l
     out.discrete    = false;
     out.position    = pt;
     out.normal      = nrm;
+    out.prim        = a.prim;
     return true;
 }
 

@@ -210,12 +210,14 @@ struct TraceState {
     uint32_t    sp;
 };
 
-__device__ __forceinline__ void trace_init(TraceState& st, F3 o, F3 d, float tmin, float tmax) {
+// `skip`: for an any-hit ray, a scene-order primitive to ignore (the emitter a shadow segment ends
+// on); it waits in res.prim, which an any-hit ray only overwrites with the hit that finishes it.
+__device__ __forceinline__ void trace_init(TraceState& st, F3 o, F3 d, float tmin, float tmax, uint32_t skip = 0xffffffffu) {
     st.r        = ray_setup(o, d);
     st.tmin     = tmin;
     st.res.t    = tmax;
     st.res.slot = 0xffffffffu;
-    st.res.prim = 0xffffffffu;
+    st.res.prim = skip;
     st.res.b0 = st.res.b1 = st.res.b2 = 0.0f;
     st.res.det = 1.0f;
     st.ref  = 0;  // the root is always an inner node
@@ -273,6 +275,7 @@ __device__ __forceinline__ bool trace_step_dyn(const TraceScene& sc, TraceState&
                     if (dot(r.d, outward) >= 0.0f) continue;
                 }
                 const uint32_t prim = __float_as_uint(a.w);
+                if (any && prim == st.res.prim) continue;
                 if (st.res.slot == 0xffffffffu || h.t < st.res.t || (h.t == st.res.t && prim < st.res.prim)) {
                     st.res.t    = h.t;
                     st.res.b0   = h.a;
@@ -332,14 +335,16 @@ __device__ __forceinline__ bool trace_step_dyn(const TraceScene& sc, TraceState&
     return st.ref == kEmptyRef && st.leaf == kEmptyRef;
 }
 
-// Closest hit (ANY = false) or any hit (ANY = true) in [tmin, tmax].
+// Closest hit (ANY = false) or any hit (ANY = true) in [tmin, tmax]; an any-hit query ignores the
+// scene-order primitive `skip`.
 template <bool ANY, bool STATS>
 __device__ __forceinline__ TraceResult trace_ray(const GpuScene& sc, const RayPre& r, float tmin, float tmax,
-                                                 const StackView& stk, uint64_t* bounds_tests, uint64_t* tri_tests) {
+                                                 const StackView& stk, uint64_t* bounds_tests, uint64_t* tri_tests,
+                                                 uint32_t skip = 0xffffffffu) {
     TraceResult res;
     res.t    = tmax;
     res.slot = 0xffffffffu;
-    res.prim = 0xffffffffu;
+    res.prim = ANY ? skip : 0xffffffffu;
     res.b0 = res.b1 = res.b2 = 0.0f;
     res.det = 1.0f;
     if (sc.node_count == 0) return res;
@@ -392,6 +397,7 @@ __device__ __forceinline__ TraceResult trace_ray(const GpuScene& sc, const RayPr
                     if (dot(r.d, outward) >= 0.0f) continue;
                 }
                 const uint32_t prim = __float_as_uint(a.w);
+                if (ANY && prim == res.prim) continue;
                 if (res.slot == 0xffffffffu || h.t < res.t || (h.t == res.t && prim < res.prim)) {
                     res.t    = h.t;
                     res.b0   = h.a;

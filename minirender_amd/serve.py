@@ -42,20 +42,23 @@ class Request:
     max_depth: int
     seed: int
     arrival: float          # seconds after the stream starts
-    cancel_after: float     # seconds after submission, or -1
+    cancel_after_passes: int  # cancelled once this many progressive passes are recorded, or -1
 
 
 def request_stream(count=64, rate=40.0, seed=5, cancel_fraction=0.05, scenes=8, side=(128, 1024), spp=(16, 1024),
                    depth=(8, 16)):
-    """Poisson arrivals at `rate` jobs/s; sizes and sample counts log-uniform; seeded."""
+    """Poisson arrivals at `rate` jobs/s (rate = inf: every request at 0); sizes and sample counts
+    log-uniform; a cancelled request is cancelled after 1-3 recorded passes, so every schedule renders
+    about the same samples of it. Seeded: the same arguments give the same requests at every rate."""
     r = np.random.default_rng(seed)
     t = 0.0
     reqs = []
     for i in range(count):
-        t += r.exponential(1.0 / rate)
+        gap = r.exponential(1.0)
+        t += gap / rate if math.isfinite(rate) else 0.0
         s = int(round(math.exp(r.uniform(math.log(side[0]), math.log(side[1])))))
         n = int(round(math.exp(r.uniform(math.log(spp[0]), math.log(spp[1])))))
-        cancel = float(r.uniform(0.0, 0.5)) if r.random() < cancel_fraction else -1.0
+        cancel = int(r.integers(1, 4)) if r.random() < cancel_fraction else -1
         reqs.append(Request(i, int(r.integers(0, scenes)), s, s, n, int(r.integers(depth[0], depth[1] + 1)),
                             int(r.integers(0, 2**31)), t, cancel))
     return reqs
@@ -75,28 +78,51 @@ def _percentile(v, q):
     return float(np.percentile(np.asarray(v, np.float64), q)) if len(v) else None
 
 
-def run_stream(device: api.Device, bvhs, scenes, requests, pass_samples=16, poll=0.002, keep_films=False):
-    """Submits each request at its arrival time, cancels the selected ones, waits for all of them and
-    returns per-job records plus aggregate throughput and latency percentiles."""
+TERMINAL = ("cancelled", "completed", "failed", "idle")
+
+
+def run_stream(device: api.Device, bvhs, scenes, requests, pass_samples=16, poll=0.001, keep_films=False):
+    """Submits each request at its arrival time, cancels the selected ones once they have recorded
+    their pass count, waits for all of them and returns (summary, per-job records, films)."""
     jobs = [None] * len(requests)
-    cancels = []
+    watch, lock, done_submitting = {}, threading.Lock(), threading.Event()
+
+    def canceller():
+        while True:
+            with lock:
+                items = list(watch.items())
+            if not items and done_submitting.is_set():
+                return
+            for index, (job, threshold) in items:
+                p = job.progress()
+                if p.state in TERMINAL or p.recorded_samples >= threshold:
+                    if p.state not in TERMINAL:
+                        job.cancel()
+                    with lock:
+                        del watch[index]
+            time.sleep(poll)
+
+    watcher = threading.Thread(target=canceller, daemon=True)
+    watcher.start()
     start = time.perf_counter()
-    for req in requests:
-        delay = req.arrival - (time.perf_counter() - start)
-        if delay > 0:
-            time.sleep(delay)
-        sc = scenes[req.scene]
-        jobs[req.index] = api.Job(bvhs[req.scene], sc.camera, api.RenderSettings(req.width, req.height, req.spp, req.seed),
-                                  api.IntegratorSettings("mis", req.max_depth, True, 3), pass_samples)
-        if req.cancel_after >= 0:
-            t = threading.Timer(req.cancel_after, jobs[req.index].cancel)
-            t.start()
-            cancels.append(t)
+    try:
+        for req in requests:
+            delay = req.arrival - (time.perf_counter() - start)
+            if delay > 0:
+                time.sleep(delay)
+            sc = scenes[req.scene]
+            job = api.Job(bvhs[req.scene], sc.camera, api.RenderSettings(req.width, req.height, req.spp, req.seed),
+                          api.IntegratorSettings("mis", req.max_depth, True, 3), pass_samples)
+            jobs[req.index] = job
+            if req.cancel_after_passes >= 0:
+                with lock:
+                    watch[req.index] = (job, req.cancel_after_passes * pass_samples * req.width * req.height)
+    finally:
+        done_submitting.set()
     for j in jobs:
         j.wait(-1)
     end = time.perf_counter()
-    for t in cancels:
-        t.join()
+    watcher.join()
     records, films = [], {}
     for req, j in zip(requests, jobs):
         p = j.progress()
@@ -120,29 +146,67 @@ def run_stream(device: api.Device, bvhs, scenes, requests, pass_samples=16, poll
     return summary, records, films
 
 
-def compare(requests=None, scenes=None, hbm_budget=64 << 30, max_jobs=64, use_graph=True, keep_films=False,
-            path_capacity=1 << 26):
-    """Replays one request stream on the plain and on the batched schedule. The serving pool is 2^26
-    paths (12 GB): a batch is the unit of progress and pause latency, and smaller batches keep the
-    time to first pass of newly arrived jobs short."""
-    scenes = scenes or scene_pool()
-    requests = requests or request_stream()
-    out = {}
-    films = {}
-    for mode in ("plain", "plain_fill", "batched"):
+SCHEDULES = ("plain", "plain_fill", "batched")
+
+
+def open_devices(scenes, hbm_budget=64 << 30, max_jobs=64, use_graph=True, path_capacity=1 << 26, warm=True):
+    """One device per schedule with every scene resident, warmed by a short stream (graph capture, first
+    touch of the path pool). The serving pool is 2^26 paths (12 GB): a batch is the unit of progress and
+    pause latency, and smaller batches keep the time to first pass of newly arrived jobs short."""
+    devices = {}
+    for mode in SCHEDULES:
         dev = api.Device(0, schedule=mode, hbm_budget_bytes=hbm_budget, max_jobs_in_flight=max_jobs,
                          use_graph=use_graph, path_capacity=path_capacity)
         bvhs = [dev.upload(s) for s in scenes]
-        summary, records, f = run_stream(dev, bvhs, scenes, requests, keep_films=keep_films)
-        out[mode] = summary
-        films[mode] = f
+        if warm:
+            run_stream(dev, bvhs, scenes, request_stream(count=8, rate=math.inf, seed=99, spp=(16, 64),
+                                                         scenes=len(scenes), cancel_fraction=0.0))
+        devices[mode] = (dev, bvhs)
+    return devices
+
+
+def close_devices(devices):
+    for dev, bvhs in devices.values():
         for b in bvhs:
             b.release()
         dev.close()
+
+
+def compare(requests=None, scenes=None, devices=None, keep_films=False, **device_options):
+    """Replays one request stream on the plain, plain-fill and batched schedules."""
+    scenes = scenes or scene_pool()
+    requests = requests or request_stream()
+    own = devices is None
+    devices = devices or open_devices(scenes, **device_options)
+    out, films = {}, {}
+    try:
+        for mode in SCHEDULES:
+            dev, bvhs = devices[mode]
+            summary, records, f = run_stream(dev, bvhs, scenes, requests, keep_films=keep_films)
+            out[mode] = summary
+            films[mode] = f
+    finally:
+        if own:
+            close_devices(devices)
     out["speedup"] = out["batched"]["msamples_per_s"] / out["plain"]["msamples_per_s"]
     out["speedup_vs_plain_fill"] = out["batched"]["msamples_per_s"] / out["plain_fill"]["msamples_per_s"]
     # No schedule can finish before the last request arrives: at a fixed arrival rate the largest
     # speedup over plain that this stream can show is plain's makespan over the last arrival time.
     out["last_arrival_s"] = max(r.arrival for r in requests)
-    out["speedup_ceiling"] = out["plain"]["makespan_s"] / out["last_arrival_s"]
+    out["speedup_ceiling"] = out["plain"]["makespan_s"] / max(out["last_arrival_s"], 1e-9)
     return out, films
+
+
+def sweep(rates=(40.0, 80.0, 160.0, 320.0, math.inf), count=64, seed=5, scenes=None, **device_options):
+    """Latency against throughput: the same request stream (same jobs and cancels) at each arrival rate,
+    on every schedule; rate inf submits every request at once (the saturated stream)."""
+    scenes = scenes or scene_pool()
+    devices = open_devices(scenes, **device_options)
+    rows = []
+    try:
+        for rate in rates:
+            out, _ = compare(request_stream(count=count, rate=rate, seed=seed, scenes=len(scenes)), scenes, devices)
+            rows.append(dict(rate=rate, **out))
+    finally:
+        close_devices(devices)
+    return rows

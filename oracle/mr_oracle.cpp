@@ -39,6 +39,7 @@
 namespace orc {
 
 constexpr double kPi = 3.14159265358979323846;
+constexpr uint32_t kNoSkip = 0xffffffffu;  // no primitive skipped by a query
 
 template <class R> constexpr R pi_v() { return static_cast<R>(kPi); }
 
@@ -519,7 +520,7 @@ template <class R> struct EnvMap {
 };
 
 template <class R> struct Lights {
-    struct Area { V3<R> v[3]; V3<R> normal; V3<R> radiance; R area; bool double_sided; };
+    struct Area { V3<R> v[3]; V3<R> normal; V3<R> radiance; R area; bool double_sided; uint32_t prim; };
     struct Dir { V3<R> direction; V3<R> radiance; };
     struct Choice { int kind; size_t index; };  // 0 area, 1 directional, 2 environment
     V3<R>               env;
@@ -818,6 +819,7 @@ struct Scene {
             a.radiance     = radiance;
             a.area         = area;
             a.double_sided = m.double_sided;
+            a.prim         = static_cast<uint32_t>(p);
             L.areas.push_back(a);
             weights.push_back(R(m.double_sided ? 2 : 1) * pi_v<R>() * area * luminance(radiance));
         }
@@ -869,9 +871,10 @@ struct Scene {
     }
 
     // Closest (or any) hit; with `second`, also the next-nearest primitive (t2 >= t1).
+    // `skip`: a scene-order primitive the query ignores (the emitter a FP32 shadow segment ends on).
     template <class R>
     std::optional<PrimHit<R>> trace(const Ray<R>& ray, R tmin, R tmax, bool any_hit, Counters* st,
-                                    std::optional<PrimHit<R>>* second = nullptr) const {
+                                    std::optional<PrimHit<R>>* second = nullptr, uint32_t skip = kNoSkip) const {
         if (!std::isfinite(tmin) || tmin < 0 || std::isnan(tmax) || tmax < tmin)
             throw std::invalid_argument("Ray interval must satisfy 0 <= minimum <= maximum");
         const RayQuery<R> q(ray);
@@ -909,6 +912,7 @@ struct Scene {
                         const V3<R> outward = cross(e1, e2) * (m.mirrored ? R(-1) : R(1));
                         if (dot(ray.d, outward) >= 0) continue;
                     }
+                    if (p == skip) continue;
                     if (second) {
                         if (!best || less(*hit, p, *best)) {
                             *second = best;
@@ -1362,6 +1366,7 @@ template <class R> struct LightSample {
     int   kind;
     size_t emitter;
     V3<R> position, normal, error;
+    uint32_t prim = kNoSkip;  // scene-order primitive of a sampled area emitter
 };
 
 template <class R> std::optional<LightSample<R>> sample_light(const Lights<R>& L, V3<R> ref, R light_u, V2<R> su) {
@@ -1392,7 +1397,7 @@ template <class R> std::optional<LightSample<R>> sample_light(const Lights<R>& L
     const V3<R> dir      = pt - ref;
     const R     distance = length(dir);
     const V3<R> err = ((vabs(b.x * a.v[0]) + vabs(b.y * a.v[1])) + vabs(b.z * a.v[2])) * (R(7) * std::numeric_limits<R>::epsilon());
-    out = {dir / distance, a.radiance, distance, sel->mass * density, false, 0, sel->index, pt, a.normal, err};
+    out = {dir / distance, a.radiance, distance, sel->mass * density, false, 0, sel->index, pt, a.normal, err, a.prim};
     return out;
 }
 
@@ -1426,6 +1431,19 @@ template <class R> void check_radiance(V3<R> v) {
         throw std::runtime_error("Path contribution is nonfinite or negative");
 }
 
+// FP32 twin deviation: a shadow segment toward an area emitter ignores that emitter's own triangle.
+// The reference ends the segment short of the light by the light point's position error
+// (7 eps |p|, light.cpp:310-313) and one ulp (path_tracer.cpp:126-151); that margin scales with the
+// light's coordinates, not with the segment's length, so for a light near a coordinate-zero plane
+// seen from farther away the rounding of the segment's own t lands past the limit: on the reference's
+// emitter case (path_tests.cpp:103-120) 24 of 1000 segments in FP64 and 37 of 1000 in FP32 report the
+// light itself as an occluder. The FP64 oracle keeps the reference's rule; the FP32 twin (and the
+// kernels, which reproduce it) skip the sampled triangle, which removes the self-occlusion exactly
+// and leaves every other occluder in place.
+template <class R> uint32_t shadow_skip(const LightSample<R>& light) {
+    return std::is_same_v<R, float> ? light.prim : kNoSkip;
+}
+
 template <class R>
 V3<R> shadow_transmittance(const Scene& s, Ray<R> ray, const LightSample<R>& light, PathCounters* st) {
     if (st) ++st->shadow;
@@ -1438,7 +1456,7 @@ V3<R> shadow_transmittance(const Scene& s, Ray<R> ray, const LightSample<R>& lig
         ray.d = diff / limit;
         limit = std::nextafter(limit, R(0));
     }
-    if (s.trace<R>(ray, R(0), limit, true, st ? &st->shadow_geom : nullptr)) return {0, 0, 0};
+    if (s.trace<R>(ray, R(0), limit, true, st ? &st->shadow_geom : nullptr, nullptr, shadow_skip(light))) return {0, 0, 0};
     return {1, 1, 1};
 }
 
@@ -1536,7 +1554,22 @@ V3<R> trace(const Scene& s, const Settings& cfg, Ray<R> ray, Sampler& sampler, R
 template <class R> struct CameraRays {
     V3<R> position, forward, right, up;
     R     half_height, aperture, focus, near_plane, far_plane;
+    // Camera::validate (scene.cpp:141-155), in float as the reference's glm::vec3 members.
+    static void validate(const OrcCamera& c) {
+        const V3<float> p{c.position[0], c.position[1], c.position[2]}, t{c.target[0], c.target[1], c.target[2]},
+            u{c.up[0], c.up[1], c.up[2]};
+        if (!(finite(p) && finite(t) && finite(u))) throw std::invalid_argument("Camera vectors must be finite");
+        if (!(length(t - p) > 1e-6f)) throw std::invalid_argument("Camera position and target must differ");
+        if (!(length(cross(t - p, u)) > 1e-6f)) throw std::invalid_argument("Camera up must not be parallel to view direction");
+        if (!(std::isfinite(c.vertical_fov_degrees) && c.vertical_fov_degrees > 0 && c.vertical_fov_degrees < 179))
+            throw std::invalid_argument("Camera FOV must be between 0 and 179 degrees");
+        if (!(std::isfinite(c.near_plane) && std::isfinite(c.far_plane) && c.near_plane > 0 && c.far_plane > c.near_plane))
+            throw std::invalid_argument("Camera clipping planes must satisfy 0 < near < far");
+        if (!(std::isfinite(c.aperture_radius) && c.aperture_radius >= 0 && std::isfinite(c.focus_distance) && c.focus_distance > 0))
+            throw std::invalid_argument("Camera lens requires nonnegative aperture and positive focus distance");
+    }
     explicit CameraRays(const OrcCamera& c) {
+        validate(c);
         position     = from_float<R>(c.position);
         forward      = normalize(from_float<R>(c.target) - from_float<R>(c.position));
         right        = normalize(cross(forward, from_float<R>(c.up)));
@@ -1632,11 +1665,45 @@ template <class R> static int surface_query(const Scene& s, const double* ray, d
     const auto   ph = s.trace<R>(r, R(0), std::numeric_limits<R>::infinity(), false, nullptr);
     if (!ph) return 0;
     const SurfaceHit<R> h = surface_hit(s, r, *ph);
-    const double v[11] = {double(h.position.x), double(h.position.y), double(h.position.z), h.geometric_normal.x,
+    const double v[14] = {double(h.position.x), double(h.position.y), double(h.position.z), h.geometric_normal.x,
                           h.geometric_normal.y, h.geometric_normal.z, h.shading_normal.x, h.shading_normal.y,
-                          h.shading_normal.z, h.uv.x, h.uv.y};
-    std::copy(v, v + 11, out);
+                          h.shading_normal.z, h.uv.x, h.uv.y, double(h.t), h.front_face ? 1.0 : 0.0, double(h.prim)};
+    std::copy(v, v + 14, out);
     return 1;
+}
+
+// The shadow segment PathTracer::trace hands to shadow_transmittance (path_tracer.cpp:126-151) for a light
+// sampled from `origin` (LightSampler::sample, light.cpp:266-322): origin, unit direction toward the offset
+// light endpoint, and the nextafter-shortened limit (+inf for infinite lights). out[0..5] ray, out[6] limit,
+// out[7] the sampled light distance, out[8] 1 when the segment is occluded in this precision.
+template <class R> static int segment_query(const Scene& s, const double* origin, double lu, const double* u, double* out) {
+    const V3<R> o{R(origin[0]), R(origin[1]), R(origin[2])};
+    const auto  l = sample_light(s.lights<R>(), o, R(lu), V2<R>{R(u[0]), R(u[1])});
+    if (!l) return 0;
+    Ray<R> ray{o, l->direction};
+    R      limit = std::numeric_limits<R>::infinity();
+    if (std::isfinite(l->distance)) {
+        const V3<R> target = offset_origin(l->position, to_float(l->normal), l->error, -l->direction);
+        const V3<R> diff   = target - ray.o;
+        limit              = length(diff);
+        if (limit == 0) return 0;
+        ray.d = diff / limit;
+        limit = std::nextafter(limit, R(0));
+    }
+    const bool blocked = s.trace<R>(ray, R(0), limit, true, nullptr, nullptr, shadow_skip(*l)).has_value();
+    const double v[9]  = {double(ray.o.x), double(ray.o.y), double(ray.o.z), double(ray.d.x), double(ray.d.y), double(ray.d.z),
+                          double(limit), double(l->distance), blocked ? 1.0 : 0.0};
+    std::copy(v, v + 9, out);
+    return 1;
+}
+
+extern "C" int orc_shadow_segment(void* sp, int precision, const double* origin, double lu, const double* u, double* out) {
+    int found = 0;
+    const int rc = guarded([&] {
+        const Scene& s = *static_cast<Scene*>(sp);
+        found = precision == 32 ? segment_query<float>(s, origin, lu, u, out) : segment_query<double>(s, origin, lu, u, out);
+    });
+    return rc ? -1 : found;
 }
 
 extern "C" int orc_surface_hit(void* sp, int precision, const double* ray, double* out) {

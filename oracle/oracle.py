@@ -108,6 +108,7 @@ def lib():
         L.orc_env_integral.argtypes = [P, C.c_int]
         L.orc_atan2_turns.argtypes = [P, P, C.c_uint64, P]
         L.orc_surface_hit.argtypes = [P, C.c_int, P, P]
+        L.orc_shadow_segment.argtypes = [P, C.c_int, P, C.c_double, P, P]
         L.orc_texture_sample.argtypes = [C.c_uint32, C.c_uint32, P, P, C.c_uint64, C.c_float, C.c_uint32, P]
         L.orc_texture_level.argtypes = [C.c_uint32, C.c_uint32, P, C.c_uint32, P, P, P]
         L.orc_integrate_ibl_brdf.argtypes = [C.c_double, C.c_double, C.c_uint32, C.c_uint64, P]
@@ -265,15 +266,30 @@ class OracleScene:
                     discrete=bool(out[8]), kind=int(out[9]), emitter=int(out[10]))
 
     def surface_hit(self, origin, direction, precision=64):
-        """SceneBvh::intersect + surface_hit: dict(position, geometric_normal, shading_normal, uv) or None."""
+        """SceneBvh::intersect + surface_hit: dict(position, geometric_normal, shading_normal, uv, distance,
+        front_face, primitive) or None."""
         ray = np.ascontiguousarray(np.concatenate([origin, direction]), np.float64)
-        out = np.zeros(11)
+        out = np.zeros(14)
         rc = lib().orc_surface_hit(self.handle, _prec(precision), _ptr(ray), _ptr(out))
         if rc < 0:
             raise ValueError(lib().orc_last_error().decode())
         if rc == 0:
             return None
-        return dict(position=out[0:3], geometric_normal=out[3:6], shading_normal=out[6:9], uv=out[9:11])
+        return dict(position=out[0:3], geometric_normal=out[3:6], shading_normal=out[6:9], uv=out[9:11],
+                    distance=out[11], front_face=bool(out[12]), primitive=int(out[13]))
+
+    def shadow_segment(self, origin, light_u, u, precision=64):
+        """The shadow segment the path tracer traces toward a light sampled from `origin`
+        (path_tracer.cpp:126-151): dict(ray (6,), limit, light_distance, occluded) or None."""
+        o = np.asarray(origin, np.float64)
+        uu = np.asarray(u, np.float64)
+        out = np.zeros(9)
+        rc = lib().orc_shadow_segment(self.handle, _prec(precision), _ptr(o), light_u, _ptr(uu), _ptr(out))
+        if rc < 0:
+            raise ValueError(lib().orc_last_error().decode())
+        if rc == 0:
+            return None
+        return dict(ray=out[0:6].copy(), limit=float(out[6]), light_distance=float(out[7]), occluded=bool(out[8]))
 
     def env_eval(self, directions, precision=64):
         """EnvironmentLight::evaluate and ::pdf of unit directions [n][3]."""
