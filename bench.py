@@ -4,11 +4,13 @@ One step = one complete render job of the workload (every pixel, every sample) t
 plain (one job at a time) schedule. Inputs (the scene) are resident in HBM before the timed region;
 the timed region is submit -> job complete on the device.
 
-Prints one JSON line with the throughput, a roofline entry for the dominant kernel (k_extend,
-closest-hit traversal; algorithmic bytes per ray from the FP64 oracle's traversal counters on the
-reference BVH topology) and a CPU baseline (the FP64 oracle on the host cores, bounded sample).
+Prints one JSON line with the throughput, a roofline entry for the dominant kernel (the bounce traversal
+launch; VALU-issue bound, quoted against the FP32 peak with the useful box / triangle test operations per
+ray of the FP64 oracle on the reference BVH, and with its measured HBM and L2 fractions), the served
+request stream at several arrival rates (plain vs batched, p50/p99 latencies) and the CPU baseline (the
+FP64 oracle built -march=native, on the host's CPU share and at the reference's default thread count).
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload mixed|cornell|intersect]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload mixed|cornell|intersect|batch|...]
 """
 from __future__ import annotations
 
@@ -24,7 +26,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3          # MI355X published FP32 vector peak (packed FP32 issue)
+FP32_UNPACKED_TFLOPS = 78.6       # the same without packed FP32 (the kernels are built without it)
 B_BOX, B_TRI, B_RAY_IN, B_HIT_OUT, B_OCCLUDED_OUT, B_FILM = 24, 36, 24, 8, 4, 12
+FLOP_BOX, FLOP_TRI = 15, 30       # useful FP32 operations of one slab box test / one watertight triangle test
 
 WORKLOADS = {
     # 2M-triangle fBm terrain + 64 spheres, sun + sky, thin lens f/2.8
@@ -63,6 +68,77 @@ def read_profile_value(name, key):
 def read_traffic(kernel, workload):
     """HBM bytes per launch measured by rocprofv3 --pmc (see profiles/), or None."""
     return read_profile_value(f"{kernel}_{workload}_traffic.txt", "traffic_bytes_per_launch")
+
+
+def host_cpu():
+    """CPU model and core counts of the host this process runs on: std::thread::hardware_concurrency
+    (logical CPUs), the CPUs this process may run on (affinity) and the cgroup CPU quota, if any."""
+    import subprocess
+    model = None
+    try:
+        for line in subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    if not model and os.path.exists("/proc/cpuinfo"):
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    hw = os.cpu_count() or 1
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else hw
+    quota = None
+    try:
+        q, period = open("/sys/fs/cgroup/cpu.max").read().split()
+        if q != "max":
+            quota = int(q) / int(period)
+    except (OSError, ValueError):
+        pass
+    share = max(1, min(usable, int(quota))) if quota else usable
+    return {"model": model, "hardware_concurrency": hw, "affinity_cpus": usable, "cgroup_cpu_quota": quota,
+            "cpu_share": share, "reference_default_threads": max(1, min(hw - 1, 32))}
+
+
+def tests_per_ray(st):
+    """Box and triangle tests per extension / shadow ray of the FP64 oracle (reference BVH topology)."""
+    return ((st["bounds_tests"] - st["shadow_bounds_tests"]) / max(st["extension_rays"], 1),
+            (st["triangle_tests"] - st["shadow_triangle_tests"]) / max(st["extension_rays"], 1),
+            st["shadow_bounds_tests"] / max(st["shadow_rays"], 1), st["shadow_triangle_tests"] / max(st["shadow_rays"], 1))
+
+
+def roofline_entry(kernel, workload, avg_s, ext_rays, sh_rays, eb, et, sb, stt, hit_bytes=B_HIT_OUT):
+    """Roofline of a traversal launch. The kernel is bound by VALU issue (its rocprofv3 counters show the VALU
+    busy in nearly every cycle, see profiles/), so the entry is quoted against the FP32 peak: achieved =
+    useful FP32 operations per launch (15 per box test, 30 per triangle test, the FP64 oracle's tests per ray on
+    the reference BVH) / the launch's average duration. The HBM and L2 fractions use the bytes the counters
+    measured; the reference-BVH gather bytes of the algorithmic model are kept as their own fields."""
+    flops = ext_rays * (FLOP_BOX * eb + FLOP_TRI * et) + sh_rays * (FLOP_BOX * sb + FLOP_TRI * stt)
+    alg = ext_rays * (B_RAY_IN + hit_bytes + B_BOX * eb + B_TRI * et) + sh_rays * (B_RAY_IN + B_OCCLUDED_OUT + B_BOX * sb + B_TRI * stt)
+    achieved = flops / avg_s / 1e12
+    e = {"kernel": kernel, "bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+         "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+         "frac_of_unpacked_fp32_peak": round(achieved / FP32_UNPACKED_TFLOPS, 4),
+         "traffic": read_traffic(kernel, workload), "avg_launch_ms": round(avg_s * 1e3, 4),
+         "rays_per_launch": round(ext_rays + sh_rays, 1), "useful_flops_per_launch": round(flops),
+         "bounds_tests_per_extension_ray": round(eb, 2), "triangle_tests_per_extension_ray": round(et, 2),
+         "bounds_tests_per_shadow_ray": round(sb, 2), "triangle_tests_per_shadow_ray": round(stt, 2),
+         "algorithmic_bytes_per_launch": round(alg), "algorithmic_gb_per_s": round(alg / avg_s / 1e9, 1)}
+    prof = "path_kernel_counters_mixed1080p.txt"
+    for key in ("valu_busy", "valu_lane_utilisation", "occupancy"):
+        v = read_profile_value(prof, f"{kernel}_{key}")
+        if v is not None:
+            e[key] = v
+    if e["traffic"]:
+        hbm = e["traffic"] / avg_s / 1e9
+        e.update({"hbm_achieved_gb_per_s": round(hbm, 1), "hbm_peak_gb_per_s": HBM_PEAK_GBS, "hbm_frac": round(hbm / HBM_PEAK_GBS, 4)})
+    l2_bytes = read_profile_value(f"{kernel}_{workload}_traffic.txt", "l2_bytes_per_launch")
+    l2_peak = read_profile_value("memory_hierarchy_probe.txt", "l2_read_gb_per_s")
+    if l2_bytes and l2_peak:
+        l2 = l2_bytes / avg_s / 1e9
+        e.update({"l2_bytes_per_launch": round(l2_bytes), "l2_achieved_gb_per_s": round(l2, 1), "l2_peak_measured_gb_per_s": l2_peak,
+                  "l2_frac": round(l2 / l2_peak, 4)})
+    return e
 
 
 def cpu_baseline(scene, w, threads):
@@ -141,79 +217,114 @@ def bench_render(args):
     }
 
     bvh.release()
+    if not args.no_cpu and not args.graph:
+        result["config1_gpu_msamples_per_s"] = round(cornell_gpu_rate(dev), 1)
     dev.close()
     if not args.no_serving:
-        result["serving"] = bench_serving()
+        result["serving"] = bench_serving(args)
 
-    # CPU baseline + per-ray algorithmic bytes from the FP64 oracle counters
-    threads = min(16, os.cpu_count() or 1)
     cpu = None
     if not args.no_cpu and args.graph:
         log("[bench] --graph: kernel times are not measured, so no roofline entry; CPU baseline skipped")
     elif not args.no_cpu:
-        log(f"[bench] cpu baseline: FP64 oracle, {threads} threads")
-        st, dt, rows = cpu_baseline(scene, w, threads)
-        cpu = {"value": round(st["paths"] / dt / 1e6, 4), "unit": "Msamples/s", "cores": threads, "kind": "port",
-               "sample": f"rows {rows[0]}-{rows[1] - 1} ({(rows[1] - rows[0]) * w['width']} px) at {w['spp']} spp, "
-                         f"FP64 oracle, {dt:.1f}s",
-               "gpu_over_cpu": round(value / (st["paths"] / dt / 1e6), 1)}
-        # algorithmic bytes per ray: ray in + result out + 24 B per box test + 36 B per triangle test,
-        # tests per ray from the FP64 oracle (reference BVH topology) on the CPU sample
-        ext_bounds = (st["bounds_tests"] - st["shadow_bounds_tests"]) / st["extension_rays"]
-        ext_tris = (st["triangle_tests"] - st["shadow_triangle_tests"]) / st["extension_rays"]
-        sh_bounds = st["shadow_bounds_tests"] / max(st["shadow_rays"], 1)
-        sh_tris = st["shadow_triangle_tests"] / max(st["shadow_rays"], 1)
-        b_ray = B_RAY_IN + B_HIT_OUT + B_BOX * ext_bounds + B_TRI * ext_tris
-        b_shadow = B_RAY_IN + B_OCCLUDED_OUT + B_BOX * sh_bounds + B_TRI * sh_tris
+        cpu, st = cpu_baselines(scene, w, value, result.get("config1_gpu_msamples_per_s"))
         fused = kt["shadow_launches"] == 0  # one traversal kernel per bounce traces both kinds of ray
         refill = fused and kt["refill_launches"] == kt["extend_launches"]
         kernel = ("k_trace_refill" if refill else "k_trace") if fused else "k_extend"
         launches = max(kt["extend_launches"], 1)
-        rays = kt["extension_rays"] + (kt["shadow_rays"] if fused else 0)
-        per_launch_bytes = (kt["extension_rays"] * b_ray + (kt["shadow_rays"] * b_shadow if fused else 0)) / launches
         avg_s = kt["extend_ms"] / launches / 1e3
-        achieved = per_launch_bytes / avg_s / 1e9 if avg_s > 0 else None
-        traffic = read_traffic(kernel, args.workload)
-        result["roofline"] = {"kernel": kernel, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                              "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                              "bytes_per_extension_ray": round(b_ray, 1), "bytes_per_shadow_ray": round(b_shadow, 1),
-                              "bounds_tests_per_extension_ray": round(ext_bounds, 2),
-                              "triangle_tests_per_extension_ray": round(ext_tris, 2),
-                              "bounds_tests_per_shadow_ray": round(sh_bounds, 2),
-                              "triangle_tests_per_shadow_ray": round(sh_tris, 2),
-                              "avg_launch_ms": round(avg_s * 1e3, 4), "rays_per_launch": round(rays / launches, 1),
-                              "algorithmic_bytes_per_launch": round(per_launch_bytes)}
-        # the same kernel against the L2: bytes through the L2 per launch (rocprofv3 TCC_REQ x 128 B) over the
-        # measured L2 read bandwidth (profiles/memory_hierarchy_probe.txt)
-        l2_bytes = read_profile_value(f"{kernel}_{args.workload}_traffic.txt", "l2_bytes_per_launch")
-        l2_peak = read_profile_value("memory_hierarchy_probe.txt", "l2_read_gb_per_s")
-        if l2_bytes and l2_peak and avg_s > 0:
-            l2_gbps = l2_bytes / avg_s / 1e9
-            result["roofline"].update({"l2_bytes_per_launch": round(l2_bytes), "l2_achieved": round(l2_gbps, 1),
-                                       "l2_peak_measured": l2_peak, "l2_frac": round(l2_gbps / l2_peak, 4)})
+        eb, et, sb, stt = tests_per_ray(st)
+        if avg_s > 0:
+            result["roofline"] = roofline_entry(kernel, args.workload, avg_s, kt["extension_rays"] / launches,
+                                                (kt["shadow_rays"] if fused else 0) / launches, eb, et, sb, stt)
     result["cpu_baseline"] = cpu
     return result
 
 
-def bench_serving():
-    """Served request stream: Poisson arrivals (40 jobs/s, 64 uneven jobs over 8 resident scenes, 64 GB HBM
-    budget, 16-spp progressive passes, 5% cancelled) replayed on the plain and the batched schedule."""
+def cornell_gpu_rate(dev):
+    """GPU Msamples/s of the Cornell box workload (512^2, 64 spp, depth 8), one warm and one timed render."""
+    from minirender_amd import api
+    w = WORKLOADS["cornell"]
+    scene = make_scene("cornell")
+    rs, ins = api.RenderSettings(w["width"], w["height"], w["spp"], w["seed"]), api.IntegratorSettings("mis", w["max_depth"], True, 3)
+    api.render(dev, scene, rs, ins)
+    t = time.perf_counter()
+    api.render(dev, scene, rs, ins)
+    return w["width"] * w["height"] * w["spp"] / (time.perf_counter() - t) / 1e6
+
+
+def cpu_baselines(scene, w, gpu_value, cornell_gpu):
+    """The FP64 oracle (the reference CPU path tracer restated: same RNG streams, 16^2 tiles, 4-sample
+    batches), built -march=native on this host, timed at the reference's default thread count
+    min(hardware_concurrency - 1, 32) (render_session.cpp:389-392) and at the host's CPU share, on a
+    fixed band of image rows at full spp; plus the whole Cornell-box image at both thread counts."""
+    from oracle import oracle as O
+    t = time.perf_counter()
+    O.use_native()
+    build_s = time.perf_counter() - t
+    host = host_cpu()
+    counts = sorted({host["reference_default_threads"], host["cpu_share"]})
+    log(f"[bench] cpu baseline: {host['model']}, hardware_concurrency {host['hardware_concurrency']}, "
+        f"share {host['cpu_share']} CPUs; threads {counts}")
+    runs, st = {}, None
+    for threads in counts:
+        st, dt, rows = cpu_baseline(scene, w, threads)
+        runs[threads] = (st["paths"] / dt / 1e6, dt)
+        log(f"[bench] cpu {threads} threads: {runs[threads][0]:.3f} Msamples/s ({dt:.1f}s)")
+    cornell = make_scene("cornell")
+    cw = WORKLOADS["cornell"]
+    oc = O.OracleScene(cornell)
+    c_runs = {}
+    for threads in counts:
+        t = time.perf_counter()
+        _, cst = oc.render(cornell.camera, cw["width"], cw["height"], cw["spp"], seed=cw["seed"], max_depth=cw["max_depth"],
+                           precision=64, threads=threads)
+        dt = time.perf_counter() - t
+        c_runs[threads] = (cst["paths"] / dt / 1e6, dt)
+    share = host["cpu_share"]
+    value = runs[share][0]
+    cpu = {"value": round(value, 4), "unit": "Msamples/s", "cores": share, "kind": "port",
+           "sample": f"rows {rows[0]}-{rows[1] - 1} ({(rows[1] - rows[0]) * w['width']} px) of the {w['width']}x{w['height']} "
+                     f"workload at {w['spp']} spp, FP64 oracle built -march=native ({build_s:.0f}s build), {runs[share][1]:.1f}s",
+           "gpu_over_cpu": round(gpu_value / value, 1),
+           "host": host,
+           "by_threads": {str(k): {"msamples_per_s": round(v[0], 4), "seconds": round(v[1], 2)} for k, v in runs.items()},
+           "gpu_over_cpu_reference_default_threads": round(gpu_value / runs[host["reference_default_threads"]][0], 1),
+           "config1_cornell_512x512_64spp": {
+               "by_threads": {str(k): {"msamples_per_s": round(v[0], 3), "seconds": round(v[1], 2)} for k, v in c_runs.items()},
+               "gpu_msamples_per_s": cornell_gpu,
+               "gpu_over_cpu": round(cornell_gpu / c_runs[share][0], 1) if cornell_gpu else None}}
+    return cpu, st
+
+
+def bench_serving(args):
+    """Served request stream (config 5): 64 uneven jobs over 8 resident scenes, 64 GB HBM budget, 16-spp
+    progressive passes, 5% of the jobs cancelled after 1-3 passes, replayed at each arrival rate of
+    --serve-rates (Poisson; inf = every request at once, the saturated stream) on the plain (one job at a
+    time), plain-fill and batched schedules: latency against throughput."""
+    import math
     from minirender_amd import serve
-    log("[bench] serving: building the 8 resident scenes and the request stream")
-    scenes = serve.scene_pool()
-    reqs = serve.request_stream()
-    out, _ = serve.compare(reqs, scenes)
+    rates = [math.inf if r == "inf" else float(r) for r in args.serve_rates.split(",")]
+    log(f"[bench] serving: building the 8 resident scenes; rates {rates}")
+    rows = serve.sweep(rates)
     keep = ("completed", "cancelled", "failed", "samples", "makespan_s", "msamples_per_s", "first_pass_p50_s",
             "first_pass_p99_s", "completion_p50_s", "completion_p99_s")
-    res = {m: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in out[m].items() if k in keep}
-           for m in ("plain", "plain_fill", "batched")}
-    res["speedup"] = round(out["speedup"], 3)
-    res["speedup_vs_plain_fill"] = round(out["speedup_vs_plain_fill"], 3)
-    res["last_arrival_s"] = round(out["last_arrival_s"], 4)
-    res["speedup_ceiling"] = round(out["speedup_ceiling"], 3)  # plain makespan / last arrival
-    res["stream"] = "64 jobs, Poisson 40/s, side 128-1024, spp 16-1024, depth 8-16, 8 scenes (36-408k tris), 5% cancel"
-    log(f"[bench] serving: plain {res['plain']['msamples_per_s']:.1f} Ms/s, batched {res['batched']['msamples_per_s']:.1f} Ms/s")
-    return res
+    out = []
+    for row in rows:
+        r = {"rate_jobs_per_s": row["rate"] if math.isfinite(row["rate"]) else "inf"}
+        for m in serve.SCHEDULES:
+            r[m] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in row[m].items() if k in keep}
+        r["speedup"] = round(row["speedup"], 3)
+        r["speedup_vs_plain_fill"] = round(row["speedup_vs_plain_fill"], 3)
+        r["last_arrival_s"] = round(row["last_arrival_s"], 4)
+        r["speedup_ceiling"] = round(row["speedup_ceiling"], 3) if math.isfinite(row["rate"]) else None
+        log(f"[bench] serving rate {r['rate_jobs_per_s']}: plain {r['plain']['msamples_per_s']:.1f}, plain_fill "
+            f"{r['plain_fill']['msamples_per_s']:.1f}, batched {r['batched']['msamples_per_s']:.1f} Msamples/s")
+        out.append(r)
+    return {"stream": "64 jobs, Poisson arrivals, side 128-1024, spp 16-1024, depth 8-16, 8 scenes (36-410k tris), "
+                      "5% cancelled after 1-3 passes, 16-spp passes, 2^26-path pool, 64 GB budget",
+            "rates": out,
+            "saturated_speedup": out[-1]["speedup"] if out and out[-1]["rate_jobs_per_s"] == "inf" else None}
 
 
 def bench_intersect(args):
@@ -255,7 +366,9 @@ def bench_intersect(args):
               "pcie_inclusive_mrays_per_s": round(n / pcie_s / 1e6, 1)}
     if not args.no_cpu:
         from oracle import oracle as O
-        threads = min(16, os.cpu_count() or 1)
+        O.use_native()
+        host = host_cpu()
+        threads = host["cpu_share"]
         o = O.OracleScene(scene)
         idx = np.arange(0, n, args.cpu_stride)
         sample = rays[idx].astype(np.float64)
@@ -264,32 +377,27 @@ def bench_intersect(args):
         dt = time.perf_counter() - t
         b = ref["stats"]["bounds_tests"] / len(idx)
         tt = ref["stats"]["triangle_tests"] / len(idx)
-        b_ray = B_RAY_IN + B_HIT_OUT + B_BOX * b + B_TRI * tt
-        achieved = n * b_ray / (avg / 1e3) / 1e9
         with np.errstate(invalid="ignore"):
             near = np.abs(ref["t2"] - ref["t"]) <= 1e-6 * np.abs(ref["t"])
         mism = hits["primitive"][idx] != ref["prim"]
         gp, gt = hits["primitive"][idx], hits["t"][idx]
         edge = np.zeros(len(idx), bool)
-        for i in np.where(mism & (gp >= 0) & (ref["prim"] >= 0))[0]:
+        for i in np.where(mism & ~near & (gp >= 0) & (ref["prim"] >= 0))[0]:
             if abs(float(gt[i]) - ref["t"][i]) <= 1e-6 * ref["t"][i]:
                 va = {tuple(v) for v in scene.positions[gp[i]].tolist()}
                 edge[i] = bool(va & {tuple(v) for v in scene.positions[ref["prim"][i]].tolist()})
         both = (~mism) & (ref["prim"] >= 0)
         rel = np.abs(hits["t"][idx][both] - ref["t"][both]) / ref["t"][both]
         result["parity"] = {"sample_rays": int(len(idx)), "id_mismatch": int(mism.sum()),
-                            "near_ties": int(near.sum()), "shared_edge_ties": int(edge.sum()),
-                            "id_mismatch_outside_near_ties": int((mism & ~near & ~edge).sum()),
+                            "near_ties": int(near.sum()), "id_mismatch_outside_near_ties": int((mism & ~near).sum()),
+                            "of_which_shared_edge_crossings": int(edge.sum()),
                             "max_t_rel": float(rel.max(initial=0))}
-        result["roofline"] = {"kernel": "k_query", "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                              "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                              "traffic": read_traffic("k_query", "intersect"),
-                              "bytes_per_ray": round(b_ray, 1), "bounds_tests_per_ray": round(b, 2),
-                              "triangle_tests_per_ray": round(tt, 2), "avg_launch_ms": round(avg, 3)}
+        result["roofline"] = roofline_entry("k_query", "intersect", avg / 1e3, n, 0, b, tt, 0, 0)
         result["cpu_baseline"] = {"value": round(len(idx) / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads,
-                                  "kind": "port", "sample": (f"all {len(idx)} rays" if args.cpu_stride == 1 else
-                                             f"every {args.cpu_stride}-th ray ({len(idx)} rays)") + ", FP64 oracle closest "
-                                  f"+ second hit, {dt:.1f}s"}
+                                  "kind": "port", "host": host,
+                                  "sample": (f"all {len(idx)} rays" if args.cpu_stride == 1 else
+                                             f"every {args.cpu_stride}-th ray ({len(idx)} rays)") + ", FP64 oracle (-march=native) "
+                                  f"closest + second hit, {dt:.1f}s"}
     bvh.release()
     dev.close()
     return result
@@ -303,7 +411,7 @@ def bench_convergence(args):
     w = WORKLOADS["cornell"]
     scene = scenegen.cornell_box()
     crop = (240, 200, 272, 232)
-    threads = min(16, os.cpu_count() or 1)
+    threads = host_cpu()["cpu_share"]
     o = O.OracleScene(scene)
     t = time.perf_counter()
     ref, _ = o.render(scene.camera, w["width"], w["height"], 65536, seed=99, max_depth=8, precision=64,
@@ -413,6 +521,8 @@ def main():
                                                                  "cache"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serve-rates", default="40,80,160,320,inf",
+                    help="arrival rates (jobs/s) of the served stream, comma-separated; inf = all at once")
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--path-capacity", type=int, default=0, help="paths in flight per batch (0 = library default)")
     ap.add_argument("--traversal", default="auto", choices=["auto", "split", "fused", "refill"])

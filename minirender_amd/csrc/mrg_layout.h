@@ -167,8 +167,13 @@ struct GpuJob {
     uint32_t    pad;
 };
 
-// A contiguous range of samples for a contiguous range of pixels of one job. Path slot i of the
-// chunk maps to sample = sample_begin + i / pixel_count, pixel = pixel_begin + i % pixel_count.
+// A contiguous range of samples for a contiguous range of pixels of one job. The kernels map a path
+// slot of the chunk to its (pixel, sample) with chunk_slot / chunk_sample / chunk_xy: a chunk of whole
+// image rows (tile_rows != 0) holds its samples in groups of 4, each group pixel-major with the 4
+// samples of a pixel side by side and the pixels in 4x4 tiles (tile rows top to bottom, tiles left to
+// right, the remainder rows and columns in the last tile row and column); samples past the last whole
+// group follow one at a time in the same pixel order. Any other chunk is sample-major in scanline
+// order: slot i -> sample = sample_begin + i / pixel_count, pixel = pixel_begin + i % pixel_count.
 struct GpuChunk {
     uint32_t job;
     uint32_t pixel_begin;
@@ -177,7 +182,7 @@ struct GpuChunk {
     uint32_t sample_count;
     uint32_t slot_begin;    // first path slot of this chunk in the pool
     uint32_t pixel_prefix;  // pixels of all earlier chunks of the batch
-    uint32_t tile_rows;     // image rows of a chunk of whole rows (pixels in 8x8 tiles), else 0 (scanline order)
+    uint32_t tile_rows;     // image rows of a chunk of whole rows (4x4 pixel tiles x 4 samples), else 0 (scanline order)
 };
 
 }  // namespace mrg

@@ -84,7 +84,7 @@ TERMINAL = ("cancelled", "completed", "failed", "idle")
 def run_stream(device: api.Device, bvhs, scenes, requests, pass_samples=16, poll=0.001, keep_films=False):
     """Submits each request at its arrival time, cancels the selected ones once they have recorded
     their pass count, waits for all of them and returns (summary, per-job records, films)."""
-    jobs = [None] * len(requests)
+    jobs = []
     watch, lock, done_submitting = {}, threading.Lock(), threading.Event()
 
     def canceller():
@@ -113,7 +113,7 @@ def run_stream(device: api.Device, bvhs, scenes, requests, pass_samples=16, poll
             sc = scenes[req.scene]
             job = api.Job(bvhs[req.scene], sc.camera, api.RenderSettings(req.width, req.height, req.spp, req.seed),
                           api.IntegratorSettings("mis", req.max_depth, True, 3), pass_samples)
-            jobs[req.index] = job
+            jobs.append(job)
             if req.cancel_after_passes >= 0:
                 with lock:
                     watch[req.index] = (job, req.cancel_after_passes * pass_samples * req.width * req.height)

@@ -15,8 +15,21 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "build", "liboracle.so")
 
 
+NATIVE_PATH = os.path.join(HERE, "build", "liboracle_native.so")
+
+
 def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def use_native():
+    """Builds the oracle with -march=native for the host this process runs on and loads that build
+    instead of the portable one (the timed CPU baseline). Call before the first use of the oracle."""
+    global LIB_PATH
+    if _lib is not None:
+        raise RuntimeError("the oracle library is already loaded")
+    subprocess.run(["make", "-s", "-B", "-C", HERE, "native"], check=True)
+    LIB_PATH = NATIVE_PATH
 
 
 class _SceneDesc(C.Structure):

@@ -1,5 +1,11 @@
 """GPU closest-hit / any-hit parity against the oracle (FP32 twin: bit-exact; FP64 reference:
-hit ids equal outside near-ties, t within 1e-5 relative)."""
+hit ids equal outside near-ties, t within 1e-5 relative).
+
+Against FP64 the id rule has measured exceptions outside near-ties: rays that cross the edge shared by
+two triangles, where the FP32 watertight test (which the kernels and the FP32 twin compute identically)
+puts the crossing on the other side of the edge than FP64 does. They are counted as mismatches, every
+one is checked to be such a crossing, and their rate is bounded by SHARED_EDGE_RATE (41 of the 16.7M
+microbench rays = 2.4e-6 measured)."""
 import numpy as np
 import pytest
 
@@ -9,10 +15,11 @@ pytestmark = pytest.mark.gpu
 
 NEAR_TIE = 1e-6   # |t2 - t1| <= 1e-6 * t1 counts as a near-tie
 T_REL = 1e-5
+SHARED_EDGE_RATE = 4e-6  # bound on id mismatches outside near-ties (all shared-edge crossings)
 
 
-def shared_edge_tie(scene, prim_a, t_a, prim_b, t_b):
-    """Mismatches where both precisions hit adjacent triangles (sharing a vertex) at t within
+def shared_edge_crossing(scene, prim_a, t_a, prim_b, t_b):
+    """Mismatches where the two precisions hit adjacent triangles (sharing a vertex) at t within
     NEAR_TIE relative: the ray crosses the common edge, and each precision assigns it to one side."""
     out = np.zeros(len(prim_a), bool)
     for i in np.where((prim_a != prim_b) & (prim_a >= 0) & (prim_b >= 0))[0]:
@@ -42,11 +49,12 @@ def _compare(device, oracle, scene, rays, tmin=0.0, tmax=np.inf):
     with np.errstate(invalid="ignore"):
         near_tie = np.isfinite(t2) & (np.abs(t2 - t1) <= NEAR_TIE * np.abs(t1))
     mismatch = gpu["primitive"] != ref["prim"]
-    edge_tie = mismatch & shared_edge_tie(scene, gpu["primitive"], gpu["t"], ref["prim"], t1)
-    bad = mismatch & ~near_tie & ~edge_tie
+    outside = mismatch & ~near_tie
+    crossing = outside & shared_edge_crossing(scene, gpu["primitive"], gpu["t"], ref["prim"], t1)
     print(f"  hits={hit.sum()} near_ties={near_tie.sum()} id_mismatch={mismatch.sum()} "
-          f"shared_edge_ties={edge_tie.sum()} outside_near_ties={bad.sum()}")
-    assert bad.sum() == 0
+          f"outside_near_ties={outside.sum()} (shared-edge crossings {crossing.sum()})")
+    assert (outside == crossing).all(), "an id mismatch outside near-ties that is not a shared-edge crossing"
+    assert outside.sum() <= max(3, SHARED_EDGE_RATE * len(rays))
     both = (gpu["primitive"] == ref["prim"]) & (ref["prim"] >= 0)
     rel = np.abs(gpu["t"][both] - t1[both]) / np.abs(t1[both])
     assert rel.max(initial=0) <= T_REL

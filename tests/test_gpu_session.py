@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from minirender_amd import _native as N, api, scenegen
-from minirender_amd.scene import concat
+from minirender_amd.scene import Camera, concat
 
 pytestmark = pytest.mark.gpu
 
@@ -176,3 +176,55 @@ def test_results_identical_alone_and_in_the_mix():
         for (s0, c0), (s1, c1) in zip(films["plain", "auto"], films[other]):
             np.testing.assert_array_equal(c0, c1)
             np.testing.assert_array_equal(s0, s1)
+
+
+def test_close_with_a_paused_job_outstanding():
+    """Closing a device cancels its unfinished jobs and wakes their waiters; the caller's job and scene
+    handles stay valid: progress and the recorded film are still readable, release works, and ray
+    queries on the closed device fail cleanly."""
+    d = api.Device(0, path_capacity=1 << 18)
+    bvh = d.upload(sky())
+    job = api.Job(bvh, Camera(), api.RenderSettings(64, 40, 1000000, 5), api.IntegratorSettings(), pass_samples=4)
+    wait_until(lambda: job.progress().recorded_samples > 0)
+    assert job.pause()
+    wait_until(lambda: job.progress().state == "paused")
+    recorded = job.progress().recorded_samples
+    print(f"[close] paused at {recorded} samples; closing the device", flush=True)
+    d.close()
+    assert job.wait(1000)
+    p = job.progress()
+    assert p.state == "cancelled" and p.recorded_samples == recorded
+    sums, counts = job.film()
+    assert int(counts.sum()) == recorded
+    with pytest.raises(N.InvalidArgument):
+        bvh.occluded(np.array([[0, 0, 0, 0, 0, 1]], np.float32))
+    job.release()
+    bvh.release()
+
+
+def test_finished_films_leave_the_admission_budget():
+    """A budget that holds two films at a time: six jobs submitted together, every handle kept until all
+    finish. Finished films leave HBM (their results stay readable), so the later jobs are admitted as the
+    earlier ones complete instead of waiting on handles the caller still holds."""
+    scene = scenegen.cornell_box()
+    film = 128 * 128 * 28
+    with api.Device(0, path_capacity=1 << 16) as probe:
+        b0 = probe.upload(scene)
+        m = probe.memory()
+        b0.release()
+    budget = m["pool_bytes"] + m["resident_bytes"] + 2 * film + film // 2
+    with api.Device(0, schedule="batched", path_capacity=1 << 16, hbm_budget_bytes=budget) as d:
+        bvh = d.upload(scene)
+        jobs = [api.Job(bvh, scene.camera, api.RenderSettings(128, 128, 8, i), api.IntegratorSettings("mis", 4), 4)
+                for i in range(6)]
+        for j in jobs:
+            assert j.wait(20000)
+        ps = [j.progress() for j in jobs]
+        assert all(p.state == "completed" for p in ps)
+        assert max(p.queued_seconds for p in ps) > 0
+        assert d.memory()["resident_bytes"] == m["resident_bytes"]
+        ref = api.render(d, scene, api.RenderSettings(128, 128, 8, 3), api.IntegratorSettings("mis", 4))[0]
+        np.testing.assert_array_equal(jobs[3].image(), ref)
+        for j in jobs:
+            j.release()
+        bvh.release()

@@ -5,6 +5,7 @@ known pixel arrays with every filter type, bit depth and colour type the decoder
 import os
 import struct
 import subprocess
+import tempfile
 import zlib
 
 import numpy as np
@@ -16,7 +17,16 @@ GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
 
 def decode(path, encoding="srgb", check=True):
-    out = str(path) + ".bin"
+    """Decodes a PNG with the command line into a temporary file (never beside a committed fixture)."""
+    fd, out = tempfile.mkstemp(suffix=".bin")
+    os.close(fd)
+    try:
+        return _decode(path, encoding, check, out)
+    finally:
+        os.remove(out)
+
+
+def _decode(path, encoding, check, out):
     r = subprocess.run([B.RENDER_CLI, "--texture", str(path), encoding, out], capture_output=True, text=True, timeout=60)
     if not check:
         return r
