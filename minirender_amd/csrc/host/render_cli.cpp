@@ -13,18 +13,21 @@
 //   mrg_render --scene-info scene.json [flat.bin]   triangle / instance counts, optional dump
 //   mrg_render --obj mesh.obj mesh.bin              OBJ loader output
 //   mrg_render --encode image.bin output-prefix [--exposure EV]   EXR + PNG of a raw image
-#include <charconv>
+#include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <functional>
 #include <iostream>
-#include <locale>
-#include <sstream>
+#include <limits>
 #include <stdexcept>
 #include <string>
 #include <string_view>
 #include <type_traits>
+#include <utility>
+#include <vector>
 
 #include "image_io.hpp"
 #include "mrg_session.hpp"
@@ -34,21 +37,51 @@ namespace {
 
 namespace gpu = minirender::gpu;
 
+// Strict numeric option values: the whole token must be the number, in the C locale, with no
+// leading blank or sign for counts. A malformed value is an input error naming the token.
+[[noreturn]] void bad_number(std::string_view text) {
+    throw std::invalid_argument("Invalid numeric render option: " + std::string(text));
+}
+
+// Unsigned decimal count: digits only, no overflow of T.
 template <typename T>
-T number(std::string_view text) {
-    T    result{};
-    bool valid = false;
-    if constexpr (std::is_floating_point_v<T>) {
-        std::istringstream input{std::string(text)};
-        input.imbue(std::locale::classic());
-        input >> std::noskipws >> result;
-        valid = static_cast<bool>(input) && input.peek() == std::char_traits<char>::eof();
-    } else {
-        const auto parsed = std::from_chars(text.data(), text.data() + text.size(), result);
-        valid             = parsed.ec == std::errc{} && parsed.ptr == text.data() + text.size();
+T parse_count(std::string_view text) {
+    static_assert(std::is_unsigned_v<T>, "counts are unsigned");
+    if (text.empty()) bad_number(text);
+    T value = 0;
+    for (const char c : text) {
+        if (c < '0' || c > '9') bad_number(text);
+        const T digit = T(c - '0');
+        if (value > (std::numeric_limits<T>::max() - digit) / 10) bad_number(text);
+        value = T(value * 10 + digit);
     }
-    if (!valid) throw std::invalid_argument("Invalid numeric render option: " + std::string(text));
-    return result;
+    return value;
+}
+
+// Decimal real: [+-] digits [. digits] [(e|E) [+-] digits], at least one mantissa digit; the scan
+// accepts exactly that grammar (no blanks, inf, nan or hexadecimal forms) and strtod converts it.
+double parse_real(std::string_view text) {
+    size_t k      = 0;
+    auto   digits = [&] {
+        const size_t from = k;
+        while (k < text.size() && text[k] >= '0' && text[k] <= '9') ++k;
+        return k - from;
+    };
+    if (k < text.size() && (text[k] == '+' || text[k] == '-')) ++k;
+    size_t mantissa = digits();
+    if (k < text.size() && text[k] == '.') {
+        ++k;
+        mantissa += digits();
+    }
+    if (mantissa == 0) bad_number(text);
+    if (k < text.size() && (text[k] == 'e' || text[k] == 'E')) {
+        ++k;
+        if (k < text.size() && (text[k] == '+' || text[k] == '-')) ++k;
+        if (digits() == 0) bad_number(text);
+    }
+    if (k != text.size()) bad_number(text);
+    const std::string token(text);
+    return std::strtod(token.c_str(), nullptr);
 }
 
 template <class T>
@@ -132,20 +165,23 @@ int dump_obj(int argc, char** argv) {
     return 0;
 }
 
-void write_outputs(const mrg_host::RgbImage& image, std::filesystem::path prefix, double exposure) {
-    if (prefix.extension() == ".png" || prefix.extension() == ".exr") prefix.replace_extension();
-    if (!prefix.parent_path().empty()) std::filesystem::create_directories(prefix.parent_path());
-    auto exr = prefix, png = prefix;
-    exr += ".exr";
-    png += ".png";
-    mrg_host::write_exr(image, exr);
-    mrg_host::write_png(mrg_host::display_image(image, exposure), png);
+// Output files of a prefix: "<prefix>.exr" (linear float) and "<prefix>.png" (display image). A
+// prefix that already names one of the two (".png" / ".exr") is taken without that extension; the
+// directory part is created when missing.
+void write_outputs(const mrg_host::RgbImage& image, const std::filesystem::path& prefix, double exposure) {
+    std::string base = prefix.u8string();
+    const bool named = base.size() > 4 && (base.compare(base.size() - 4, 4, ".png") == 0 || base.compare(base.size() - 4, 4, ".exr") == 0);
+    if (named) base.resize(base.size() - 4);
+    const std::filesystem::path stem = std::filesystem::u8path(base);
+    if (stem.has_parent_path()) std::filesystem::create_directories(stem.parent_path());
+    mrg_host::write_exr(image, std::filesystem::u8path(base + ".exr"));
+    mrg_host::write_png(mrg_host::display_image(image, exposure), std::filesystem::u8path(base + ".png"));
 }
 
 int encode(int argc, char** argv) {
     if (argc < 2) throw std::invalid_argument("Usage: --encode image.bin output-prefix [--exposure EV]");
     double exposure = 0;
-    if (argc >= 4 && std::string_view(argv[2]) == "--exposure") exposure = number<double>(argv[3]);
+    if (argc >= 4 && std::string_view(argv[2]) == "--exposure") exposure = parse_real(argv[3]);
     std::ifstream in(argv[0], std::ios::binary);
     mrg_host::RgbImage image;
     in.read(reinterpret_cast<char*>(&image.width), 4);
@@ -169,34 +205,48 @@ int render(int argc, char** argv) {
     uint32_t spp = 64, depth = 8, rr_depth = 3, threads = 0, tile = 16, batch = 4, mode = MRG_MODE_MIS;
     bool     rr       = true;
     double   exposure = 0;
-    for (int index = 2; index < argc; ++index) {
-        const std::string_view option(argv[index]);
-        if (option == "--no-rr") {
+    // Every option with a value, and what its value sets. --no-rr is the one switch without a value.
+    struct Setting {
+        std::string_view                        flag;
+        std::function<void(std::string_view)> set;
+    };
+    const Setting settings[] = {
+        {"--width", [&](std::string_view v) { width = parse_count<uint64_t>(v); }},
+        {"--height", [&](std::string_view v) { height = parse_count<uint64_t>(v); }},
+        {"--spp", [&](std::string_view v) { spp = parse_count<uint32_t>(v); }},
+        {"--depth", [&](std::string_view v) { depth = parse_count<uint32_t>(v); }},
+        {"--rr-depth", [&](std::string_view v) { rr_depth = parse_count<uint32_t>(v); }},
+        {"--threads", [&](std::string_view v) { threads = parse_count<uint32_t>(v); }},
+        {"--tile", [&](std::string_view v) { tile = parse_count<uint32_t>(v); }},
+        {"--batch", [&](std::string_view v) { batch = parse_count<uint32_t>(v); }},
+        {"--seed", [&](std::string_view v) { seed = parse_count<uint64_t>(v); }},
+        {"--exposure", [&](std::string_view v) { exposure = parse_real(v); }},
+        {"--mode",
+         [&](std::string_view v) {
+             static const std::pair<std::string_view, uint32_t> modes[] = {
+                 {"bsdf", MRG_MODE_BSDF}, {"light", MRG_MODE_LIGHT}, {"mis", MRG_MODE_MIS}};
+             const auto* m = std::find_if(std::begin(modes), std::end(modes), [&](const auto& e) { return e.first == v; });
+             if (m == std::end(modes)) throw std::invalid_argument("Sampling mode must be bsdf, light or mis");
+             mode = m->second;
+         }},
+        {"--integrator",
+         [&](std::string_view v) {
+             if (v == "gpu") return;
+             if (v == "path" || v == "sppm")
+                 throw std::invalid_argument("The CPU integrators (path, sppm) are not part of this build; use --integrator gpu");
+             throw std::invalid_argument("Integrator must be gpu");
+         }},
+    };
+    std::vector<std::string_view> args(argv + 2, argv + argc);
+    for (size_t k = 0; k < args.size(); ++k) {
+        if (args[k] == "--no-rr") {
             rr = false;
             continue;
         }
-        if (index + 1 >= argc) throw std::invalid_argument("Render option is missing its value: " + std::string(option));
-        const std::string_view value(argv[++index]);
-        if (option == "--width") width = number<uint64_t>(value);
-        else if (option == "--height") height = number<uint64_t>(value);
-        else if (option == "--spp") spp = number<uint32_t>(value);
-        else if (option == "--depth") depth = number<uint32_t>(value);
-        else if (option == "--rr-depth") rr_depth = number<uint32_t>(value);
-        else if (option == "--threads") threads = number<uint32_t>(value);
-        else if (option == "--tile") tile = number<uint32_t>(value);
-        else if (option == "--batch") batch = number<uint32_t>(value);
-        else if (option == "--seed") seed = number<uint64_t>(value);
-        else if (option == "--mode") {
-            if (value == "bsdf") mode = MRG_MODE_BSDF;
-            else if (value == "light") mode = MRG_MODE_LIGHT;
-            else if (value == "mis") mode = MRG_MODE_MIS;
-            else throw std::invalid_argument("Sampling mode must be bsdf, light or mis");
-        } else if (option == "--integrator") {
-            if (value == "path" || value == "sppm")
-                throw std::invalid_argument("The CPU integrators (path, sppm) are not part of this build; use --integrator gpu");
-            if (value != "gpu") throw std::invalid_argument("Integrator must be gpu");
-        } else if (option == "--exposure") exposure = number<double>(value);
-        else throw std::invalid_argument("Unknown render option: " + std::string(option));
+        if (k + 1 == args.size()) throw std::invalid_argument("Render option is missing its value: " + std::string(args[k]));
+        const auto* s = std::find_if(std::begin(settings), std::end(settings), [&](const Setting& e) { return e.flag == args[k]; });
+        if (s == std::end(settings)) throw std::invalid_argument("Unknown render option: " + std::string(args[k]));
+        s->set(args[++k]);
     }
     // RenderSettings / IntegratorSettings / SessionSettings::validate
     if (width == 0 || height == 0 || width > 16384 || height > 16384 || width * height > 64u * 1024u * 1024u || spp == 0)

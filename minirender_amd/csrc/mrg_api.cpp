@@ -201,6 +201,9 @@ struct mrg_device {
     std::condition_variable            cv;
     std::map<std::string, mrg_scene*>  scenes;
     std::deque<mrg_job*>               jobs;  // admission order
+    // Finished jobs whose film is still in HBM (and in the budget), oldest first: admission moves
+    // them to host memory only when a waiting job needs the room, so finishing a job costs nothing.
+    std::deque<mrg_job*>               resident_films;
     uint64_t                           next_job_id = 1;
     bool                               shutdown = false;
     // Set once mrg_device_close has stopped the scheduler and freed the device's own buffers and
@@ -576,10 +579,9 @@ void uncount_film_locked(mrg_device* d, mrg_job* j) {
     j->film_counted = false;
 }
 
-// Copies a finished job's film into host memory and frees its HBM, so admission counts only the films
-// of jobs that can still render while every film stays readable. The caller holds the job's
-// film_mutex, and no batch that writes the film is in flight. On a copy failure the film stays on the
-// device (and in the budget) until the job is released.
+// Copies a finished job's film into host memory and frees its HBM; the film stays readable. The
+// caller holds the job's film_mutex, and no batch that writes the film is in flight. On a copy
+// failure the film stays on the device (and in the budget) until the job is released.
 void retire_film(mrg_device* d, mrg_job* j) {
     if (!j->film_sums.ptr || j->film_on_host) return;
     const uint64_t px = uint64_t(j->desc.width) * j->desc.height;
@@ -599,21 +601,29 @@ void retire_film(mrg_device* d, mrg_job* j) {
     uncount_film_locked(d, j);
 }
 
-// Terminal state; `film_held`: the caller already holds the job's film_mutex (a batch just landed).
-void finish_job_locked(mrg_device* d, mrg_job* j, uint32_t state, bool film_held = false) {
+// Terminal state. An admitted job's film stays in HBM, counted in the budget, on the device's list of
+// resident films, until admission needs its room, the job is released or the device closes.
+void finish_job_locked(mrg_device* d, mrg_job* j, uint32_t state) {
     j->state       = state;
     j->finished    = true;
     j->finished_at = Clock::now();
     if (j->admitted) {
-        if (film_held) {
-            retire_film(d, j);
-        } else {
-            std::lock_guard<std::mutex> fl(j->film_mutex);
-            retire_film(d, j);
-        }
+        if (j->film_counted) d->resident_films.push_back(j);
         j->admitted = false;
     }
     d->cv.notify_all();
+}
+
+// Moves the oldest finished films to host memory until `needed` more bytes fit the budget (or
+// none are left); returns whether they fit. Scheduler thread, between batches.
+bool make_room_locked(mrg_device* d, uint64_t needed) {
+    while (d->resident_bytes + needed + d->pool_bytes > budget(d) && !d->resident_films.empty()) {
+        mrg_job* j = d->resident_films.front();
+        d->resident_films.pop_front();
+        std::lock_guard<std::mutex> fl(j->film_mutex);
+        retire_film(d, j);
+    }
+    return d->resident_bytes + needed + d->pool_bytes <= budget(d);
 }
 
 // Admission in submission order while the budget allows (films; scenes are already resident).
@@ -624,7 +634,7 @@ void admit_locked(mrg_device* d) {
             finish_job_locked(d, j, MRG_STATE_CANCELLED);
             continue;
         }
-        if (d->resident_bytes + j->film_bytes + d->pool_bytes > budget(d)) break;  // FIFO: later jobs wait too
+        if (!make_room_locked(d, j->film_bytes)) break;  // FIFO: later jobs wait too
         j->film_sums.alloc(uint64_t(j->desc.width) * j->desc.height * 3 * 8);
         j->film_counts.alloc(uint64_t(j->desc.width) * j->desc.height * 4);
         j->error_flag.alloc(16);
@@ -944,13 +954,13 @@ void run_batch(mrg_device* d, BatchPlan& b) {
         if (flags[i * 4] != 0) {
             j->status = MRG_NONFINITE;
             j->error  = (flags[i * 4] & 1u) ? "Path contribution is nonfinite or negative" : "Render sample exceeds the float image range";
-            finish_job_locked(d, j, MRG_STATE_FAILED, true);
+            finish_job_locked(d, j, MRG_STATE_FAILED);
         } else if (j->cancel_requested) {
             j->status = MRG_CANCELLED;
-            finish_job_locked(d, j, MRG_STATE_CANCELLED, true);
+            finish_job_locked(d, j, MRG_STATE_CANCELLED);
         } else if (j->next_sample >= j->desc.samples_per_pixel && j->next_pixel == 0) {
             j->state = MRG_STATE_FINALIZING;
-            finish_job_locked(d, j, MRG_STATE_COMPLETED, true);
+            finish_job_locked(d, j, MRG_STATE_COMPLETED);
         } else if (j->pause_requested) {
             j->state = MRG_STATE_PAUSED;
         }
@@ -975,8 +985,14 @@ void scheduler(mrg_device* d) {
                             j->status = MRG_CANCELLED;
                             finish_job_locked(d, j, MRG_STATE_CANCELLED);
                         }
-                        release_job_locked(j);
                     }
+                    // the device buffers go away with the device: every film still in HBM moves to host memory
+                    for (mrg_job* j : d->resident_films) {
+                        std::lock_guard<std::mutex> fl(j->film_mutex);
+                        retire_film(d, j);
+                    }
+                    d->resident_films.clear();
+                    for (mrg_job* j : d->jobs) release_job_locked(j);
                     d->jobs.clear();
                     d->cv.notify_all();
                     return;
@@ -1027,7 +1043,7 @@ void scheduler(mrg_device* d) {
                 j->in_flight = false;
                 j->status    = MRG_DEVICE_ERROR;
                 j->error     = e.what();
-                if (!j->finished) finish_job_locked(d, j, MRG_STATE_FAILED, true);
+                if (!j->finished) finish_job_locked(d, j, MRG_STATE_FAILED);
                 j->film_mutex.unlock();
                 release_job_locked(j);
             }
@@ -1043,6 +1059,7 @@ bool release_job_locked(mrg_job* j) {
     if (--j->refs == 0) {
         mrg_scene* s = j->scene;
         uncount_film_locked(d, j);
+        d->resident_films.erase(std::remove(d->resident_films.begin(), d->resident_films.end(), j), d->resident_films.end());
         delete j;
         d->live--;
         if (s && --s->refs == 0) {
@@ -1269,32 +1286,45 @@ int mrg_scene_info_get(mrg_scene* s, mrg_scene_info* out) {
     });
 }
 
-static void run_query(mrg_scene* s, const void* d_rays, uint64_t n, float tmin, float tmax, void* d_hits, float* d_bary,
-                      uint8_t* d_occ, bool any, mrg_trace_stats* stats, float* kernel_ms) {
+static void check_interval(uint64_t n, float tmin, float tmax) {
     if (!(tmin >= 0) || !std::isfinite(tmin) || std::isnan(tmax) || tmax < tmin)
         throw std::invalid_argument("Ray interval must satisfy 0 <= minimum <= maximum");
     if (n >= (uint64_t(1) << 32) - 64) throw std::invalid_argument("too many rays for one call");
+}
+
+// One query launch on the device's query stream, the caller holding query_mutex. `work` (64 B) and
+// `stats_dev` (16 B) are the launch's counters; they are zeroed here.
+static void query_locked(mrg_scene* s, const void* d_rays, uint64_t n, float tmin, float tmax, void* d_hits, float* d_bary,
+                         uint8_t* d_occ, bool any, uint32_t* work, uint64_t* stats_dev, mrg_trace_stats* stats, float* kernel_ms) {
     mrg_device* d = s->device;
-    std::lock_guard<std::mutex> lock(d->query_mutex);  // one query at a time per device (shared spill area)
-    if (d->closed) throw std::invalid_argument("device is closed");
-    check_hip(hipSetDevice(d->ordinal), "hipSetDevice");
-    DevBuf st(16);
-    check_hip(hipMemsetAsync(st.ptr, 0, 16, d->query_stream), "hipMemsetAsync");
-    check_hip(hipMemsetAsync(d->query_work.ptr, 0, 64, d->query_stream), "hipMemsetAsync");
+    check_hip(hipMemsetAsync(stats_dev, 0, 16, d->query_stream), "hipMemsetAsync");
+    check_hip(hipMemsetAsync(work, 0, 64, d->query_stream), "hipMemsetAsync");
     hipEvent_t e0 = d->query_events[0], e1 = d->query_events[1];
     check_hip(hipEventRecord(e0, d->query_stream), "hipEventRecord");
     check_hip(launch_query(s->gpu, static_cast<const float*>(d_rays), uint32_t(n), tmin, tmax, static_cast<mrg_hit_raw*>(d_hits), d_bary,
-                           d_occ, d->query_work.as<uint32_t>(), d->query_pool, st.as<uint64_t>(), any, stats != nullptr, d->shape, d->query_stream),
+                           d_occ, work, d->query_pool, stats_dev, any, stats != nullptr, d->shape, d->query_stream),
               "k_query");
     check_hip(hipEventRecord(e1, d->query_stream), "hipEventRecord");
     uint64_t h[2] = {0, 0};
-    if (stats) check_hip(hipMemcpyAsync(h, st.ptr, 16, hipMemcpyDeviceToHost, d->query_stream), "read stats");
+    if (stats) check_hip(hipMemcpyAsync(h, stats_dev, 16, hipMemcpyDeviceToHost, d->query_stream), "read stats");
     check_hip(hipStreamSynchronize(d->query_stream), "hipStreamSynchronize");
     if (kernel_ms) check_hip(hipEventElapsedTime(kernel_ms, e0, e1), "hipEventElapsedTime");
     if (stats) {
         stats->bounds_tests   = h[0];
         stats->triangle_tests = h[1];
     }
+}
+
+static void run_query(mrg_scene* s, const void* d_rays, uint64_t n, float tmin, float tmax, void* d_hits, float* d_bary,
+                      uint8_t* d_occ, bool any, mrg_trace_stats* stats, float* kernel_ms) {
+    check_interval(n, tmin, tmax);
+    mrg_device* d = s->device;
+    std::lock_guard<std::mutex> lock(d->query_mutex);  // one query at a time per device (shared spill area)
+    if (d->closed) throw std::invalid_argument("device is closed");
+    check_hip(hipSetDevice(d->ordinal), "hipSetDevice");
+    DevBuf st(16);
+    query_locked(s, d_rays, n, tmin, tmax, d_hits, d_bary, d_occ, any, d->query_work.as<uint32_t>(), st.as<uint64_t>(), stats,
+                 kernel_ms);
 }
 
 int mrg_intersect(mrg_scene* s, const float* rays, uint64_t n, float tmin, float tmax, mrg_hit* hits, float* bary, mrg_trace_stats* stats) {

@@ -204,8 +204,9 @@ def test_close_with_a_paused_job_outstanding():
 
 def test_finished_films_leave_the_admission_budget():
     """A budget that holds two films at a time: six jobs submitted together, every handle kept until all
-    finish. Finished films leave HBM (their results stay readable), so the later jobs are admitted as the
-    earlier ones complete instead of waiting on handles the caller still holds."""
+    finish. Admission moves finished films out of HBM when a waiting job needs the room (their results
+    stay readable), so the later jobs are admitted as the earlier ones complete instead of waiting on
+    handles the caller still holds; the budget is never exceeded."""
     scene = scenegen.cornell_box()
     film = 128 * 128 * 28
     with api.Device(0, path_capacity=1 << 16) as probe:
@@ -222,9 +223,12 @@ def test_finished_films_leave_the_admission_budget():
         ps = [j.progress() for j in jobs]
         assert all(p.state == "completed" for p in ps)
         assert max(p.queued_seconds for p in ps) > 0
-        assert d.memory()["resident_bytes"] == m["resident_bytes"]
+        assert d.memory()["resident_bytes"] + m["pool_bytes"] <= budget
         ref = api.render(d, scene, api.RenderSettings(128, 128, 8, 3), api.IntegratorSettings("mis", 4))[0]
-        np.testing.assert_array_equal(jobs[3].image(), ref)
+        np.testing.assert_array_equal(jobs[3].image(), ref)  # a film moved to host memory
+        np.testing.assert_array_equal(jobs[0].image(), api.render(d, scene, api.RenderSettings(128, 128, 8, 0),
+                                                                  api.IntegratorSettings("mis", 4))[0])
         for j in jobs:
             j.release()
+        assert d.memory()["resident_bytes"] == m["resident_bytes"]
         bvh.release()
