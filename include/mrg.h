@@ -243,6 +243,19 @@ int mrg_intersect_device(mrg_scene* scene, const void* device_rays, uint64_t n, 
                          void* device_hits, mrg_trace_stats* stats, float* kernel_ms);
 
 /*
+ * mrg_intersect without any allocation: no counterpart in the reference. For callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same arguments and results as mrg_intersect (host rays in, host hits / barycentrics /
+ * counters out), but the rays, hits, barycentrics and the query's counters are staged through one
+ * fixed workspace per GPU (a 44 MiB static device array inside libmrg, held from load until the process
+ * ends) instead of buffers allocated per call. At most 1,048,576 rays per call (MRG_INVALID_ARGUMENT
+ * beyond). Calls through one device handle are serialised; calls through two handles of the same GPU
+ * share the workspace and race.
+ */
+int mrg_intersect_noalloc(mrg_scene* scene, const float* rays, uint64_t n, float tmin, float tmax, mrg_hit* hits,
+                          float* barycentric, mrg_trace_stats* stats);
+
+/*
  * Jobs (RenderSession::start/pause/resume/cancel/wait/progress/preview/result,
  * render_session.hpp:82-115). Submitting admits the job when the HBM budget allows, otherwise
  * queues it; the device loop renders it progressively in passes of pass_samples spp.

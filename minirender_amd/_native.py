@@ -114,6 +114,7 @@ SIGNATURES = {
     "mrg_intersect": (C.c_int, [P, P, C.c_uint64, C.c_float, C.c_float, P, P, P]),
     "mrg_occluded": (C.c_int, [P, P, C.c_uint64, C.c_float, C.c_float, P, P]),
     "mrg_intersect_device": (C.c_int, [P, P, C.c_uint64, C.c_float, C.c_float, P, P, C.POINTER(C.c_float)]),
+    "mrg_intersect_noalloc": (C.c_int, [P, P, C.c_uint64, C.c_float, C.c_float, P, P, P]),
     "mrg_job_submit": (C.c_int, [P, P, C.POINTER(JobDesc), C.POINTER(P)]),
     "mrg_job_pause": (C.c_int, [P, C.POINTER(C.c_int)]),
     "mrg_job_resume": (C.c_int, [P, C.POINTER(C.c_int)]),

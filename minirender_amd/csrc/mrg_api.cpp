@@ -193,6 +193,7 @@ struct mrg_device {
     DevBuf             pool_mem, spill_ref, spill_t, counters, stats, header, jobs_buf, chunks_buf, query_work;
     DevBuf             query_spill_ref, query_spill_t;
     DevBuf             energy;  // conductor energy table (mrg_energy.h), float4 per node
+    void*              noalloc_scratch = nullptr;  // this GPU's address of the allocation-free query's workspace
     std::mutex         query_mutex;
     uint32_t           max_chunks = 0, max_jobs = 0;
     uint64_t           pool_bytes = 0;
@@ -1127,6 +1128,8 @@ int mrg_device_open(int ordinal, const mrg_device_options* options, mrg_device**
         if (d->opt.max_jobs_in_flight == 0) d->opt.max_jobs_in_flight = 64;
         if (d->opt.max_jobs_in_flight > 4096) throw std::invalid_argument("max_jobs_in_flight too large");
         alloc_pool(d.get());
+        // the allocation-free query's workspace: one static array per GPU, looked up with the device current
+        check_hip(noalloc_scratch_address(&d->noalloc_scratch), "hipGetSymbolAddress(scratch)");
         upload(d->energy, conductor_energy_table());
         d->pool.energy       = d->energy.as<float>();
         d->query_pool.energy = d->energy.as<float>();
@@ -1360,6 +1363,33 @@ int mrg_intersect_device(mrg_scene* s, const void* rays, uint64_t n, float tmin,
         if (!s || (n && (!rays || !hits))) throw std::invalid_argument("null argument");
         if (n == 0) return;
         run_query(s, rays, n, tmin, tmax, hits, nullptr, nullptr, false, stats, kernel_ms);
+    });
+}
+
+int mrg_intersect_noalloc(mrg_scene* s, const float* rays, uint64_t n, float tmin, float tmax, mrg_hit* hits, float* bary,
+                          mrg_trace_stats* stats) {
+    return guarded([&] {
+        if (!s || (n && (!rays || !hits))) throw std::invalid_argument("null argument");
+        if (n > kNoAllocMaxRays)
+            throw std::invalid_argument("mrg_intersect_noalloc takes at most 1048576 rays per call (its workspace is fixed)");
+        check_interval(n, tmin, tmax);
+        if (n == 0) return;
+        mrg_device* d = s->device;
+        // the workspace is this GPU's static array: query_mutex serialises the calls on one device handle
+        std::lock_guard<std::mutex> lock(d->query_mutex);
+        if (d->closed) throw std::invalid_argument("device is closed");
+        if (!d->noalloc_scratch) throw Error(MRG_DEVICE_ERROR, "the allocation-free query's workspace is not available");
+        check_hip(hipSetDevice(d->ordinal), "hipSetDevice");
+        char* base = static_cast<char*>(d->noalloc_scratch);
+        check_hip(hipMemcpyAsync(base + kNoAllocRaysOffset, rays, n * 24, hipMemcpyHostToDevice, d->query_stream), "stage rays");
+        query_locked(s, base + kNoAllocRaysOffset, n, tmin, tmax, base + kNoAllocHitsOffset,
+                     bary ? reinterpret_cast<float*>(base + kNoAllocBaryOffset) : nullptr, nullptr, false,
+                     reinterpret_cast<uint32_t*>(base + kNoAllocWorkOffset), reinterpret_cast<uint64_t*>(base + kNoAllocStatsOffset),
+                     stats, nullptr);
+        check_hip(hipMemcpyAsync(hits, base + kNoAllocHitsOffset, n * 8, hipMemcpyDeviceToHost, d->query_stream), "read hits");
+        if (bary)
+            check_hip(hipMemcpyAsync(bary, base + kNoAllocBaryOffset, n * 12, hipMemcpyDeviceToHost, d->query_stream), "read barycentrics");
+        check_hip(hipStreamSynchronize(d->query_stream), "hipStreamSynchronize");
     });
 }
 

@@ -13,6 +13,19 @@ struct mrg_hit_raw {
     int32_t primitive;
 };
 
+// Workspace of the allocation-free ray query (mrg_intersect_noalloc): one file-scope device array
+// per GPU, laid out as the query's work counters, its statistics counters, then the staged rays, hits
+// and barycentrics of at most kNoAllocMaxRays rays.
+constexpr uint64_t kNoAllocMaxRays       = uint64_t(1) << 20;
+constexpr uint64_t kNoAllocWorkOffset    = 0;    // 64 B of work-claim counters (zeroed per call)
+constexpr uint64_t kNoAllocStatsOffset   = 128;  // 2 x u64 traversal counters (zeroed per call)
+constexpr uint64_t kNoAllocRaysOffset    = 256;
+constexpr uint64_t kNoAllocHitsOffset    = kNoAllocRaysOffset + kNoAllocMaxRays * 24;
+constexpr uint64_t kNoAllocBaryOffset    = kNoAllocHitsOffset + kNoAllocMaxRays * 8;
+constexpr uint64_t kNoAllocScratchBytes  = kNoAllocBaryOffset + kNoAllocMaxRays * 12;  // 44 MiB + 256 B
+// Device address of the workspace on the current device (hipGetSymbolAddress of the array).
+hipError_t noalloc_scratch_address(void** out);
+
 // Path-state flags kept in prev.w: bit 0 = previous event was discrete (delta); bits 16.. = job slot.
 constexpr uint32_t kPrevDelta = 1u;
 constexpr uint32_t kMaxBounces = 130;  // depth <= 128 (IntegratorSettings::validate) plus one

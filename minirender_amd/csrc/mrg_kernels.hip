@@ -946,6 +946,13 @@ __global__ void __launch_bounds__(256) k_accumulate(Pool P, const GpuJob* __rest
 
 static uint32_t grid_for(const LaunchShape& s, uint32_t per_cu) { return s.compute_units * per_cu; }
 
+// The allocation-free query's workspace. Host code never passes `scratch` itself to a kernel or a copy
+// (in host code the name is a stand-in, not the device address): it looks the address up once per
+// device with noalloc_scratch_address and uses that pointer.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+hipError_t noalloc_scratch_address(void** out) { return hipGetSymbolAddress(out, HIP_SYMBOL(scratch)); }
+
 hipError_t launch_query(const GpuScene& sc, const float* rays, uint32_t n, float tmin, float tmax, mrg_hit_raw* hits, float* bary,
                         uint8_t* occluded, uint32_t* work, const Pool& P, uint64_t* stats, bool any, bool with_stats,
                         const LaunchShape& shape, hipStream_t stream) {

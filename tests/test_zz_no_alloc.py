@@ -1,0 +1,62 @@
+"""mrg_intersect_noalloc (the allocation-free ray query, staged through the library's static per-GPU
+workspace) against mrg_intersect on the same rays: identical hit ids, distances and barycentrics, and
+the same traversal counters within 1e-3, on two calls in a row (the second smaller, so stale workspace contents would show),
+at the workspace's maximum of 1,048,576 rays, and a clear error one ray beyond it."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from minirender_amd import _native as N, api, scenegen
+
+pytestmark = pytest.mark.gpu
+
+MAX_RAYS = 1 << 20
+HIT = np.dtype([("t", "<f4"), ("primitive", "<i4")])
+
+
+def _query(fn, bvh, rays, tmin=0.0, tmax=np.inf, stats=True):
+    n = len(rays)
+    rays = np.ascontiguousarray(rays, np.float32)
+    hits = np.empty(n, HIT)
+    bary = np.empty((n, 3), np.float32)
+    counters = np.zeros(2, np.uint64)  # mrg_trace_stats: bounds_tests, triangle_tests
+    p = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+    N.check(getattr(N.lib(), fn)(bvh.handle, p(rays), n, tmin, tmax, p(hits), p(bary), p(counters) if stats else None))
+    return hits, bary, counters
+
+
+def _same(bvh, rays, tmin=0.0, tmax=np.inf, stats=True):
+    a = _query("mrg_intersect", bvh, rays, tmin, tmax, stats)
+    b = _query("mrg_intersect_noalloc", bvh, rays, tmin, tmax, stats)
+    np.testing.assert_array_equal(a[0]["primitive"], b[0]["primitive"])
+    np.testing.assert_array_equal(a[0]["t"], b[0]["t"])
+    hit = a[0]["primitive"] >= 0
+    assert hit.any()
+    np.testing.assert_array_equal(a[1][hit], b[1][hit])
+    if stats:
+        # box / triangle test counts depend on which rays the refill driver groups into a wave (claimed with
+        # atomics, so they vary from run to run by about 1e-4); hits do not
+        assert a[2][0] > 0 and a[2][1] > 0
+        np.testing.assert_allclose(b[2].astype(np.float64), a[2].astype(np.float64), rtol=1e-3)
+    else:
+        assert (a[2] == 0).all() and (b[2] == 0).all()
+    return b
+
+
+def test_noalloc_query_equals_the_allocating_query():
+    scene = scenegen.icosphere(level=6, seed=3)
+    with api.Device(0) as dev:
+        bvh = dev.upload(scene)
+        print("[noalloc] 262144 rays, then 4096 rays, then 1048576 rays", flush=True)
+        big = scenegen.intersection_rays(scene, 1 << 18, seed=11)
+        _same(bvh, big)
+        small = scenegen.intersection_rays(scene, 1 << 12, seed=12)
+        _same(bvh, small, 0.01, 3.0, stats=False)
+        full = scenegen.intersection_rays(scene, MAX_RAYS, seed=13)
+        _same(bvh, full)
+        over = np.zeros((MAX_RAYS + 1, 6), np.float32)
+        over[:, 5] = 1
+        with pytest.raises(N.InvalidArgument, match="1048576"):
+            _query("mrg_intersect_noalloc", bvh, over)
+        bvh.release()
