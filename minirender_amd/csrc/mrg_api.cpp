@@ -368,7 +368,8 @@ mrg_scene* build_scene(mrg_device* d, const mrg_scene_desc& s, const std::string
     if (bvh.max_depth > kMaxDepth || bvh.max_stack > kStackTotal)
         throw Error(MRG_DEVICE_ERROR, "BVH exceeds the traversal stack bound");
     // leaf-order triangle records
-    std::vector<GpuTri>   tris(n), nrms(n);
+    std::vector<GpuTri>        tris(n);
+    std::vector<GpuTriNormals> nrms(n);
     std::vector<uint32_t> slot_instance(n);
     std::vector<int32_t>  slot_emitter(n, -1);
     // light table (LightSampler<float> of the oracle, light.cpp:197-264)
@@ -458,12 +459,16 @@ mrg_scene* build_scene(mrg_device* d, const mrg_scene_desc& s, const std::string
         t.prim = p;
         t.pad1 = (m.double_sided ? 0u : 1u) | (m.mirrored ? 2u : 0u);
         t.pad2 = 0;
-        GpuTri& r = nrms[slot];
-        r.v0[0] = nm[0]; r.v0[1] = nm[1]; r.v0[2] = nm[2];
-        r.v1[0] = nm[3]; r.v1[1] = nm[4]; r.v1[2] = nm[5];
-        r.v2[0] = nm[6]; r.v2[1] = nm[7]; r.v2[2] = nm[8];
-        r.prim = p;
-        r.pad1 = r.pad2 = 0;
+        // surface_hit's per-triangle constants in its float operations (normalize = v * (1 / sqrt(v.v))):
+        // the normalized corner normals and the geometric normal before a mirrored instance's flip
+        GpuTriNormals& r  = nrms[slot];
+        const V3f      gn = normalize(cross(sub(ld(v + 3), ld(v)), sub(ld(v + 6), ld(v))));
+        st(r.n0, normalize(ld(nm)));
+        st(r.n1, normalize(ld(nm + 3)));
+        st(r.n2, normalize(ld(nm + 6)));
+        r.gx = gn.x;
+        r.gy = gn.y;
+        r.gz = gn.z;
         slot_instance[slot] = s.instance[p];
         slot_emitter[slot]  = area_choice[p];
     }
@@ -520,7 +525,7 @@ mrg_scene* build_scene(mrg_device* d, const mrg_scene_desc& s, const std::string
     GpuScene& g     = sc->gpu;
     g.nodes         = sc->nodes.as<GpuNode4>();
     g.tris          = sc->tris.as<GpuTri>();
-    g.normals       = sc->normals.as<GpuTri>();
+    g.normals       = sc->normals.as<GpuTriNormals>();
     g.slot_instance = sc->slot_instance.as<uint32_t>();
     g.slot_emitter  = sc->slot_emitter.as<int32_t>();
     g.materials     = sc->materials.as<GpuMaterial>();

@@ -668,17 +668,19 @@ __device__ __forceinline__ void shade_path(const Pool& P, const GpuJob& job, uin
             const float4  na = gld(np + 0), nb = gld(np + 1), nc = gld(np + 2);
             const uint32_t inst = gld(sc.slot_instance + slot);
             const GpuMaterial& mat = sc.materials[inst];
-            F3 geometric = normalize(cross(v1 - v0, v2 - v0));
+            // corner normals arrive normalized and the geometric normal in their w components
+            // (computed at upload with these same float operations)
+            F3 geometric = mk(na.w, nb.w, nc.w);
             if (tflags & kTriMirrored) geometric = -geometric;
             const float b0 = hv.y, b1 = hv.z, b2 = hv.w;
             F3 shading  = mk(0.0f, 0.0f, 0.0f);
             F3 position = mk(0.0f, 0.0f, 0.0f);
             F3 absolute = mk(0.0f, 0.0f, 0.0f);
-            shading = shading + b0 * normalize(mk(na.x, na.y, na.z));
+            shading = shading + b0 * mk(na.x, na.y, na.z);
             { const F3 term = b0 * v0; position = position + term; absolute = absolute + vabs(term); }
-            shading = shading + b1 * normalize(mk(nb.x, nb.y, nb.z));
+            shading = shading + b1 * mk(nb.x, nb.y, nb.z);
             { const F3 term = b1 * v1; position = position + term; absolute = absolute + vabs(term); }
-            shading = shading + b2 * normalize(mk(nc.x, nc.y, nc.z));
+            shading = shading + b2 * mk(nc.x, nc.y, nc.z);
             { const F3 term = b2 * v2; position = position + term; absolute = absolute + vabs(term); }
             if (dot(shading, shading) < 1e-24f) {
                 shading = geometric;

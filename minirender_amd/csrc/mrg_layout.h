@@ -6,7 +6,8 @@
 //                              four-box test. Depth-first order, root = 0.
 //   tris       [n]             GpuTri, 48 B: three float4 vertices in leaf order; v0.w carries the
 //                              scene-order primitive index (the tie-break key of bvh.cpp:273-276).
-//   normals    [n]             GpuTri layout, per-corner world normals in leaf order.
+//   normals    [n]             GpuTriNormals, 48 B: the three corner normals normalized, with the
+//                              triangle's unit geometric normal in the w components, in leaf order.
 //   slot_info  [n]             uint2 (instance, area-light choice or -1) in leaf order.
 //   materials  [instances]     GpuMaterial.
 //   uvs        [n]             per-corner texture coordinates in leaf order (3 x float2), if any.
@@ -65,6 +66,16 @@ struct alignas(16) GpuTri {
 };
 static_assert(sizeof(GpuTri) == 48, "GpuTri must be 48 bytes");
 
+// Shading normals of one triangle, computed once at upload with the shading kernel's float operations:
+// n_i = normalize(corner normal i) and g = normalize(cross(v1 - v0, v2 - v0)) (before the flip of a
+// mirrored instance), so surface_hit reads what it would otherwise recompute at every hit.
+struct alignas(16) GpuTriNormals {
+    float n0[3], gx;
+    float n1[3], gy;
+    float n2[3], gz;
+};
+static_assert(sizeof(GpuTriNormals) == 48, "GpuTriNormals must be 48 bytes");
+
 struct GpuMaterial {
     uint32_t kind;
     float    color[3];
@@ -113,7 +124,7 @@ struct GpuDirectional {
 struct GpuScene {
     const GpuNode4*       nodes;
     const GpuTri*         tris;
-    const GpuTri*         normals;
+    const GpuTriNormals*  normals;
     const uint32_t*       slot_instance;
     const int32_t*        slot_emitter;
     const GpuMaterial*    materials;
