@@ -750,6 +750,59 @@ GpuJob gpu_job(mrg_job* j) {
     return g;
 }
 
+// Hands the capacity left after the fair shares back out: whole-image chunks take more samples (up to a
+// pass each round), pixel-split chunks take more pixels of their current sample range.
+void grow_chunks(BatchPlan& b, uint32_t cap) {
+    bool grew = true;
+    while (grew && b.slots < cap) {
+        grew = false;
+        for (size_t i = 0; i < b.chunks.size() && b.slots < cap; ++i) {
+            GpuChunk& c = b.chunks[i];
+            mrg_job*  j = b.jobs[c.job];
+            const uint32_t pixels = j->desc.width * j->desc.height;
+            if (c.pixel_count == pixels) {
+                if (j->next_sample >= j->desc.samples_per_pixel) continue;
+                const uint32_t add = std::min({pass_of(j), j->desc.samples_per_pixel - j->next_sample, (cap - b.slots) / pixels});
+                if (add == 0) continue;
+                c.sample_count += add;
+                j->next_sample += add;
+                b.slots += add * pixels;
+                grew = true;
+            } else if (j->next_pixel == c.pixel_begin + c.pixel_count && j->next_pixel != 0) {
+                const uint32_t add = std::min(pixels - j->next_pixel, (cap - b.slots) / c.sample_count);
+                if (add == 0) continue;
+                c.pixel_count += add;
+                j->next_pixel += add;
+                b.slots += add * c.sample_count;
+                b.pixels += add;
+                if (j->next_pixel == pixels) {
+                    j->next_pixel = 0;
+                    j->next_sample += c.sample_count;
+                }
+                grew = true;
+            }
+        }
+    }
+}
+
+// Plans one chunk per job (a pixel never appears twice in one accumulate launch) for each job of `group`
+// within a fair share of the capacity still free.
+void plan_group(mrg_device* d, const std::vector<mrg_job*>& group, uint32_t cap, bool fill, BatchPlan& b) {
+    if (group.empty() || b.slots >= cap) return;
+    const uint32_t share = std::max<uint32_t>((cap - b.slots) / uint32_t(group.size()), 1u);
+    for (mrg_job* j : group) {
+        if (b.slots >= cap) break;
+        const uint32_t slot = uint32_t(b.jobs.size());
+        if (!plan_chunk(j, slot, std::min(cap - b.slots, share), fill, b)) continue;
+        b.jobs.push_back(j);
+        b.gjobs.push_back(gpu_job(j));
+        b.extended  = b.extended || j->scene->extended;
+        b.refill    = b.refill || refill_scene(d, j->scene);
+        b.one_scene = b.one_scene && j->scene == b.jobs.front()->scene;
+        b.max_depth = std::max(b.max_depth, j->desc.maximum_depth);
+    }
+}
+
 BatchPlan plan_batch_locked(mrg_device* d) {
     BatchPlan b;
     const uint32_t cap      = d->pool.capacity;
@@ -762,52 +815,21 @@ BatchPlan plan_batch_locked(mrg_device* d) {
     for (int pass = 0; pass < (batched ? 2 : 1); ++pass)
         for (mrg_job* j : d->jobs)
             if (runnable(j) && (!batched || j->first_pass_done == (pass == 1)) && cand.size() < max_jobs) cand.push_back(j);
-    // Each co-scheduled job gets at most a fair share of the path pool in this batch.
-    const uint32_t share = cand.empty() ? cap : std::max<uint32_t>(cap / uint32_t(cand.size()), 1u);
-    for (mrg_job* j : cand) {
-        if (b.slots >= cap) break;
-        const uint32_t slot = uint32_t(b.jobs.size());
-        // one chunk per job per batch: a pixel never appears twice in one accumulate launch
-        if (!plan_chunk(j, slot, std::min(cap - b.slots, share), fill, b)) continue;
-        b.jobs.push_back(j);
-        b.gjobs.push_back(gpu_job(j));
-        b.extended = b.extended || j->scene->extended;
-        b.refill   = b.refill || refill_scene(d, j->scene);
-        b.one_scene = b.one_scene && j->scene == b.jobs.front()->scene;
-        b.max_depth = std::max(b.max_depth, j->desc.maximum_depth);
-    }
-    if (batched && !b.jobs.empty() && b.slots < cap) {
-        // Hand the capacity left by the fair shares back out: whole-image chunks take more samples,
-        // pixel-split chunks take more pixels of their current sample range.
-        bool grew = true;
-        while (grew && b.slots < cap) {
-            grew = false;
-            for (size_t i = 0; i < b.chunks.size() && b.slots < cap; ++i) {
-                GpuChunk& c = b.chunks[i];
-                mrg_job*  j = b.jobs[c.job];
-                const uint32_t pixels = j->desc.width * j->desc.height;
-                if (c.pixel_count == pixels) {
-                    if (j->next_sample >= j->desc.samples_per_pixel) continue;
-                    const uint32_t add = std::min({pass_of(j), j->desc.samples_per_pixel - j->next_sample, (cap - b.slots) / pixels});
-                    if (add == 0) continue;
-                    c.sample_count += add;
-                    j->next_sample += add;
-                    b.slots += add * pixels;
-                    grew = true;
-                } else if (j->next_pixel == c.pixel_begin + c.pixel_count && j->next_pixel != 0) {
-                    const uint32_t add = std::min(pixels - j->next_pixel, (cap - b.slots) / c.sample_count);
-                    if (add == 0) continue;
-                    c.pixel_count += add;
-                    j->next_pixel += add;
-                    b.slots += add * c.sample_count;
-                    b.pixels += add;
-                    if (j->next_pixel == pixels) {
-                        j->next_pixel = 0;
-                        j->next_sample += c.sample_count;
-                    }
-                    grew = true;
-                }
-            }
+    if (!batched || cand.empty()) {
+        plan_group(d, cand, cap, fill, b);
+    } else {
+        // Scene-affine batches: the batch renders the scene of its most urgent job, shared by every
+        // runnable job of that scene, and so runs the one-scene traversal and shading kernels (scene
+        // pointers in scalar registers, no per-ray job lookup). Jobs of other scenes join only when the
+        // lead scene's jobs leave more than half of the pool empty; they lead the following batches.
+        mrg_scene*            lead = cand.front()->scene;
+        std::vector<mrg_job*> same, other;
+        for (mrg_job* j : cand) (j->scene == lead ? same : other).push_back(j);
+        plan_group(d, same, cap, fill, b);
+        grow_chunks(b, cap);
+        if (b.slots < cap / 2) {
+            plan_group(d, other, cap, fill, b);
+            grow_chunks(b, cap);
         }
         // recompute slot and pixel offsets
         uint32_t s = 0, px = 0;
