@@ -322,7 +322,7 @@ def bench_serving(args):
             f"{r['plain_fill']['msamples_per_s']:.1f}, batched {r['batched']['msamples_per_s']:.1f} Msamples/s")
         out.append(r)
     return {"stream": "64 jobs, Poisson arrivals, side 128-1024, spp 16-1024, depth 8-16, 8 scenes (36-410k tris), "
-                      "5% cancelled after 1-3 passes, 16-spp passes, 2^26-path pool, 64 GB budget",
+                      "5% cancelled after 1-3 passes, 16-spp passes, 2^27-path pool, 64 GB budget",
             "rates": out,
             "saturated_speedup": out[-1]["speedup"] if out and out[-1]["rate_jobs_per_s"] == "inf" else None}
 
@@ -445,7 +445,8 @@ def bench_batch(args):
     jobs = serve.batch_jobs(seed=4)
     calib = serve.batch_jobs(seed=4, spp_divisor=8)
     knobs = {"path_capacity": [1 << 23, 1 << 24, 1 << 25, 1 << 26, 1 << 27, 1 << 28], "persistent_blocks_per_cu": [0, 4, 6],
-             "traversal": ["split", "fused", "refill", "auto"], "use_graph": [False, True], "max_jobs_in_flight": [16, 64]}
+             "traversal": ["split", "fused", "refill", "auto"], "use_graph": [False, True], "max_jobs_in_flight": [16, 64],
+             "batches_in_flight": [1, 2]}
     tuned = {}
     for mode in ("plain", "plain_fill", "batched"):
         t = time.perf_counter()

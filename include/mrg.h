@@ -195,6 +195,9 @@ typedef struct mrg_device_options {
     uint32_t persistent_blocks_per_cu; /* 0 = automatic */
     uint32_t traversal;          /* MRG_TRAVERSAL_*; AUTO = refill for a batch of >= 2^23 paths with a
                                     scene of >= 2^14 triangles, fused otherwise */
+    uint32_t batches_in_flight;  /* 2: a second path pool renders the next batch while the previous one
+                                    drains its tail of late bounces (films still commit in batch order);
+                                    0 or 1: one batch at a time */
 } mrg_device_options;
 
 /* Hit of the intersection entry points: closest t and scene-order primitive index (-1 = miss). */

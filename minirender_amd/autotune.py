@@ -11,7 +11,9 @@ The knobs are the launch and queue choices of the wavefront integrator:
                                 or ``auto`` (refill for batches of >= 2^23 paths holding a
                                 scene of >= 2^14 triangles, fused otherwise),
 * ``use_graph``                 the bounce loop captured once in a hipGraph and replayed,
-* ``max_jobs_in_flight``        (batched schedule) jobs co-scheduled in one batch.
+* ``max_jobs_in_flight``        (batched schedule) jobs co-scheduled in one batch,
+* ``batches_in_flight``         1, or 2 (a second path pool renders the next batch while the previous
+                                one drains its tail of late bounces).
 
 A calibration workload (a list of scenes and job requests, all submitted at once) is rendered with
 each candidate; the score is samples per second over the whole workload, best of ``repeats`` runs
@@ -32,10 +34,11 @@ KNOBS = {
     "traversal": ["split", "fused", "refill", "auto"],
     "use_graph": [False, True],
     "max_jobs_in_flight": [8, 16, 32, 64],
+    "batches_in_flight": [1, 2],
 }
 
 DEFAULTS = {"path_capacity": 1 << 26, "persistent_blocks_per_cu": 0, "traversal": "auto", "use_graph": False,
-            "max_jobs_in_flight": 64}
+            "max_jobs_in_flight": 64, "batches_in_flight": 1}
 
 
 def run_workload(options, schedule, scenes, requests, repeats=2, pass_samples=16):

@@ -87,6 +87,29 @@ struct DevBuf {
     template <class T> T* as() const { return static_cast<T*>(ptr); }
 };
 
+// Pinned host allocation owned by RAII: staging of a batch's uploads and read-backs, so that copies
+// queued behind a running batch do not block the host thread.
+struct PinnedBuf {
+    void*  ptr   = nullptr;
+    size_t bytes = 0;
+    PinnedBuf() = default;
+    PinnedBuf(const PinnedBuf&) = delete;
+    PinnedBuf& operator=(const PinnedBuf&) = delete;
+    ~PinnedBuf() { release(); }
+    void alloc(size_t n) {
+        release();
+        if (n == 0) return;
+        check_hip(hipHostMalloc(&ptr, n, hipHostMallocDefault), "hipHostMalloc");
+        bytes = n;
+    }
+    void release() {
+        if (ptr) (void)hipHostFree(ptr);
+        ptr   = nullptr;
+        bytes = 0;
+    }
+    template <class T> T* as() const { return static_cast<T*>(ptr); }
+};
+
 // ---------------------------------------------------------------------------------------------
 // Float vector arithmetic in the FP32 twin's order (host side: camera constants, light table).
 struct V3f {
@@ -158,7 +181,10 @@ struct mrg_job {
     uint64_t      id = 0;
     // guarded by device->mutex
     uint32_t      state = MRG_STATE_PREPARING;
-    bool          admitted = false, cancel_requested = false, pause_requested = false, in_flight = false;
+    bool          admitted = false, cancel_requested = false, pause_requested = false;
+    int           in_flight = 0;   // batches holding samples of this job that have not landed
+    int           readers = 0;     // film reads waiting for the job's in-flight batches to land
+    bool          fail_pending = false;  // a landed batch failed; the job fails when the others land
     int32_t       status = MRG_OK;
     std::string   error;
     uint32_t      next_sample = 0;   // first sample of the next chunk
@@ -176,22 +202,54 @@ struct mrg_job {
     std::vector<double>   host_sums;
     std::vector<uint32_t> host_counts;
     bool                  film_on_host = false;
-    std::mutex    film_mutex;  // held while a batch that writes this film is in flight, and by film reads
+    std::mutex    film_mutex;  // film reads against the film's move to host memory
+};
+
+// The chunks of one batch and the jobs they sample.
+struct BatchPlan {
+    std::vector<mrg_job*>  jobs;
+    std::vector<GpuJob>    gjobs;
+    std::vector<GpuChunk>  chunks;
+    uint32_t               slots = 0, pixels = 0, max_depth = 0;
+    bool                   extended = false;  // some job's scene needs the extended shading kernel
+    bool                   refill   = false;  // bounce traversal with the refill driver (k_trace_refill)
+    bool                   one_scene = true;  // every job of the batch renders the same scene
+};
+
+// The resources of one batch in flight: a path pool with its queues, counters and stack-spill area,
+// the job and chunk tables, pinned staging for uploads and read-backs, a stream, timing events and
+// the captured bounce-loop graphs. A device runs its batches on one or two tracks; with two, the next
+// batch renders while the previous one drains its tail of late bounces, and each batch's film commit
+// waits for the previous batch's (so every pixel still sums its samples in index order).
+struct Track {
+    hipStream_t                        stream = nullptr;
+    Pool                               pool{};
+    DevBuf                             pool_mem, spill_ref, spill_t, counters, stats, header, jobs_buf, chunks_buf;
+    PinnedBuf                          up_jobs, up_chunks, up_header, down_counters, down_stats, down_flags;
+    std::vector<hipEvent_t>            events;
+    hipEvent_t                         committed = nullptr;  // after the batch's validate + accumulate
+    std::map<uint32_t, hipGraphExec_t> graphs;  // by bounce count, extended shading, refill traversal, one scene
+    // the batch in flight
+    bool                               busy = false;
+    BatchPlan                          plan;
+    uint32_t                           iters = 0;
+    bool                               timed = false;
+    std::vector<std::pair<int, int>>   marks;
 };
 
 struct mrg_device {
     int                ordinal = 0;
     mrg_device_options opt{};
-    hipStream_t        stream = nullptr;
+    hipStream_t        stream = nullptr;       // admission work (film clears)
     hipStream_t        copy_stream = nullptr;
     hipStream_t        query_stream = nullptr;  // ray queries: own stream and stack-spill area
     Pool               query_pool{};
     hipDeviceProp_t    props{};
     LaunchShape        shape{};
-    // path pool
-    Pool               pool{};
-    DevBuf             pool_mem, spill_ref, spill_t, counters, stats, header, jobs_buf, chunks_buf, query_work;
-    DevBuf             query_spill_ref, query_spill_t;
+    // batch tracks (path pools)
+    Track              tracks[2];
+    uint32_t           track_count = 1;
+    DevBuf             query_work, query_spill_ref, query_spill_t;
     DevBuf             energy;  // conductor energy table (mrg_energy.h), float4 per node
     void*              noalloc_scratch = nullptr;  // this GPU's address of the allocation-free query's workspace
     std::mutex         query_mutex;
@@ -216,66 +274,103 @@ struct mrg_device {
     std::thread                        worker;
     uint64_t                           resident_bytes = 0;  // scenes + admitted films
     mrg_kernel_times                   times{};
-    std::vector<hipEvent_t>            events;
     hipEvent_t                         query_events[2] = {nullptr, nullptr};
-    std::map<uint32_t, hipGraphExec_t> graphs;  // by bounce count, extended shading, refill traversal, one scene
 };
 
 namespace {
 
 uint64_t budget(const mrg_device* d) { return d->opt.hbm_budget_bytes ? d->opt.hbm_budget_bytes : (uint64_t(64) << 30); }
 
-void alloc_pool(mrg_device* d) {
+void alloc_track(mrg_device* d, Track& t) {
     const uint64_t P = d->opt.path_capacity;
     // per slot: ray_o, ray_d, hit, beta, rad, prev, rng, sh_o, sh_d, sh_c (16 B each) + 4 u32
     const uint64_t per = 10 * 16 + 4 * 4;
-    d->pool_mem.alloc(P * per);
-    char* base = d->pool_mem.as<char>();
+    t.pool_mem.alloc(P * per);
+    char* base = t.pool_mem.as<char>();
     auto take  = [&](uint64_t bytes) {
         char* p = base;
         base += bytes;
         return p;
     };
-    Pool& pl   = d->pool;
-    pl.ray_o   = reinterpret_cast<float4*>(take(P * 16));
-    pl.ray_d   = reinterpret_cast<float4*>(take(P * 16));
-    pl.hit     = reinterpret_cast<float4*>(take(P * 16));
-    pl.beta    = reinterpret_cast<float4*>(take(P * 16));
-    pl.rad     = reinterpret_cast<float4*>(take(P * 16));
-    pl.prev    = reinterpret_cast<float4*>(take(P * 16));
-    pl.rng     = reinterpret_cast<uint4*>(take(P * 16));
-    pl.sh_o    = reinterpret_cast<float4*>(take(P * 16));
-    pl.sh_d    = reinterpret_cast<float4*>(take(P * 16));
-    pl.sh_c    = reinterpret_cast<float4*>(take(P * 16));
+    Pool& pl    = t.pool;
+    pl.ray_o    = reinterpret_cast<float4*>(take(P * 16));
+    pl.ray_d    = reinterpret_cast<float4*>(take(P * 16));
+    pl.hit      = reinterpret_cast<float4*>(take(P * 16));
+    pl.beta     = reinterpret_cast<float4*>(take(P * 16));
+    pl.rad      = reinterpret_cast<float4*>(take(P * 16));
+    pl.prev     = reinterpret_cast<float4*>(take(P * 16));
+    pl.rng      = reinterpret_cast<uint4*>(take(P * 16));
+    pl.sh_o     = reinterpret_cast<float4*>(take(P * 16));
+    pl.sh_d     = reinterpret_cast<float4*>(take(P * 16));
+    pl.sh_c     = reinterpret_cast<float4*>(take(P * 16));
     pl.hit_slot = reinterpret_cast<uint32_t*>(take(P * 4));
-    pl.q_ext0  = reinterpret_cast<uint32_t*>(take(P * 4));
-    pl.q_ext1  = reinterpret_cast<uint32_t*>(take(P * 4));
-    pl.q_sh    = reinterpret_cast<uint32_t*>(take(P * 4));
+    pl.q_ext0   = reinterpret_cast<uint32_t*>(take(P * 4));
+    pl.q_ext1   = reinterpret_cast<uint32_t*>(take(P * 4));
+    pl.q_sh     = reinterpret_cast<uint32_t*>(take(P * 4));
     // spill columns for the larger of the two persistent grids (a lane's column is its global id)
     const uint64_t lanes = uint64_t(std::max({d->shape.traversal_blocks, d->shape.refill_blocks, 1u})) * 256;
-    d->spill_ref.alloc(lanes * (kStackTotal - kStackLds) * 4);
-    d->spill_t.alloc(lanes * (kStackTotal - kStackLds) * 4);
-    d->counters.alloc(kCounterWords * 4);
-    d->stats.alloc(kStatWords * 8);
-    d->header.alloc(kHdrWords * 4);
-    pl.spill_ref   = d->spill_ref.as<uint32_t>();
-    pl.spill_t     = d->spill_t.as<float>();
+    t.spill_ref.alloc(lanes * (kStackTotal - kStackLds) * 4);
+    t.spill_t.alloc(lanes * (kStackTotal - kStackLds) * 4);
+    t.counters.alloc(kCounterWords * 4);
+    t.stats.alloc(kStatWords * 8);
+    t.header.alloc(kHdrWords * 4);
+    pl.spill_ref   = t.spill_ref.as<uint32_t>();
+    pl.spill_t     = t.spill_t.as<float>();
     pl.spill_lanes = uint32_t(lanes);
-    pl.counters    = d->counters.as<uint32_t>();
-    pl.stats       = d->stats.as<uint64_t>();
-    pl.header      = d->header.as<uint32_t>();
+    pl.counters    = t.counters.as<uint32_t>();
+    pl.stats       = t.stats.as<uint64_t>();
+    pl.header      = t.header.as<uint32_t>();
     pl.capacity    = uint32_t(P);
-    d->max_jobs    = std::max(1u, d->opt.max_jobs_in_flight);
-    d->max_chunks  = d->max_jobs * 4;
-    d->jobs_buf.alloc(sizeof(GpuJob) * d->max_jobs);
-    d->chunks_buf.alloc(sizeof(GpuChunk) * d->max_chunks);
+    t.jobs_buf.alloc(sizeof(GpuJob) * d->max_jobs);
+    t.chunks_buf.alloc(sizeof(GpuChunk) * d->max_chunks);
+    t.up_jobs.alloc(sizeof(GpuJob) * d->max_jobs);
+    t.up_chunks.alloc(sizeof(GpuChunk) * d->max_chunks);
+    t.up_header.alloc(kHdrWords * 4);
+    t.down_counters.alloc(kCounterWords * 4);
+    t.down_stats.alloc(kStatWords * 8);
+    t.down_flags.alloc(size_t(d->max_jobs) * 16);
+    check_hip(hipStreamCreateWithFlags(&t.stream, hipStreamNonBlocking), "hipStreamCreate");
+    t.events.resize(2 * (3 * kMaxBounces + 8));
+    for (auto& e : t.events) check_hip(hipEventCreate(&e), "hipEventCreate");
+    check_hip(hipEventCreateWithFlags(&t.committed, hipEventDisableTiming), "hipEventCreate");
+}
+
+uint64_t track_bytes(const Track& t) { return t.pool_mem.bytes + t.spill_ref.bytes + t.spill_t.bytes; }
+
+void free_track(Track& t) {
+    if (t.stream) (void)hipStreamSynchronize(t.stream);
+    for (auto& g : t.graphs) (void)hipGraphExecDestroy(g.second);
+    t.graphs.clear();
+    for (auto& e : t.events) (void)hipEventDestroy(e);
+    t.events.clear();
+    if (t.committed) (void)hipEventDestroy(t.committed);
+    t.committed = nullptr;
+    if (t.stream) (void)hipStreamDestroy(t.stream);
+    t.stream = nullptr;
+    for (DevBuf* b : {&t.pool_mem, &t.spill_ref, &t.spill_t, &t.counters, &t.stats, &t.header, &t.jobs_buf, &t.chunks_buf}) b->release();
+    for (PinnedBuf* b : {&t.up_jobs, &t.up_chunks, &t.up_header, &t.down_counters, &t.down_stats, &t.down_flags}) b->release();
+}
+
+// The device's batch tracks and the ray queries' own spill area. Two tracks (two path pools) when the
+// options ask for two batches in flight. One is the default: on the served stream (64 uneven jobs,
+// 8 scenes) two overlapped 2^26-path pools gave 3,005 Msamples/s at a first-pass p50 of 0.136 s, one
+// 2^27-path pool 3,095 at 0.154 s and one 2^26-path pool 2,900 at 0.098 s: a queued second batch adds
+// a batch of latency for about what a larger single pool gains.
+void alloc_pool(mrg_device* d) {
+    d->max_jobs   = std::max(1u, d->opt.max_jobs_in_flight);
+    d->max_chunks = d->max_jobs * 4;
+    alloc_track(d, d->tracks[0]);
+    d->track_count = d->opt.batches_in_flight == 2 ? 2u : 1u;
+    if (d->track_count == 2) alloc_track(d, d->tracks[1]);
+    const uint64_t lanes = uint64_t(std::max({d->shape.traversal_blocks, d->shape.refill_blocks, 1u})) * 256;
     d->query_work.alloc(64);
     d->query_spill_ref.alloc(lanes * (kStackTotal - kStackLds) * 4);
     d->query_spill_t.alloc(lanes * (kStackTotal - kStackLds) * 4);
-    d->query_pool           = pl;
+    d->query_pool           = d->tracks[0].pool;
     d->query_pool.spill_ref = d->query_spill_ref.as<uint32_t>();
     d->query_pool.spill_t   = d->query_spill_t.as<float>();
-    d->pool_bytes = d->pool_mem.bytes + d->spill_ref.bytes + d->spill_t.bytes + d->query_spill_ref.bytes + d->query_spill_t.bytes;
+    d->pool_bytes = d->query_spill_ref.bytes + d->query_spill_t.bytes;
+    for (uint32_t k = 0; k < d->track_count; ++k) d->pool_bytes += track_bytes(d->tracks[k]);
 }
 
 // Camera constants: CameraRays<float> of the oracle (scene.cpp:171-199 in FP32; tan in FP64).
@@ -658,16 +753,6 @@ void admit_locked(mrg_device* d) {
     }
 }
 
-struct BatchPlan {
-    std::vector<mrg_job*>  jobs;
-    std::vector<GpuJob>    gjobs;
-    std::vector<GpuChunk>  chunks;
-    uint32_t               slots = 0, pixels = 0, max_depth = 0;
-    bool                   extended = false;  // some job's scene needs the extended shading kernel
-    bool                   refill   = false;  // bounce traversal with the refill driver (k_trace_refill)
-    bool                   one_scene = true;  // every job of the batch renders the same scene
-};
-
 // Traversal kernel of a batch under MRG_TRAVERSAL_AUTO: the refill driver (parked leaves, six waves
 // per SIMD) when the batch holds a scene of at least 2^14 triangles and at least 2^23 paths, the
 // fused launch otherwise. Measured refill/fused throughput: by scene (768 x 768 x 64 spp batches,
@@ -685,8 +770,8 @@ bool refill_scene(const mrg_device* d, const mrg_scene* s) {
 }
 
 bool runnable(const mrg_job* j) {
-    return j->admitted && !j->finished && !j->cancel_requested && !j->pause_requested && j->state == MRG_STATE_RUNNING &&
-           j->next_sample < j->desc.samples_per_pixel;
+    return j->admitted && !j->finished && !j->cancel_requested && !j->pause_requested && !j->fail_pending && j->readers == 0 &&
+           j->state == MRG_STATE_RUNNING && j->next_sample < j->desc.samples_per_pixel;
 }
 
 uint32_t pass_of(const mrg_job* j) {
@@ -803,9 +888,8 @@ void plan_group(mrg_device* d, const std::vector<mrg_job*>& group, uint32_t cap,
     }
 }
 
-BatchPlan plan_batch_locked(mrg_device* d) {
+BatchPlan plan_batch_locked(mrg_device* d, uint32_t cap) {
     BatchPlan b;
-    const uint32_t cap      = d->pool.capacity;
     const bool     batched  = d->opt.schedule == MRG_SCHEDULE_BATCHED;
     const bool     fill     = d->opt.schedule == MRG_SCHEDULE_PLAIN_FILL;
     const uint32_t max_jobs = batched ? d->max_jobs : 1u;
@@ -842,7 +926,7 @@ BatchPlan plan_batch_locked(mrg_device* d) {
     }
     if (d->opt.traversal == MRG_TRAVERSAL_AUTO && b.slots < kRefillMinPaths) b.refill = false;
     for (mrg_job* j : b.jobs) {
-        j->in_flight = true;
+        j->in_flight++;
         j->refs++;
     }
     return b;
@@ -850,162 +934,231 @@ BatchPlan plan_batch_locked(mrg_device* d) {
 
 bool fused_traversal(const mrg_device* d) { return d->opt.traversal != MRG_TRAVERSAL_SPLIT; }
 
-void launch_iteration_kernels(mrg_device* d, uint32_t iters, bool extended, bool refill, bool one_scene, bool timed,
-                              std::vector<std::pair<int, int>>& marks) {
-    auto trace = [&](const Pool& P, const GpuJob* jobs, uint32_t it, const LaunchShape& shape, hipStream_t stream) {
-        return refill ? launch_trace_refill(P, jobs, it, one_scene, shape, stream) : launch_trace(P, jobs, it, one_scene, shape, stream);
+// The batch's bounce loop on track t: reset, camera rays, (trace, shade) per bounce, the last shadow rays.
+void launch_bounce_loop(mrg_device* d, Track& t, bool extended, bool refill, bool one_scene, bool timed) {
+    auto trace = [&](uint32_t it) {
+        const GpuJob* jobs = t.jobs_buf.as<GpuJob>();
+        return refill ? launch_trace_refill(t.pool, jobs, it, one_scene, d->shape, t.stream)
+                      : launch_trace(t.pool, jobs, it, one_scene, d->shape, t.stream);
     };
-    const GpuJob*   jobs   = d->jobs_buf.as<GpuJob>();
-    const GpuChunk* chunks = d->chunks_buf.as<GpuChunk>();
-    int             ev     = 0;
+    const GpuJob*   jobs   = t.jobs_buf.as<GpuJob>();
+    const GpuChunk* chunks = t.chunks_buf.as<GpuChunk>();
     auto            rec    = [&](int cls) {
         if (!timed) return;
-        check_hip(hipEventRecord(d->events[ev], d->stream), "hipEventRecord");
-        marks.push_back({ev, cls});
-        ++ev;
+        const int ev = int(t.marks.size());
+        check_hip(hipEventRecord(t.events[ev], t.stream), "hipEventRecord");
+        t.marks.push_back({ev, cls});
     };
-    check_hip(launch_reset(d->pool, d->stream), "k_reset");
+    check_hip(launch_reset(t.pool, t.stream), "k_reset");
     rec(-1);
-    check_hip(launch_generate(d->pool, jobs, chunks, d->shape, d->stream), "k_generate");
+    check_hip(launch_generate(t.pool, jobs, chunks, d->shape, t.stream), "k_generate");
     rec(0);
     if (fused_traversal(d)) {
-        for (uint32_t it = 0; it < iters; ++it) {
-            check_hip(trace(d->pool, jobs, it, d->shape, d->stream), "k_trace");
+        for (uint32_t it = 0; it < t.iters; ++it) {
+            check_hip(trace(it), "k_trace");
             rec(1);
-            check_hip(launch_shade(d->pool, jobs, it, extended, d->shape, d->stream), "k_shade");
+            check_hip(launch_shade(t.pool, jobs, it, extended, d->shape, t.stream), "k_shade");
             rec(2);
         }
         // the last shading pass's shadow rays (its extension queue is empty: no path continues
         // beyond the batch's deepest bounce)
-        check_hip(trace(d->pool, jobs, iters, d->shape, d->stream), "k_trace");
+        check_hip(trace(t.iters), "k_trace");
         rec(1);
     } else {
-        for (uint32_t it = 0; it < iters; ++it) {
-            check_hip(launch_extend(d->pool, jobs, it, d->shape, d->stream), "k_extend");
+        for (uint32_t it = 0; it < t.iters; ++it) {
+            check_hip(launch_extend(t.pool, jobs, it, d->shape, t.stream), "k_extend");
             rec(1);
-            check_hip(launch_shade(d->pool, jobs, it, extended, d->shape, d->stream), "k_shade");
+            check_hip(launch_shade(t.pool, jobs, it, extended, d->shape, t.stream), "k_shade");
             rec(2);
-            check_hip(launch_shadow(d->pool, jobs, it, d->shape, d->stream), "k_shadow");
+            check_hip(launch_shadow(t.pool, jobs, it, d->shape, t.stream), "k_shadow");
             rec(3);
         }
     }
-    check_hip(launch_validate(d->pool, jobs, chunks, d->shape, d->stream), "k_validate");
-    check_hip(launch_accumulate(d->pool, jobs, chunks, d->shape, d->stream), "k_accumulate");
-    rec(4);
 }
 
-void run_batch(mrg_device* d, BatchPlan& b) {
-    const uint32_t iters = b.max_depth + 1;
-    uint32_t       hdr[kHdrWords] = {uint32_t(b.chunks.size()), b.slots, b.pixels, iters};
+// Queues the batch planned into t.plan on track t without waiting for it. `before`: the commit event
+// of the batch queued just before (on the other track), or null; this batch's film commit waits for it.
+void enqueue_batch(mrg_device* d, Track& t, hipEvent_t before) {
+    BatchPlan& b = t.plan;
+    t.iters      = b.max_depth + 1;
+    t.timed      = d->opt.use_graph == 0;
+    t.marks.clear();
     for (GpuChunk& c : b.chunks) {
         const uint32_t w = b.gjobs[c.job].width;
         c.tile_rows      = c.pixel_begin % w == 0 && c.pixel_count % w == 0 ? c.pixel_count / w : 0u;
     }
-    check_hip(hipMemcpyAsync(d->jobs_buf.ptr, b.gjobs.data(), b.gjobs.size() * sizeof(GpuJob), hipMemcpyHostToDevice, d->stream), "upload jobs");
-    check_hip(hipMemcpyAsync(d->chunks_buf.ptr, b.chunks.data(), b.chunks.size() * sizeof(GpuChunk), hipMemcpyHostToDevice, d->stream), "upload chunks");
-    check_hip(hipMemcpyAsync(d->header.ptr, hdr, sizeof hdr, hipMemcpyHostToDevice, d->stream), "upload header");
-    std::vector<std::pair<int, int>> marks;
-    const bool timed = d->opt.use_graph == 0;
+    const uint32_t hdr[kHdrWords] = {uint32_t(b.chunks.size()), b.slots, b.pixels, t.iters};
+    std::memcpy(t.up_jobs.ptr, b.gjobs.data(), b.gjobs.size() * sizeof(GpuJob));
+    std::memcpy(t.up_chunks.ptr, b.chunks.data(), b.chunks.size() * sizeof(GpuChunk));
+    std::memcpy(t.up_header.ptr, hdr, sizeof hdr);
+    check_hip(hipMemcpyAsync(t.jobs_buf.ptr, t.up_jobs.ptr, b.gjobs.size() * sizeof(GpuJob), hipMemcpyHostToDevice, t.stream), "upload jobs");
+    check_hip(hipMemcpyAsync(t.chunks_buf.ptr, t.up_chunks.ptr, b.chunks.size() * sizeof(GpuChunk), hipMemcpyHostToDevice, t.stream),
+              "upload chunks");
+    check_hip(hipMemcpyAsync(t.header.ptr, t.up_header.ptr, sizeof hdr, hipMemcpyHostToDevice, t.stream), "upload header");
     if (d->opt.use_graph) {
-        const uint32_t key = ((iters * 2u + (b.extended ? 1u : 0u)) * 2u + (b.refill ? 1u : 0u)) * 2u + (b.one_scene ? 1u : 0u);
-        auto           it  = d->graphs.find(key);
-        if (it == d->graphs.end()) {
+        const uint32_t key = ((t.iters * 2u + (b.extended ? 1u : 0u)) * 2u + (b.refill ? 1u : 0u)) * 2u + (b.one_scene ? 1u : 0u);
+        auto           it  = t.graphs.find(key);
+        if (it == t.graphs.end()) {
             hipGraph_t graph;
-            check_hip(hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-            launch_iteration_kernels(d, iters, b.extended, b.refill, b.one_scene, false, marks);
-            check_hip(hipStreamEndCapture(d->stream, &graph), "hipStreamEndCapture");
+            check_hip(hipStreamBeginCapture(t.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+            launch_bounce_loop(d, t, b.extended, b.refill, b.one_scene, false);
+            check_hip(hipStreamEndCapture(t.stream, &graph), "hipStreamEndCapture");
             hipGraphExec_t exec;
             check_hip(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
             (void)hipGraphDestroy(graph);
-            it = d->graphs.emplace(key, exec).first;
+            it = t.graphs.emplace(key, exec).first;
         }
-        check_hip(hipGraphLaunch(it->second, d->stream), "hipGraphLaunch");
+        check_hip(hipGraphLaunch(it->second, t.stream), "hipGraphLaunch");
     } else {
-        launch_iteration_kernels(d, iters, b.extended, b.refill, b.one_scene, timed, marks);
+        launch_bounce_loop(d, t, b.extended, b.refill, b.one_scene, t.timed);
     }
-    std::vector<uint32_t> ctr(kCounterWords);
-    uint64_t              st[kStatWords];
-    check_hip(hipMemcpyAsync(ctr.data(), d->counters.ptr, kCounterWords * 4, hipMemcpyDeviceToHost, d->stream), "read counters");
-    check_hip(hipMemcpyAsync(st, d->stats.ptr, sizeof st, hipMemcpyDeviceToHost, d->stream), "read stats");
-    std::vector<uint32_t> flags(b.jobs.size() * 4);
+    // film commit, in batch order: pixels shared with the previous batch get its samples first
+    if (before) check_hip(hipStreamWaitEvent(t.stream, before, 0), "hipStreamWaitEvent");
+    if (t.timed) {  // the commit's interval starts after the wait
+        const int ev = int(t.marks.size());
+        check_hip(hipEventRecord(t.events[ev], t.stream), "hipEventRecord");
+        t.marks.push_back({ev, -1});
+    }
+    const GpuJob*   jobs   = t.jobs_buf.as<GpuJob>();
+    const GpuChunk* chunks = t.chunks_buf.as<GpuChunk>();
+    check_hip(launch_validate(t.pool, jobs, chunks, d->shape, t.stream), "k_validate");
+    check_hip(launch_accumulate(t.pool, jobs, chunks, d->shape, t.stream), "k_accumulate");
+    if (t.timed) {
+        const int ev = int(t.marks.size());
+        check_hip(hipEventRecord(t.events[ev], t.stream), "hipEventRecord");
+        t.marks.push_back({ev, 4});
+    }
+    check_hip(hipEventRecord(t.committed, t.stream), "hipEventRecord");
+    check_hip(hipMemcpyAsync(t.down_counters.ptr, t.counters.ptr, kCounterWords * 4, hipMemcpyDeviceToHost, t.stream), "read counters");
+    check_hip(hipMemcpyAsync(t.down_stats.ptr, t.stats.ptr, kStatWords * 8, hipMemcpyDeviceToHost, t.stream), "read stats");
     for (size_t i = 0; i < b.jobs.size(); ++i)
-        check_hip(hipMemcpyAsync(&flags[i * 4], b.jobs[i]->error_flag.ptr, 16, hipMemcpyDeviceToHost, d->stream), "read flags");
-    check_hip(hipStreamSynchronize(d->stream), "hipStreamSynchronize");
-    uint64_t ext = 0, sh = 0;
-    for (uint32_t i = 0; i < iters; ++i) {
-        ext += ctr[kCtrExt + i];
-        sh += ctr[kCtrShadow + i];
-    }
-    double acc[5] = {0, 0, 0, 0, 0};
-    if (timed && !marks.empty()) {
-        for (size_t i = 1; i < marks.size(); ++i) {
-            float ms = 0;
-            check_hip(hipEventElapsedTime(&ms, d->events[marks[i - 1].first], d->events[marks[i].first]), "hipEventElapsedTime");
-            acc[marks[i].second] += ms;
+        check_hip(hipMemcpyAsync(t.down_flags.as<uint32_t>() + i * 4, b.jobs[i]->error_flag.ptr, 16, hipMemcpyDeviceToHost, t.stream),
+                  "read flags");
+    t.busy = true;
+}
+
+// A landed batch's jobs: samples recorded, statistics, first pass, and the terminal or paused state once
+// the job has no other batch in flight. Device mutex held.
+void settle_batch_locked(mrg_device* d, Track& t, const double* acc, bool failed, const std::string& why) {
+    BatchPlan&      b     = t.plan;
+    const uint32_t* ctr   = t.down_counters.as<uint32_t>();
+    const uint64_t* st    = t.down_stats.as<uint64_t>();
+    const uint32_t* flags = t.down_flags.as<uint32_t>();
+    uint64_t        ext = 0, sh = 0;
+    if (!failed)
+        for (uint32_t i = 0; i < t.iters; ++i) {
+            ext += ctr[kCtrExt + i];
+            sh += ctr[kCtrShadow + i];
         }
-    }
-    std::lock_guard<std::mutex> lock(d->mutex);
     d->times.generate_ms += acc[0];
     d->times.extend_ms += acc[1];
     d->times.shade_ms += acc[2];
     d->times.shadow_ms += acc[3];
     d->times.accumulate_ms += acc[4];
     d->times.batches++;
-    d->times.extend_launches += fused_traversal(d) ? iters + 1 : iters;
-    d->times.shadow_launches += fused_traversal(d) ? 0 : iters;
-    d->times.refill_launches += fused_traversal(d) && b.refill ? iters + 1 : 0;
+    d->times.extend_launches += fused_traversal(d) ? t.iters + 1 : t.iters;
+    d->times.shadow_launches += fused_traversal(d) ? 0 : t.iters;
+    d->times.refill_launches += fused_traversal(d) && b.refill ? t.iters + 1 : 0;
     d->times.extension_rays += ext;
     d->times.shadow_rays += sh;
     const bool shared = b.jobs.size() > 1;
     for (size_t i = 0; i < b.jobs.size(); ++i) {
         mrg_job* j = b.jobs[i];
-        uint64_t samples = 0;
-        for (const auto& c : b.chunks)
-            if (c.job == i) samples += uint64_t(c.pixel_count) * c.sample_count;
-        if (flags[i * 4] == 0) {
+        j->in_flight--;
+        const uint32_t flag = failed ? 0u : flags[i * 4];
+        if (failed) {
+            if (!j->fail_pending) {
+                j->status = MRG_DEVICE_ERROR;
+                j->error  = why;
+            }
+            j->fail_pending = true;
+        } else if (flag != 0) {
+            if (!j->fail_pending) {
+                j->status = MRG_NONFINITE;
+                j->error  = (flag & 1u) ? "Path contribution is nonfinite or negative" : "Render sample exceeds the float image range";
+            }
+            j->fail_pending = true;
+        } else if (!j->fail_pending && !j->finished) {
+            uint64_t samples = 0;
+            for (const auto& c : b.chunks)
+                if (c.job == i) samples += uint64_t(c.pixel_count) * c.sample_count;
             j->recorded += samples;
             j->paths += samples;
-        }
-        if (!shared) {
-            j->ext_rays += ext;
-            j->shadow_rays += st[kStatShadowRays];
-            j->scatter += st[kStatScatter];
-            j->rr_term += st[kStatRoulette];
+            if (!shared) {
+                j->ext_rays += ext;
+                j->shadow_rays += st[kStatShadowRays];
+                j->scatter += st[kStatScatter];
+                j->rr_term += st[kStatRoulette];
+            }
         }
         const uint64_t pixels = uint64_t(j->desc.width) * j->desc.height;
-        j->completed_pixels   = j->next_pixel == 0 && j->next_sample == j->desc.samples_per_pixel ? pixels : j->completed_pixels;
-        if (!j->first_pass_done && (j->next_sample >= std::min(pass_of(j), j->desc.samples_per_pixel)) && j->next_pixel == 0) {
+        if (!j->first_pass_done && j->recorded >= uint64_t(std::min(pass_of(j), j->desc.samples_per_pixel)) * pixels) {
             j->first_pass_done = true;
             j->first_pass_at   = Clock::now();
         }
-        j->in_flight = false;
-        if (flags[i * 4] != 0) {
-            j->status = MRG_NONFINITE;
-            j->error  = (flags[i * 4] & 1u) ? "Path contribution is nonfinite or negative" : "Render sample exceeds the float image range";
-            finish_job_locked(d, j, MRG_STATE_FAILED);
-        } else if (j->cancel_requested) {
-            j->status = MRG_CANCELLED;
-            finish_job_locked(d, j, MRG_STATE_CANCELLED);
-        } else if (j->next_sample >= j->desc.samples_per_pixel && j->next_pixel == 0) {
-            j->state = MRG_STATE_FINALIZING;
-            finish_job_locked(d, j, MRG_STATE_COMPLETED);
-        } else if (j->pause_requested) {
-            j->state = MRG_STATE_PAUSED;
+        const bool all_planned = j->next_sample >= j->desc.samples_per_pixel && j->next_pixel == 0;
+        if (j->in_flight == 0 && !j->finished) {
+            if (j->fail_pending) {
+                finish_job_locked(d, j, MRG_STATE_FAILED);
+            } else if (j->cancel_requested) {
+                j->status = MRG_CANCELLED;
+                finish_job_locked(d, j, MRG_STATE_CANCELLED);
+            } else if (all_planned) {
+                j->completed_pixels = pixels;
+                j->state            = MRG_STATE_FINALIZING;
+                finish_job_locked(d, j, MRG_STATE_COMPLETED);
+            } else if (j->pause_requested) {
+                j->state = MRG_STATE_PAUSED;
+            }
         }
-        j->film_mutex.unlock();
         release_job_locked(j);
     }
+    b = BatchPlan{};
+    t.busy = false;
     d->cv.notify_all();
+}
+
+// Waits for track t's batch, reads its kernel times and settles its jobs.
+void land_batch(mrg_device* d, Track& t) {
+    double      acc[5] = {0, 0, 0, 0, 0};
+    bool        failed = false;
+    std::string why;
+    try {
+        check_hip(hipStreamSynchronize(t.stream), "hipStreamSynchronize");
+        for (size_t i = 1; t.timed && i < t.marks.size(); ++i) {
+            float ms = 0;
+            check_hip(hipEventElapsedTime(&ms, t.events[t.marks[i - 1].first], t.events[t.marks[i].first]), "hipEventElapsedTime");
+            if (t.marks[i].second >= 0) acc[t.marks[i].second] += ms;
+        }
+    } catch (const std::exception& e) {
+        failed = true;
+        why    = e.what();
+    }
+    std::lock_guard<std::mutex> lock(d->mutex);
+    settle_batch_locked(d, t, acc, failed, why);
+}
+
+// Enqueue failure: the planned batch never ran; its jobs fail once nothing else of theirs is in flight.
+void abandon_batch(mrg_device* d, Track& t, const std::string& why) {
+    (void)hipStreamSynchronize(t.stream);
+    std::lock_guard<std::mutex> lock(d->mutex);
+    const double acc[5] = {0, 0, 0, 0, 0};
+    settle_batch_locked(d, t, acc, true, why);
 }
 
 void scheduler(mrg_device* d) {
     (void)hipSetDevice(d->ordinal);
+    std::deque<Track*> inflight;  // in queue order
+    hipEvent_t         last_commit = nullptr;  // commit event of the most recently queued batch
     for (;;) {
-        BatchPlan b;
+        Track* free_track = nullptr;
+        for (uint32_t k = 0; k < d->track_count && !free_track; ++k)
+            if (!d->tracks[k].busy) free_track = &d->tracks[k];
+        bool planned = false;
         {
             std::unique_lock<std::mutex> lock(d->mutex);
             for (;;) {
-                if (d->shutdown) {
+                if (d->shutdown && inflight.empty()) {
                     // every job still queued ends cancelled (its film, if admitted, retired to host memory
                     // so it stays readable), waiters wake, and the queue drops its references
                     for (mrg_job* j : d->jobs) {
@@ -1025,56 +1178,68 @@ void scheduler(mrg_device* d) {
                     d->cv.notify_all();
                     return;
                 }
-                try {
-                    admit_locked(d);
-                } catch (const std::exception& e) {
-                    for (mrg_job* j : d->jobs)
-                        if (!j->admitted && !j->finished) {
-                            j->status = MRG_DEVICE_ERROR;
-                            j->error  = e.what();
-                            finish_job_locked(d, j, MRG_STATE_FAILED);
+                if (!d->shutdown) {
+                    try {
+                        admit_locked(d);
+                    } catch (const std::exception& e) {
+                        for (mrg_job* j : d->jobs)
+                            if (!j->admitted && !j->finished) {
+                                j->status = MRG_DEVICE_ERROR;
+                                j->error  = e.what();
+                                finish_job_locked(d, j, MRG_STATE_FAILED);
+                                break;
+                            }
+                    }
+                    // settle pause/cancel of jobs with no batch in flight
+                    for (mrg_job* j : d->jobs) {
+                        if (j->finished || j->in_flight > 0) continue;
+                        if (j->cancel_requested && j->admitted) {
+                            j->status = MRG_CANCELLED;
+                            finish_job_locked(d, j, MRG_STATE_CANCELLED);
+                        } else if (j->pause_requested && j->state == MRG_STATE_PAUSING) {
+                            j->state = MRG_STATE_PAUSED;
+                            d->cv.notify_all();
+                        }
+                    }
+                    // drop finished jobs from the queue (the caller's handle stays valid)
+                    for (auto it = d->jobs.begin(); it != d->jobs.end();) {
+                        if ((*it)->finished) {
+                            mrg_job* j = *it;
+                            it         = d->jobs.erase(it);
+                            release_job_locked(j);
+                        } else {
+                            ++it;
+                        }
+                    }
+                    if (free_track) {
+                        free_track->plan = plan_batch_locked(d, free_track->pool.capacity);
+                        if (!free_track->plan.jobs.empty()) {
+                            planned = true;
                             break;
                         }
-                }
-                // settle pause/cancel of jobs not in flight
-                for (mrg_job* j : d->jobs) {
-                    if (j->finished || j->in_flight) continue;
-                    if (j->cancel_requested && j->admitted) {
-                        j->status = MRG_CANCELLED;
-                        finish_job_locked(d, j, MRG_STATE_CANCELLED);
-                    } else if (j->pause_requested && j->state == MRG_STATE_PAUSING) {
-                        j->state = MRG_STATE_PAUSED;
-                        d->cv.notify_all();
                     }
                 }
-                // drop finished jobs from the queue (the caller's handle stays valid)
-                for (auto it = d->jobs.begin(); it != d->jobs.end();) {
-                    if ((*it)->finished) {
-                        mrg_job* j = *it;
-                        it         = d->jobs.erase(it);
-                        release_job_locked(j);
-                    } else {
-                        ++it;
-                    }
-                }
-                b = plan_batch_locked(d);
-                if (!b.jobs.empty()) break;
+                if (!inflight.empty()) break;  // land the oldest batch
                 d->cv.wait(lock);
             }
         }
-        for (mrg_job* j : b.jobs) j->film_mutex.lock();
-        try {
-            run_batch(d, b);
-        } catch (const std::exception& e) {
-            std::lock_guard<std::mutex> lock(d->mutex);
-            for (mrg_job* j : b.jobs) {
-                j->in_flight = false;
-                j->status    = MRG_DEVICE_ERROR;
-                j->error     = e.what();
-                if (!j->finished) finish_job_locked(d, j, MRG_STATE_FAILED);
-                j->film_mutex.unlock();
-                release_job_locked(j);
+        if (planned) {
+            try {
+                enqueue_batch(d, *free_track, last_commit);
+                last_commit = free_track->committed;
+                inflight.push_back(free_track);
+            } catch (const std::exception& e) {
+                abandon_batch(d, *free_track, e.what());
             }
+            // keep a second batch rendering while the first drains its tail
+            bool more = false;
+            for (uint32_t k = 0; k < d->track_count; ++k) more = more || !d->tracks[k].busy;
+            if (more) continue;
+        }
+        if (!inflight.empty()) {
+            Track* t = inflight.front();
+            inflight.pop_front();
+            land_batch(d, *t);
         }
     }
 }
@@ -1154,14 +1319,13 @@ int mrg_device_open(int ordinal, const mrg_device_options* options, mrg_device**
         if (d->opt.traversal > MRG_TRAVERSAL_REFILL) throw std::invalid_argument("unknown traversal kernel mode");
         if (d->opt.max_jobs_in_flight == 0) d->opt.max_jobs_in_flight = 64;
         if (d->opt.max_jobs_in_flight > 4096) throw std::invalid_argument("max_jobs_in_flight too large");
+        if (d->opt.batches_in_flight > 2) throw std::invalid_argument("batches_in_flight must be 0 (automatic), 1 or 2");
         alloc_pool(d.get());
         // the allocation-free query's workspace: one static array per GPU, looked up with the device current
         check_hip(noalloc_scratch_address(&d->noalloc_scratch), "hipGetSymbolAddress(scratch)");
         upload(d->energy, conductor_energy_table());
-        d->pool.energy       = d->energy.as<float>();
+        for (Track& t : d->tracks) t.pool.energy = d->energy.as<float>();
         d->query_pool.energy = d->energy.as<float>();
-        d->events.resize(2 * (3 * kMaxBounces + 8));
-        for (auto& e : d->events) check_hip(hipEventCreate(&e), "hipEventCreate");
         for (auto& e : d->query_events) check_hip(hipEventCreate(&e), "hipEventCreate");
         d->resident_bytes = 0;
         mrg_device* raw   = d.release();
@@ -1187,18 +1351,13 @@ int mrg_device_close(mrg_device* d) {
             std::lock_guard<std::mutex> q(d->query_mutex);  // no ray query is using the streams
             (void)hipStreamSynchronize(d->stream);
             (void)hipStreamSynchronize(d->query_stream);
-            for (auto& g : d->graphs) (void)hipGraphExecDestroy(g.second);
-            d->graphs.clear();
-            for (auto& e : d->events) (void)hipEventDestroy(e);
-            d->events.clear();
+            for (Track& t : d->tracks) free_track(t);
             for (auto& e : d->query_events) (void)hipEventDestroy(e);
             (void)hipStreamDestroy(d->stream);
             (void)hipStreamDestroy(d->copy_stream);
             (void)hipStreamDestroy(d->query_stream);
             d->stream = d->copy_stream = d->query_stream = nullptr;
-            for (DevBuf* b : {&d->pool_mem, &d->spill_ref, &d->spill_t, &d->counters, &d->stats, &d->header, &d->jobs_buf,
-                              &d->chunks_buf, &d->query_work, &d->query_spill_ref, &d->query_spill_t, &d->energy})
-                b->release();
+            for (DevBuf* b : {&d->query_work, &d->query_spill_ref, &d->query_spill_t, &d->energy}) b->release();
             d->closed = true;
         }
         bool last;
@@ -1526,10 +1685,29 @@ int mrg_job_progress(mrg_job* j, mrg_progress* p) {
     });
 }
 
+// A film read waits until the job's batches in flight have landed (no new batch of the job starts
+// meanwhile), so it never sees a partly committed batch.
 static void read_film_raw(mrg_job* j, std::vector<double>& sums, std::vector<uint32_t>& counts) {
     const uint64_t px = uint64_t(j->desc.width) * j->desc.height;
     sums.assign(px * 3, 0.0);
     counts.assign(px, 0);
+    mrg_device* d = j->device;
+    {
+        std::unique_lock<std::mutex> lock(d->mutex);
+        j->readers++;
+        d->cv.wait(lock, [&] { return j->in_flight == 0; });
+    }
+    struct Done {
+        mrg_device* d;
+        mrg_job*    j;
+        ~Done() {
+            {
+                std::lock_guard<std::mutex> lock(d->mutex);
+                j->readers--;
+            }
+            d->cv.notify_all();
+        }
+    } done{d, j};
     std::lock_guard<std::mutex> fl(j->film_mutex);
     if (j->film_on_host) {  // finished: the film was copied out of HBM
         sums   = j->host_sums;

@@ -232,3 +232,53 @@ def test_finished_films_leave_the_admission_budget():
             j.release()
         assert d.memory()["resident_bytes"] == m["resident_bytes"]
         bvh.release()
+
+
+@pytest.mark.parametrize("schedule", ["plain_fill", "batched"])
+def test_two_batches_in_flight_keep_films_and_session_semantics(schedule):
+    """Two path pools: the next batch renders while the previous one drains. Films still commit in batch
+    order, so every job equals its plain single-pool render bit for bit; pause parks the job once both
+    batches have landed, previews read whole batches only, cancel keeps the recorded samples."""
+    scenes = [scenegen.cornell_box(), scenegen.mixed_terrain(cells=48, spheres=8, sphere_level=2)]
+    specs = [(0, 96, 64, 24, 8), (1, 128, 72, 16, 16), (0, 33, 17, 40, 6), (1, 64, 48, 12, 10)]
+    films = {}
+    for mode, tracks in (("plain", 1), (schedule, 2)):
+        print(f"[two tracks] {mode} batches_in_flight={tracks}", flush=True)
+        with api.Device(0, schedule=mode, path_capacity=1 << 15, batches_in_flight=tracks) as d:
+            bvhs = [d.upload(s) for s in scenes]
+            jobs = [api.Job(bvhs[k], scenes[k].camera, api.RenderSettings(w, h, n, 3 + i), api.IntegratorSettings("mis", dep),
+                            pass_samples=2) for i, (k, w, h, n, dep) in enumerate(specs)]
+            for j in jobs:
+                assert j.wait(60000)
+                assert j.progress().state == "completed"
+            films[mode] = [j.film() for j in jobs]
+            for j in jobs:
+                j.release()
+            for b in bvhs:
+                b.release()
+    for (s0, c0), (s1, c1) in zip(films["plain"], films[schedule]):
+        np.testing.assert_array_equal(c0, c1)
+        np.testing.assert_array_equal(s0, s1)
+    with api.Device(0, schedule=schedule, path_capacity=1 << 14, batches_in_flight=2) as d:
+        s = api.RenderSession(d)
+        s.start(sky(), api.RenderSettings(64, 40, 1000000, 31), api.IntegratorSettings(), pass_samples=1)
+        wait_until(lambda: s.progress().recorded_samples > 0)
+        for _ in range(6):
+            img, rec = s.preview(0)
+            sums, counts = s.film()
+            assert int(counts.sum()) % (64 * 40) == 0  # whole passes only
+        assert s.pause()
+        wait_until(lambda: s.progress().state == "paused")
+        count = s.progress().recorded_samples
+        time.sleep(0.05)
+        assert s.progress().recorded_samples == count
+        sums, counts = s.film()
+        assert int(counts.sum()) == count
+        assert s.resume()
+        wait_until(lambda: s.progress().recorded_samples > count)
+        s.cancel()
+        assert s.wait_for(10000)
+        p = s.progress()
+        assert p.state == "cancelled"
+        sums, counts = s.film()
+        assert int(counts.sum()) == p.recorded_samples
