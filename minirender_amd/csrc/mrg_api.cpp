@@ -715,6 +715,16 @@ void finish_job_locked(mrg_device* d, mrg_job* j, uint32_t state) {
     d->cv.notify_all();
 }
 
+// Raises an admitted, unfinished job's cancel word for the kernels of its batches in flight: their
+// shading ends its paths at the next bounce and their film commit skips it. Device mutex held.
+void signal_cancel_locked(mrg_device* d, mrg_job* j) {
+    if (!j->admitted || j->finished || d->closed) return;
+    std::lock_guard<std::mutex> fl(j->film_mutex);
+    if (j->film_on_host || !j->error_flag.ptr) return;
+    (void)hipSetDevice(d->ordinal);
+    if (launch_signal(j->error_flag.as<uint32_t>() + 1, 1u, d->copy_stream) == hipSuccess) (void)hipStreamSynchronize(d->copy_stream);
+}
+
 // Moves the oldest finished films to host memory until `needed` more bytes fit the budget (or
 // none are left); returns whether they fit. Scheduler thread, between batches.
 bool make_room_locked(mrg_device* d, uint64_t needed) {
@@ -1021,6 +1031,7 @@ void enqueue_batch(mrg_device* d, Track& t, hipEvent_t before) {
     }
     const GpuJob*   jobs   = t.jobs_buf.as<GpuJob>();
     const GpuChunk* chunks = t.chunks_buf.as<GpuChunk>();
+    check_hip(launch_commit_gate(jobs, uint32_t(b.jobs.size()), t.stream), "k_commit_gate");
     check_hip(launch_validate(t.pool, jobs, chunks, d->shape, t.stream), "k_validate");
     check_hip(launch_accumulate(t.pool, jobs, chunks, d->shape, t.stream), "k_accumulate");
     if (t.timed) {
@@ -1065,20 +1076,21 @@ void settle_batch_locked(mrg_device* d, Track& t, const double* acc, bool failed
     for (size_t i = 0; i < b.jobs.size(); ++i) {
         mrg_job* j = b.jobs[i];
         j->in_flight--;
-        const uint32_t flag = failed ? 0u : flags[i * 4];
+        const uint32_t flag  = failed ? 0u : flags[i * 4];
+        const bool     gated = !failed && flags[i * 4 + 2] != 0u;  // cancelled during the batch: nothing committed
         if (failed) {
             if (!j->fail_pending) {
                 j->status = MRG_DEVICE_ERROR;
                 j->error  = why;
             }
             j->fail_pending = true;
-        } else if (flag != 0) {
+        } else if (flag != 0 && !gated) {
             if (!j->fail_pending) {
                 j->status = MRG_NONFINITE;
                 j->error  = (flag & 1u) ? "Path contribution is nonfinite or negative" : "Render sample exceeds the float image range";
             }
             j->fail_pending = true;
-        } else if (!j->fail_pending && !j->finished) {
+        } else if (!gated && !j->fail_pending && !j->finished) {
             uint64_t samples = 0;
             for (const auto& c : b.chunks)
                 if (c.job == i) samples += uint64_t(c.pixel_count) * c.sample_count;
@@ -1340,7 +1352,10 @@ int mrg_device_close(mrg_device* d) {
         {
             std::lock_guard<std::mutex> lock(d->mutex);
             if (d->shutdown) throw std::invalid_argument("device is already closed");
-            for (mrg_job* j : d->jobs) j->cancel_requested = true;
+            for (mrg_job* j : d->jobs) {
+                j->cancel_requested = true;
+                signal_cancel_locked(d, j);
+            }
             d->shutdown = true;
             d->cv.notify_all();
         }
@@ -1641,6 +1656,7 @@ int mrg_job_cancel(mrg_job* j) {
         j->cancel_requested = true;
         j->pause_requested  = false;
         j->state            = MRG_STATE_CANCELLING;
+        signal_cancel_locked(j->device, j);
         j->device->cv.notify_all();
     });
 }
@@ -1783,6 +1799,7 @@ int mrg_job_release(mrg_job* j) {
             if (!terminal(j->state)) {
                 j->cancel_requested = true;
                 j->state            = MRG_STATE_CANCELLING;
+                signal_cancel_locked(d, j);
                 d->cv.notify_all();
             }
             last = release_job_locked(j);

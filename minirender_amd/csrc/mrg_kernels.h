@@ -102,6 +102,12 @@ hipError_t launch_trace(const Pool& P, const GpuJob* jobs, uint32_t iter, bool o
 hipError_t launch_trace_refill(const Pool& P, const GpuJob* jobs, uint32_t iter, bool one_scene, const LaunchShape& shape,
                                hipStream_t stream);
 hipError_t launch_validate(const Pool& P, const GpuJob* jobs, const GpuChunk* chunks, const LaunchShape& shape, hipStream_t stream);
+// Cancellation of a job while its batch renders: the host raises word 1 of the job's flags (k_signal);
+// shading ends the job's paths at their next bounce, and before the film commit the batch freezes the
+// word into word 2 (k_commit_gate) so that validation and accumulation either commit all of the job's
+// samples of the batch or none.
+hipError_t launch_signal(uint32_t* word, uint32_t value, hipStream_t stream);
+hipError_t launch_commit_gate(const GpuJob* jobs, uint32_t job_count, hipStream_t stream);
 hipError_t launch_accumulate(const Pool& P, const GpuJob* jobs, const GpuChunk* chunks, const LaunchShape& shape, hipStream_t stream);
 
 }  // namespace mrg

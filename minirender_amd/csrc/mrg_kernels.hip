@@ -626,6 +626,9 @@ template <bool EXT>
 __device__ __forceinline__ void shade_path(const Pool& P, const GpuJob& job, uint32_t path, float4 pv, uint32_t iter, bool& cont,
                                            bool& emit, uint32_t& ext_key, uint32_t& sh_key, float4& so, float4& sd, float4& sc4,
                                            uint32_t& n_scatter, uint32_t& n_rr, uint32_t& n_shadow) {
+        // the cancel word, read first and used last so that its latency hides behind the shading; the host
+        // raises it while kernels run, so it is read past the per-XCD L2s
+        const bool     cancelled = __hip_atomic_load(job.error_flag + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
         const uint32_t flags = __float_as_uint(pv.w);
         const uint32_t depth = iter;
         const GpuScene& sc  = job.scene;
@@ -814,6 +817,10 @@ __device__ __forceinline__ void shade_path(const Pool& P, const GpuJob& job, uin
             P.rng[path]   = make_uint4(static_cast<uint32_t>(rng.state), static_cast<uint32_t>(rng.state >> 32),
                                        static_cast<uint32_t>(rng.inc), static_cast<uint32_t>(rng.inc >> 32));
         } while (false);
+        if (cancelled) {  // the job was cancelled: its paths stop here and its batch commits none of them
+            cont = false;
+            emit = false;
+        }
         P.rad[path] = make_float4(L.x, L.y, L.z, eta_scale);
         if (bad) atomicOr(job.error_flag, 1u);
 }
@@ -894,6 +901,7 @@ __global__ void __launch_bounds__(256) k_validate(Pool P, const GpuJob* __restri
         const GpuJob&  job = jobs[ch.job];
         uint32_t       px, py;
         chunk_xy(ch, job.width, p, px, py);
+        if (job.error_flag[2] != 0u) continue;  // cancelled during the batch: nothing of it is committed
         const uint32_t pixel = py * job.width + px;
         double sx = job.film_sums[static_cast<size_t>(pixel) * 3 + 0];
         double sy = job.film_sums[static_cast<size_t>(pixel) * 3 + 1];
@@ -914,6 +922,19 @@ __global__ void __launch_bounds__(256) k_validate(Pool P, const GpuJob* __restri
     }
 }
 
+// One word raised for kernels that are running (device-scope store).
+__global__ void k_signal(uint32_t* word, uint32_t value) {
+    if (threadIdx.x == 0) __hip_atomic_store(word, value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Before a batch's film commit: each job's cancel word, frozen for validation and accumulation.
+__global__ void k_commit_gate(const GpuJob* __restrict__ jobs, uint32_t job_count) {
+    for (uint32_t i = threadIdx.x; i < job_count; i += blockDim.x) {
+        uint32_t* f = jobs[i].error_flag;
+        f[2] = __hip_atomic_load(f + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
 // Film commit: per pixel, samples in index order, FP64 sums (render_session.cpp:39-74).
 __global__ void __launch_bounds__(256) k_accumulate(Pool P, const GpuJob* __restrict__ jobs, const GpuChunk* __restrict__ chunks) {
     const uint32_t chunk_count  = P.header[kHdrChunks];
@@ -923,7 +944,7 @@ __global__ void __launch_bounds__(256) k_accumulate(Pool P, const GpuJob* __rest
         const GpuChunk ch = chunks[ci];
         const uint32_t p  = g - ch.pixel_prefix;
         const GpuJob&  job = jobs[ch.job];
-        if (*job.error_flag != 0u) continue;  // a failed batch publishes nothing
+        if (job.error_flag[0] != 0u || job.error_flag[2] != 0u) continue;  // a failed or cancelled batch publishes nothing
         uint32_t       px, py;
         chunk_xy(ch, job.width, p, px, py);
         const uint32_t pixel = py * job.width + px;
@@ -1017,6 +1038,16 @@ hipError_t launch_validate(const Pool& P, const GpuJob* jobs, const GpuChunk* ch
 
 hipError_t launch_accumulate(const Pool& P, const GpuJob* jobs, const GpuChunk* chunks, const LaunchShape& shape, hipStream_t stream) {
     hipLaunchKernelGGL(k_accumulate, dim3(grid_for(shape, 4)), dim3(256), 0, stream, P, jobs, chunks);
+    return hipGetLastError();
+}
+
+hipError_t launch_signal(uint32_t* word, uint32_t value, hipStream_t stream) {
+    hipLaunchKernelGGL(k_signal, dim3(1), dim3(64), 0, stream, word, value);
+    return hipGetLastError();
+}
+
+hipError_t launch_commit_gate(const GpuJob* jobs, uint32_t job_count, hipStream_t stream) {
+    hipLaunchKernelGGL(k_commit_gate, dim3(1), dim3(64), 0, stream, jobs, job_count);
     return hipGetLastError();
 }
 

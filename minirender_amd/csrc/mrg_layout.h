@@ -174,7 +174,7 @@ struct GpuJob {
     uint64_t    seed;
     double*     film_sums;    // [w*h*3]
     uint32_t*   film_counts;  // [w*h]
-    uint32_t*   error_flag;
+    uint32_t*   error_flag;   // [0] error bits, [1] cancel requested (host), [2] this batch's commit gate
     uint32_t    pad;
 };
 

@@ -282,3 +282,33 @@ def test_two_batches_in_flight_keep_films_and_session_semantics(schedule):
         assert p.state == "cancelled"
         sums, counts = s.film()
         assert int(counts.sum()) == p.recorded_samples
+
+
+def test_cancel_lands_within_a_bounce_not_a_batch():
+    """Cancellation reaches a batch in flight: its shading ends the job's paths at the next bounce and its
+    film commit skips the job, so cancel-to-terminal takes a fraction of a batch (the reference checks
+    cancellation per BVH node, bvh.cpp:237, and per bounce, path_tracer.cpp:165). The film keeps exactly
+    the recorded samples."""
+    scene = scenegen.mixed_terrain(cells=256, spheres=16, sphere_level=2)
+    with api.Device(0, schedule="plain_fill", path_capacity=1 << 27) as d:
+        bvh = d.upload(scene)
+        job = api.Job(bvh, scene.camera, api.RenderSettings(2048, 2048, 4096, 1), api.IntegratorSettings("mis", 16), 32)
+        wait_until(lambda: job.progress().recorded_samples > 0, timeout=60)
+        t0, r0 = time.perf_counter(), job.progress().recorded_samples
+        wait_until(lambda: job.progress().recorded_samples > r0, timeout=60)
+        batch = time.perf_counter() - t0  # one batch of 2^27 paths
+        r1 = job.progress().recorded_samples
+        time.sleep(batch * 0.3)
+        t1 = time.perf_counter()
+        job.cancel()
+        assert job.wait(60000)
+        latency = time.perf_counter() - t1
+        p = job.progress()
+        print(f"[cancel] batch {batch * 1e3:.1f} ms, cancel to terminal {latency * 1e3:.1f} ms, "
+              f"recorded {r1} -> {p.recorded_samples}", flush=True)
+        assert p.state == "cancelled"
+        assert latency < 0.5 * batch
+        sums, counts = job.film()
+        assert int(counts.sum()) == p.recorded_samples
+        job.release()
+        bvh.release()
