@@ -17,7 +17,9 @@ The knobs are the launch and queue choices of the wavefront integrator:
 
 A calibration workload (a list of scenes and job requests, all submitted at once) is rendered with
 each candidate; the score is samples per second over the whole workload, best of ``repeats`` runs
-after one warm-up. Coordinate descent: starting from the library defaults, each knob in turn is
+after one warm-up. A candidate the device refuses (a pool that leaves no room for any job in the
+HBM budget, such as two 2^28-path pools in 64 GiB) scores zero and is recorded as infeasible.
+Coordinate descent: starting from the library defaults, each knob in turn is
 swept with the others fixed and the best value kept; ``rounds`` passes over all knobs. Results are
 plain JSON (``save`` / ``load``), so a C++ caller can read the same file into ``mrg_device_options``.
 """
@@ -26,6 +28,7 @@ from __future__ import annotations
 import json
 import time
 
+from . import _native as N
 from . import api
 
 KNOBS = {
@@ -87,11 +90,18 @@ def tune(scenes, requests, schedule="batched", knobs=None, rounds=1, repeats=2, 
     def score(opts):
         key = json.dumps(opts, sort_keys=True)
         if key not in cache:
-            value, secs = measure(opts)
+            try:
+                value, secs = measure(opts)
+                trial = {"options": dict(opts), "samples_per_s": value, "seconds": secs}
+                note = f"{value / 1e6:.1f} Msamples/s"
+            except N.NotAdmitted as e:
+                value = 0.0
+                trial = {"options": dict(opts), "samples_per_s": 0.0, "infeasible": str(e)}
+                note = f"infeasible ({e})"
             cache[key] = value
-            trials.append({"options": dict(opts), "samples_per_s": value, "seconds": secs})
+            trials.append(trial)
             if log:
-                log(f"[autotune] {schedule} {opts} -> {value / 1e6:.1f} Msamples/s")
+                log(f"[autotune] {schedule} {opts} -> {note}")
         return cache[key]
 
     best_value = score(best)

@@ -67,3 +67,22 @@ def test_save_and_load_round_trip(tmp_path):
     path = tmp_path / "tuned.json"
     autotune.save(r, path)
     assert autotune.load(path) == r["options"]
+
+
+def test_refused_candidates_score_zero_and_are_recorded():
+    """A candidate whose pools leave no room for a job in the HBM budget makes the device refuse every
+    job (NotAdmitted); the search records it as infeasible and keeps going."""
+    from minirender_amd import _native as N
+
+    def scorer(o):
+        if o["path_capacity"] == 1 << 28 and o.get("batches_in_flight") == 2:
+            raise N.NotAdmitted(3, "job exceeds the device HBM budget")
+        v = math.log2(o["path_capacity"]) + (0.5 if o.get("batches_in_flight") == 2 else 0.0)
+        return v, [0.01]
+
+    knobs = {"path_capacity": [1 << 26, 1 << 28], "batches_in_flight": [1, 2]}
+    r = autotune.tune(None, None, schedule="plain", knobs=knobs, scorer=scorer)
+    assert r["options"] == {"path_capacity": 1 << 28, "batches_in_flight": 1}
+    refused = [t for t in r["trials"] if "infeasible" in t]
+    assert len(refused) == 1 and refused[0]["samples_per_s"] == 0.0
+    assert refused[0]["options"] == {"path_capacity": 1 << 28, "batches_in_flight": 2}
