@@ -624,7 +624,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k
 // through a wave-uniform index, so the job's scene fields load through the scalar cache.
 template <bool EXT>
 __device__ __forceinline__ void shade_path(const Pool& P, const GpuJob& job, uint32_t path, float4 pv, uint32_t iter, bool& cont,
-                                           bool& emit, uint32_t& ext_key, uint32_t& sh_key, float4& so, float4& sd, float4& sc4,
+                                           bool& emit, uint32_t& ext_key, uint32_t& sh_key, float4* s_shadow,
                                            uint32_t& n_scatter, uint32_t& n_rr, uint32_t& n_shadow) {
         // the cancel word, read first and used last so that its latency hides behind the shading; the host
         // raises it while kernels run, so it is read past the per-XCD L2s
@@ -775,10 +775,12 @@ __device__ __forceinline__ void shade_path(const Pool& P, const GpuJob& job, uin
                             const F3 c = (((beta * bsdf_eval_n(bp, won, win)) * wi.z) * ls.radiance) * (weight / ls.probability);
                             emit   = true;
                             sh_key = octant(sdir.x, sdir.y, sdir.z);
-                            so   = make_float4(origin.x, origin.y, origin.z, limit);
+                            // the shadow record waits in LDS (this thread's row) until the block has
+                            // reserved its queue slots, so it holds no registers through BSDF sampling
+                            s_shadow[threadIdx.x] = make_float4(origin.x, origin.y, origin.z, limit);
                             // w: the emitter triangle the segment ends on, which its any-hit skips
-                            sd   = make_float4(sdir.x, sdir.y, sdir.z, __uint_as_float(ls.prim));
-                            sc4  = make_float4(c.x, c.y, c.z, 0.0f);
+                            s_shadow[kShadeThreads + threadIdx.x]     = make_float4(sdir.x, sdir.y, sdir.z, __uint_as_float(ls.prim));
+                            s_shadow[2 * kShadeThreads + threadIdx.x] = make_float4(c.x, c.y, c.z, 0.0f);
                         }
                     }
                 }
@@ -841,6 +843,7 @@ __global__ void __launch_bounds__(kShadeThreads) __attribute__((amdgpu_waves_per
     uint32_t        n_scatter = 0, n_rr = 0, n_shadow = 0;
     __shared__ uint32_t s_off[16 * (kShadeThreads / 64)];
     __shared__ uint32_t s_base[2];
+    __shared__ float4   s_shadow[3 * kShadeThreads];  // origin + limit, direction + emitter, contribution
     // block-uniform loop: the grouped compaction below synchronises the block
     for (uint32_t bb = blockIdx.x * blockDim.x; bb < n; bb += stride) {
         const uint32_t i        = bb + threadIdx.x;
@@ -848,25 +851,24 @@ __global__ void __launch_bounds__(kShadeThreads) __attribute__((amdgpu_waves_per
         bool           emit     = false;
         uint32_t       ext_key  = 0, sh_key = 0;
         uint32_t       path     = 0;
-        float4         so = make_float4(0, 0, 0, 0), sd = so, sc4 = so;
         if (i < n) {
             path = queue ? queue[i] : i;
             const float4   pv  = P.prev[path];
             const uint32_t jid = __float_as_uint(pv.w) >> 16;
             const uint32_t j0  = __builtin_amdgcn_readfirstlane(jid);
             if (__ballot(jid != j0) == 0)
-                shade_path<EXT>(P, jobs[j0], path, pv, iter, cont, emit, ext_key, sh_key, so, sd, sc4, n_scatter, n_rr, n_shadow);
+                shade_path<EXT>(P, jobs[j0], path, pv, iter, cont, emit, ext_key, sh_key, s_shadow, n_scatter, n_rr, n_shadow);
             else
-                shade_path<EXT>(P, jobs[jid], path, pv, iter, cont, emit, ext_key, sh_key, so, sd, sc4, n_scatter, n_rr, n_shadow);
+                shade_path<EXT>(P, jobs[jid], path, pv, iter, cont, emit, ext_key, sh_key, s_shadow, n_scatter, n_rr, n_shadow);
         }
         const GroupedSlots g = block_reserve_grouped<kShadeThreads / 64>(next_n, cont, ext_key, sh_n, emit, sh_key, s_off, s_base);
         const uint32_t cpos = g.ext, spos = g.shadow;
         if (cont) next_q[cpos] = path;
         if (emit) {
             P.q_sh[spos] = path;
-            P.sh_o[spos] = so;
-            P.sh_d[spos] = sd;
-            P.sh_c[spos] = sc4;
+            P.sh_o[spos] = s_shadow[threadIdx.x];
+            P.sh_d[spos] = s_shadow[kShadeThreads + threadIdx.x];
+            P.sh_c[spos] = s_shadow[2 * kShadeThreads + threadIdx.x];
         }
     }
     wave_add64(P.stats + kStatShadowRays, n_shadow);
