@@ -4,7 +4,7 @@ The knobs are the launch and queue choices of the wavefront integrator:
 
 * ``path_capacity``             paths in flight per batch (the extension / shadow queue capacity),
 * ``persistent_blocks_per_cu``  resident 256-thread blocks per CU of the persistent traversal grids
-                                (0: the library's choice, 4 for fused / split and 6 for refill),
+                                (0: the library's choice, 4 for fused / split and 7 for refill),
 * ``traversal``                 ``split`` (closest-hit and any-hit kernels per bounce), ``fused``
                                 (one traversal launch per bounce for both kinds of ray), ``refill``
                                 (the fused launch where lanes that finish take the next ray at once)

@@ -308,7 +308,7 @@ void alloc_track(mrg_device* d, Track& t) {
     pl.q_ext1   = reinterpret_cast<uint32_t*>(take(P * 4));
     pl.q_sh     = reinterpret_cast<uint32_t*>(take(P * 4));
     // spill columns for the larger of the two persistent grids (a lane's column is its global id)
-    const uint64_t lanes = uint64_t(std::max({d->shape.traversal_blocks, d->shape.refill_blocks, 1u})) * 256;
+    const uint64_t lanes = uint64_t(std::max({d->shape.traversal_blocks, d->shape.refill_blocks, d->shape.query_blocks, 1u})) * 256;
     t.spill_ref.alloc(lanes * (kStackTotal - kStackLds) * 4);
     t.spill_t.alloc(lanes * (kStackTotal - kStackLds) * 4);
     t.counters.alloc(kCounterWords * 4);
@@ -362,7 +362,7 @@ void alloc_pool(mrg_device* d) {
     alloc_track(d, d->tracks[0]);
     d->track_count = d->opt.batches_in_flight == 2 ? 2u : 1u;
     if (d->track_count == 2) alloc_track(d, d->tracks[1]);
-    const uint64_t lanes = uint64_t(std::max({d->shape.traversal_blocks, d->shape.refill_blocks, 1u})) * 256;
+    const uint64_t lanes = uint64_t(std::max({d->shape.traversal_blocks, d->shape.refill_blocks, d->shape.query_blocks, 1u})) * 256;
     d->query_work.alloc(64);
     d->query_spill_ref.alloc(lanes * (kStackTotal - kStackLds) * 4);
     d->query_spill_t.alloc(lanes * (kStackTotal - kStackLds) * 4);
@@ -763,7 +763,7 @@ void admit_locked(mrg_device* d) {
     }
 }
 
-// Traversal kernel of a batch under MRG_TRAVERSAL_AUTO: the refill driver (parked leaves, six waves
+// Traversal kernel of a batch under MRG_TRAVERSAL_AUTO: the refill driver (parked leaves, seven waves
 // per SIMD) when the batch holds a scene of at least 2^14 triangles and at least 2^23 paths, the
 // fused launch otherwise. Measured refill/fused throughput: by scene (768 x 768 x 64 spp batches,
 // and the 1080p terrain job) 0.85-0.90 at 36-5k triangles, 1.03-1.10 at 21k-82k, 1.09-1.21 at
@@ -1310,11 +1310,12 @@ int mrg_device_open(int ordinal, const mrg_device_options* options, mrg_device**
         const uint32_t cus = uint32_t(std::max(1, d->props.multiProcessorCount));
         // Resident 256-thread blocks per CU of the persistent grids, unless the options fix one count
         // for both: 4 for the fused / split traversal kernels (at most 5 waves per SIMD by their
-        // registers; 4 measured best), 6 for the refill-driver kernels (80 VGPRs and 26 KB of LDS:
-        // six blocks fill the CU; 6 vs 4 measured +8% on the 1080p terrain job).
+        // registers; 4 measured best), 7 for the refill-driver kernels (72 VGPRs and 22 KB of LDS:
+        // seven blocks fill the CU; 6 vs 4 measured +8%, 7 vs 6 +3% on the 1080p terrain job).
         const uint32_t per    = d->opt.persistent_blocks_per_cu ? d->opt.persistent_blocks_per_cu : 4u;
-        const uint32_t per_rf = d->opt.persistent_blocks_per_cu ? d->opt.persistent_blocks_per_cu : 6u;
-        d->shape              = LaunchShape{cus, cus * per, cus * 8, cus * per_rf};
+        const uint32_t per_rf = d->opt.persistent_blocks_per_cu ? d->opt.persistent_blocks_per_cu : 7u;
+        const uint32_t per_q  = d->opt.persistent_blocks_per_cu ? d->opt.persistent_blocks_per_cu : 6u;  // k_query: six
+        d->shape              = LaunchShape{cus, cus * per, cus * 8, cus * per_rf, cus * per_q};
         // Automatic pool: 2^28 paths (47 GB of path state, within the default 64 GiB budget) on a
         // device of at least 192 GiB, 2^26 (12 GB) from 64 GiB, else 2^23. Fewer, fuller batches pay
         // each batch's latency-bound tail of late bounces (a few ms, whatever the batch size) less

@@ -55,7 +55,7 @@ constexpr uint32_t kStatWords      = 8;
 struct Pool {
     float4*   ray_o;     // origin, tmin
     float4*   ray_d;     // direction, tmax
-    float4*   hit;       // t, barycentrics
+    float4*   hit;       // closest hit: edge functions e0, e1, e2 and their sum (barycentric i = e_i / sum)
     uint32_t* hit_slot;  // leaf-order slot or 0xffffffff
     float4*   beta;      // throughput, previous pdf
     float4*   rad;       // radiance, eta scale
@@ -81,7 +81,8 @@ struct LaunchShape {
     uint32_t compute_units;
     uint32_t traversal_blocks;  // persistent grid of k_trace, k_extend and k_shadow (256-thread blocks)
     uint32_t shade_blocks;
-    uint32_t refill_blocks;     // persistent grid of the refill-driver kernels (k_trace_refill, k_query)
+    uint32_t refill_blocks;     // persistent grid of the refill-driver bounce kernel (k_trace_refill)
+    uint32_t query_blocks;      // persistent grid of the ray-query kernel (k_query)
 };
 
 hipError_t launch_query(const GpuScene& sc, const float* rays, uint32_t n, float tmin, float tmax, mrg_hit_raw* hits, float* bary,

@@ -173,8 +173,9 @@ __device__ __forceinline__ GroupedSlots block_reserve_grouped(uint32_t* ext_coun
     return g;
 }
 
-__device__ __forceinline__ StackView make_stack(uint32_t* lds_ref, float* lds_t, const Pool& P) {
+__device__ __forceinline__ StackView make_stack(uint32_t* lds_ref, float* lds_t, const Pool& P, uint32_t depth = kStackLds) {
     StackView s;
+    s.depth     = depth;
     s.lds_ref   = lds_ref;
     s.lds_t     = lds_t;
     s.spill_ref = P.spill_ref;
@@ -206,9 +207,9 @@ constexpr int      kQueryLeafBatch    = 24;
 constexpr int      kBounceLeafBatch   = 24;
 constexpr uint32_t kRefillClaim = 128;  // entries per atomic claim (>= 64; 128 measured best, +61% over an atomic per refill)
 
-template <int REFILL_LANES, int LEAF_BATCH, bool STATS, bool UNIFORM, class Start, class Finish>
+template <int REFILL_LANES, int LEAF_BATCH, bool STATS, bool UNIFORM, class Start, class Finish, class Keep>
 __device__ __forceinline__ void persistent_trace_mixed(const StackView& stk, uint32_t n, uint32_t* work, const TraceScene& uscene,
-                                                       Start start, Finish finish, uint64_t& nb, uint64_t& nt) {
+                                                       Start start, Finish finish, const Keep& keep, uint64_t& nb, uint64_t& nt) {
     TraceState st;
     TraceScene ts{nullptr, nullptr, 0};
     bool       active  = false;
@@ -253,7 +254,7 @@ __device__ __forceinline__ void persistent_trace_mixed(const StackView& stk, uin
             }
         }
         if (__ballot(active) == 0) break;
-        if (active && trace_step_dyn<LEAF_BATCH, STATS>(UNIFORM ? uscene : ts, st, stk, nb, nt, any)) {
+        if (active && trace_step_dyn<LEAF_BATCH, STATS>(UNIFORM ? uscene : ts, st, stk, nb, nt, any, keep)) {
             active  = false;
             pending = true;
         }
@@ -272,7 +273,7 @@ __device__ __forceinline__ void persistent_trace(const StackView& stk, uint32_t 
             start(i, st);
             return scene;
         },
-        finish, nb, nt);
+        finish, KeepInResult{}, nb, nt);
 }
 
 // Reported distance of the winning hit: the ray/plane solve of that one triangle in FP64
@@ -293,15 +294,16 @@ __device__ __forceinline__ float refine_distance(const GpuScene& sc, uint32_t sl
 
 // ---------------------------------------------------------------------------------------------
 // Intersection microbenchmark / SceneBvh::intersect and ::occluded over a packed ray array.
-// Six waves per SIMD, as k_trace_refill.
+// Six waves per SIMD (at most 80 VGPRs, 26 KB of LDS stack per block: the incoherent rays of queries
+// stack deeper than path rays, and measured 2.5% slower at seven waves with 22 KB stacks).
 template <bool ANY, bool STATS>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k_query(GpuScene sc, const float* __restrict__ rays, uint32_t n, float tmin, float tmax,
                                                mrg_hit_raw* __restrict__ hits, float* __restrict__ bary,
                                                uint8_t* __restrict__ occluded, uint32_t* __restrict__ work, Pool P,
                                                uint64_t* __restrict__ stats) {
-    __shared__ uint32_t s_ref[kStackLds * kTraversalBlock];
-    __shared__ float    s_t[kStackLds * kTraversalBlock];
-    const StackView stk = make_stack(s_ref, s_t, P);
+    __shared__ uint32_t s_ref[kStackLdsQuery * kTraversalBlock];
+    __shared__ float    s_t[kStackLdsQuery * kTraversalBlock];
+    const StackView stk = make_stack(s_ref, s_t, P, kStackLdsQuery);
     uint64_t        nb = 0, nt = 0;
     const TraceScene ts = trace_scene(sc);
     persistent_trace<ANY, STATS>(
@@ -495,7 +497,7 @@ __device__ __forceinline__ void extend_one(const Pool& P, const GpuJob* __restri
     const uint32_t job   = ONE_SCENE ? 0u : __float_as_uint(P.prev[path].w) >> 16;
     const RayPre   rp    = ray_setup(mk(ro.x, ro.y, ro.z), mk(rd.x, rd.y, rd.z));
     const TraceResult res = trace_ray<false, false>(jobs[job].scene, rp, ro.w, rd.w, stk, &dummy, &dummy);
-    P.hit[path]      = make_float4(res.t, res.b0, res.b1, res.b2);
+    P.hit[path]      = make_float4(res.b0, res.b1, res.b2, 1.0f);  // already divided: their sum stands in as 1
     P.hit_slot[path] = res.slot;
 }
 
@@ -566,9 +568,11 @@ __global__ void __launch_bounds__(256) k_trace(Pool P, const GpuJob* __restrict_
 // bounce iter - 1 first, then the extension rays of bounce iter), but lanes that finish their ray
 // take the next queue entry at once, so a wave can trace shadow and extension rays side by side.
 // ONE_SCENE: every job of the batch renders the same scene (jobs[0]'s), read once per wave.
-// Six waves per SIMD (at most 80 VGPRs, 26 KB of LDS stack per block): six resident blocks per CU.
+// Seven waves per SIMD (at most 72 VGPRs, 22 KB of LDS stack per block): seven resident blocks per CU.
+// The kernel is bound by latency, not by VALU issue (the SIMDs issue two VALU instructions in a quarter
+// of their issue slots), so a seventh wave pays: 137.3 -> 131.3 ms of traversal per 1080p job.
 template <bool ONE_SCENE>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k_trace_refill(Pool P, const GpuJob* __restrict__ jobs, uint32_t iter) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) k_trace_refill(Pool P, const GpuJob* __restrict__ jobs, uint32_t iter) {
     __shared__ uint32_t s_ref[kStackLds * kTraversalBlock];
     __shared__ float    s_t[kStackLds * kTraversalBlock];
     const StackView stk   = make_stack(s_ref, s_t, P);
@@ -610,11 +614,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k
                     P.rad[path] = l;
                 }
             } else {
-                finish_barycentrics(res);
-                P.hit[path]      = make_float4(res.t, res.b0, res.b1, res.b2);
-                P.hit_slot[path] = res.slot;
+                P.hit_slot[path] = res.slot;  // the hit record holds the winner's edge functions already
             }
         },
+        [&](TraceResult&, const TriHit& h) { P.hit[path] = make_float4(h.a, h.b, h.c, h.det); },
         nb, nt);
 }
 
@@ -675,7 +678,9 @@ __device__ __forceinline__ void shade_path(const Pool& P, const GpuJob& job, uin
             // (computed at upload with these same float operations)
             F3 geometric = mk(na.w, nb.w, nc.w);
             if (tflags & kTriMirrored) geometric = -geometric;
-            const float b0 = hv.y, b1 = hv.z, b2 = hv.w;
+            // the hit record holds the edge functions and their sum: barycentric i = e_i / sum, the
+        // division the watertight test does when it accepts the hit
+        const float b0 = hv.x / hv.w, b1 = hv.y / hv.w, b2 = hv.z / hv.w;
             F3 shading  = mk(0.0f, 0.0f, 0.0f);
             F3 position = mk(0.0f, 0.0f, 0.0f);
             F3 absolute = mk(0.0f, 0.0f, 0.0f);
@@ -981,7 +986,7 @@ hipError_t noalloc_scratch_address(void** out) { return hipGetSymbolAddress(out,
 hipError_t launch_query(const GpuScene& sc, const float* rays, uint32_t n, float tmin, float tmax, mrg_hit_raw* hits, float* bary,
                         uint8_t* occluded, uint32_t* work, const Pool& P, uint64_t* stats, bool any, bool with_stats,
                         const LaunchShape& shape, hipStream_t stream) {
-    const dim3 grid(shape.refill_blocks), block(256);
+    const dim3 grid(shape.query_blocks), block(256);
     if (any) {
         if (with_stats) hipLaunchKernelGGL((k_query<true, true>), grid, block, 0, stream, sc, rays, n, tmin, tmax, hits, bary, occluded, work, P, stats);
         else hipLaunchKernelGGL((k_query<true, false>), grid, block, 0, stream, sc, rays, n, tmin, tmax, hits, bary, occluded, work, P, stats);

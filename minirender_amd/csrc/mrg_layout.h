@@ -31,7 +31,8 @@ constexpr uint32_t kLeafBit    = 0x80000000u;
 constexpr uint32_t kEmptyRef   = 0xffffffffu;
 constexpr uint32_t kMaxLeaf    = 16;   // triangles per leaf (count-1 packed in 4 bits)
 constexpr uint32_t kMaxDepth   = 30;   // binary-tree depth bound of the builder
-constexpr uint32_t kStackLds   = 13;   // stack entries per lane kept in LDS: 26 KB per 256-lane block, six blocks per CU
+constexpr uint32_t kStackLds   = 11;   // stack entries per lane kept in LDS by the path kernels: 22 KB per 256-lane block, seven blocks per CU
+constexpr uint32_t kStackLdsQuery = 13;  // the same for ray queries (incoherent rays stack deeper): 26 KB per block, six blocks per CU
 constexpr uint32_t kStackTotal = 64;   // LDS + global spill entries per lane (builder-checked bound)
 
 MRG_HD inline uint32_t leaf_ref(uint32_t first, uint32_t count) { return kLeafBit | ((count - 1u) << 27) | first; }

@@ -1,5 +1,5 @@
 // BVH traversal for one ray per lane: four-wide 128 B nodes from HBM, the current node in VGPRs, a
-// stack of (child reference, entry distance) pairs in LDS (kStackLds per lane) that spills to a
+// stack of (child reference, entry distance) pairs in LDS (a kernel's `depth` entries per lane) that spills to a
 // per-lane global region for deeper entries (the builder bounds the worst-case stack by kStackTotal).
 // Children of a node are visited nearest first (a five-exchange sorting network on entry distance).
 // Two drivers: trace_ray runs one ray to its end per call (per-ray loop); trace_step_dyn advances
@@ -20,9 +20,12 @@
 namespace mrg {
 
 struct RayPre {
-    F3    o, d, inv;
-    int   kx, ky, kz;
-    float sx, sy, sz;
+    F3       o, d, inv;
+    uint32_t axes;  // the watertight test's axis permutation kx | ky << 2 | kz << 4 (one register)
+    float    sx, sy, sz;
+    __device__ __forceinline__ int kx() const { return static_cast<int>(axes & 3u); }
+    __device__ __forceinline__ int ky() const { return static_cast<int>((axes >> 2) & 3u); }
+    __device__ __forceinline__ int kz() const { return static_cast<int>(axes >> 4); }
 };
 
 __device__ __forceinline__ RayPre ray_setup(F3 o, F3 d) {
@@ -42,9 +45,7 @@ __device__ __forceinline__ RayPre ray_setup(F3 o, F3 d) {
         kx          = ky;
         ky          = t;
     }
-    r.kx = kx;
-    r.ky = ky;
-    r.kz = kz;
+    r.axes = static_cast<uint32_t>(kx) | static_cast<uint32_t>(ky) << 2 | static_cast<uint32_t>(kz) << 4;
     const float dz = comp(d, kz);
     r.sx = comp(d, kx) / dz;
     r.sy = comp(d, ky) / dz;
@@ -82,7 +83,7 @@ struct TriHit {
 
 __device__ __forceinline__ void permute(const RayPre& r, F3 v, float& px, float& py, float& pz) {
     const F3    t  = v - r.o;
-    const float tx = comp(t, r.kx), ty = comp(t, r.ky), tz = comp(t, r.kz);
+    const float tx = comp(t, r.kx()), ty = comp(t, r.ky()), tz = comp(t, r.kz());
     px = tx - r.sx * tz;
     py = ty - r.sy * tz;
     pz = tz * r.sz;
@@ -131,22 +132,23 @@ constexpr uint32_t kTriMirrored    = 2u;
 constexpr uint32_t kTraversalBlock = 256;
 
 struct StackView {
-    uint32_t* lds_ref;   // [kStackLds][kTraversalBlock]
+    uint32_t* lds_ref;   // [depth][kTraversalBlock]
     float*    lds_t;
-    uint32_t* spill_ref; // [kStackTotal - kStackLds][lanes]
+    uint32_t* spill_ref; // [kStackTotal - depth][lanes]
     float*    spill_t;
     uint32_t  lane;      // threadIdx.x
     uint32_t  glane;     // global lane id (spill column)
     uint32_t  lanes;     // total lanes of the grid (spill row pitch)
+    uint32_t  depth;     // entries per lane in LDS (kernel-uniform)
 };
 
 __device__ __forceinline__ void stack_push(const StackView& s, uint32_t& sp, uint32_t ref, float t) {
     if (sp >= kStackTotal) return;  // unreachable: the builder bounds the worst-case stack by kStackTotal
-    if (sp < kStackLds) {
+    if (sp < s.depth) {
         s.lds_ref[sp * kTraversalBlock + s.lane] = ref;
         s.lds_t[sp * kTraversalBlock + s.lane]   = t;
     } else {
-        const uint32_t k = (sp - kStackLds) * s.lanes + s.glane;
+        const uint32_t k = (sp - s.depth) * s.lanes + s.glane;
         gst(s.spill_ref + k, ref);
         gst(s.spill_t + k, t);
     }
@@ -155,11 +157,11 @@ __device__ __forceinline__ void stack_push(const StackView& s, uint32_t& sp, uin
 
 __device__ __forceinline__ void stack_pop(const StackView& s, uint32_t& sp, uint32_t& ref, float& t) {
     --sp;
-    if (sp < kStackLds) {
+    if (sp < s.depth) {
         ref = s.lds_ref[sp * kTraversalBlock + s.lane];
         t   = s.lds_t[sp * kTraversalBlock + s.lane];
     } else {
-        const uint32_t k = (sp - kStackLds) * s.lanes + s.glane;
+        const uint32_t k = (sp - s.depth) * s.lanes + s.glane;
         ref              = gld(s.spill_ref + k);
         t                = gld(s.spill_t + k);
     }
@@ -236,15 +238,27 @@ __device__ __forceinline__ void trace_init(TraceState& st, F3 o, F3 d, float tmi
 // parked leaf has lowered the closest distance is tested against a larger bound, which costs work,
 // never a hit.
 
+// Where a closest-hit ray keeps the edge functions of its best hit so far: in its TraceResult
+// (KeepInResult), or wherever a caller's functor puts them (the render traversal writes them to the
+// path's hit record at once, so they hold no registers while the ray goes on).
+struct KeepInResult {
+    __device__ __forceinline__ void operator()(TraceResult& res, const TriHit& h) const {
+        res.b0  = h.a;
+        res.b1  = h.b;
+        res.b2  = h.c;
+        res.det = h.det;
+    }
+};
+
 // One traversal step (all lanes of the wave that trace call it together): the parked leaves' triangles
 // or one inner node per lane, then the lane parks a leaf it reached and takes stack entries until it
 // has an inner node to visit. Returns true once the ray is finished: closest hit (any = false) or any
-// hit (any = true) in [tmin, tmax], result in st.res with the barycentrics still undivided (the
-// caller applies finish_barycentrics where it needs them). `any` may differ between the lanes of a
+// hit (any = true) in [tmin, tmax], result in st.res; a closest hit's edge functions go to `keep`
+// undivided (with KeepInResult the caller applies finish_barycentrics where it needs them). `any` may differ between the lanes of a
 // wave.
-template <int LEAF_BATCH, bool STATS>
+template <int LEAF_BATCH, bool STATS, class Keep>
 __device__ __forceinline__ bool trace_step_dyn(const TraceScene& sc, TraceState& st, const StackView& stk, uint64_t& bounds_tests,
-                                               uint64_t& tri_tests, const bool any) {
+                                               uint64_t& tri_tests, const bool any, const Keep& keep) {
     if (sc.node_count == 0) return true;
     const RayPre& r          = st.r;
     const bool    parked     = st.leaf != kEmptyRef;
@@ -278,13 +292,10 @@ __device__ __forceinline__ bool trace_step_dyn(const TraceScene& sc, TraceState&
                 if (any && prim == st.res.prim) continue;
                 if (st.res.slot == 0xffffffffu || h.t < st.res.t || (h.t == st.res.t && prim < st.res.prim)) {
                     st.res.t    = h.t;
-                    st.res.b0   = h.a;
-                    st.res.b1   = h.b;
-                    st.res.b2   = h.c;
-                    st.res.det  = h.det;
                     st.res.slot = slot;
                     st.res.prim = prim;
                     if (any) return true;
+                    keep(st.res, h);
                 }
             }
         }
