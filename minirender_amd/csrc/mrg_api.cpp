@@ -602,7 +602,7 @@ mrg_scene* build_scene(mrg_device* d, const mrg_scene_desc& s, const std::string
         g.pad[0] = g.pad[1] = 0;
     }
     check_hip(hipSetDevice(d->ordinal), "hipSetDevice");
-    upload(sc->nodes, bvh.nodes);
+    upload(sc->nodes, bvh.qnodes);
     upload(sc->tris, tris);
     upload(sc->normals, nrms);
     upload(sc->slot_instance, slot_instance);
@@ -618,7 +618,7 @@ mrg_scene* build_scene(mrg_device* d, const mrg_scene_desc& s, const std::string
     upload(sc->env_cdf, env_tables.cdf);
     upload(sc->env_rows, env_tables.rows);
     GpuScene& g     = sc->gpu;
-    g.nodes         = sc->nodes.as<GpuNode4>();
+    g.nodes         = sc->nodes.as<GpuNodeQ>();
     g.tris          = sc->tris.as<GpuTri>();
     g.normals       = sc->normals.as<GpuTriNormals>();
     g.slot_instance = sc->slot_instance.as<uint32_t>();
@@ -1832,18 +1832,29 @@ int mrg_bvh_build_host(const float* positions, uint64_t n, uint64_t* out) {
     return guarded([&] {
         if (!out || (n && !positions)) throw std::invalid_argument("null argument");
         const BvhBuildResult b = build_bvh(positions, n, 0);
-        // structural check: every primitive exactly once, leaves <= kMaxLeaf, child boxes inside parents
+        // structural check: every primitive exactly once, leaves <= kMaxLeaf, and every quantized child
+        // box (as the device decodes it) contains the exact child box
         std::vector<uint32_t> seen(n, 0);
         uint64_t max_leaf = 0, leaf_prims = 0, bad = 0;
-        for (const GpuNode4& g : b.nodes) {
-            for (uint32_t r : g.ref) {
+        if (b.qnodes.size() != b.nodes.size()) ++bad;
+        for (size_t i = 0; i < b.nodes.size() && i < b.qnodes.size(); ++i) {
+            const GpuNode4& g = b.nodes[i];
+            const GpuNodeQ& q = b.qnodes[i];
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t r = g.ref[k];
+                if (q.ref[k] != r) ++bad;
                 if (r == kEmptyRef) continue;
+                float lo[3], hi[3];
+                decode_child(q, k, lo, hi);
+                if (!(lo[0] <= g.lox[k] && lo[1] <= g.loy[k] && lo[2] <= g.loz[k] && hi[0] >= g.hix[k] && hi[1] >= g.hiy[k] &&
+                      hi[2] >= g.hiz[k]))
+                    ++bad;
                 if (is_leaf(r)) {
                     const uint32_t f = leaf_first(r), c = leaf_count(r);
                     max_leaf = std::max<uint64_t>(max_leaf, c);
-                    for (uint32_t k = 0; k < c; ++k) {
-                        if (f + k >= n) { ++bad; continue; }
-                        seen[b.order[f + k]]++;
+                    for (uint32_t m = 0; m < c; ++m) {
+                        if (f + m >= n) { ++bad; continue; }
+                        seen[b.order[f + m]]++;
                         ++leaf_prims;
                     }
                 } else if (r >= b.nodes.size()) {

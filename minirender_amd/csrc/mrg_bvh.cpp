@@ -256,7 +256,6 @@ struct Collapser {
             g.hiy[k] = b.hi[1];
             g.hiz[k] = b.hi[2];
             g.ref[k] = refs[k];
-            g.pad[k] = 0;
         }
         return index;
     }
@@ -306,7 +305,99 @@ BvhBuildResult build_bvh(const float* positions, uint64_t count, unsigned thread
     }
     out.leaves    = b.leaves.load();
     out.max_depth = b.max_depth.load();
+    out.qnodes.resize(out.nodes.size());
+    for (size_t i = 0; i < out.nodes.size(); ++i) out.qnodes[i] = quantize_node(out.nodes[i]);
     return out;
+}
+
+namespace {
+
+// One axis of a quantized node: the grid origin (a multiple of the power-of-two scale, with
+// |origin / scale| + 255 <= 2^24 so that origin + q * scale is an exact float for every byte q) and
+// the rounded-out bytes of each used child.
+struct AxisGrid {
+    float   origin = 0, scale = 1;
+    uint8_t qlo[4] = {}, qhi[4] = {};
+};
+
+float grid_plane(float origin, float scale, uint32_t q) { return origin + float(q) * scale; }
+
+AxisGrid quantize_axis(const float* lo, const float* hi, int used) {
+    float L = lo[0], H = hi[0];
+    for (int k = 1; k < used; ++k) {
+        L = std::min(L, lo[k]);
+        H = std::max(H, hi[k]);
+    }
+    // Smallest scale whose 255 steps span [L, H] and whose grid stays within float's exact range.
+    const double span = double(H) - double(L);
+    const double mag  = std::max(std::fabs(double(L)), std::fabs(double(H)));
+    int          e    = -120;
+    if (span > 0) e = std::max(e, int(std::ceil(std::log2(span / 255.0))) - 1);
+    if (mag > 0) e = std::max(e, int(std::ceil(std::log2(mag / double((1u << 24) - 512)))));
+    AxisGrid g;
+    for (;; ++e) {
+        if (e > 100) throw std::invalid_argument("node bounds exceed the quantization range");
+        const double s = std::ldexp(1.0, e);
+        const double k = std::floor(double(L) / s);
+        if (std::fabs(k) + 255.0 > double(1u << 24)) continue;
+        if ((k + 255.0) * s < double(H)) continue;
+        g.origin = float(k * s);  // exact: |k| < 2^24 and s a power of two in float's normal range
+        g.scale  = float(s);
+        break;
+    }
+    for (int k = 0; k < used; ++k) {
+        double ql = std::floor((double(lo[k]) - double(g.origin)) / double(g.scale));
+        double qh = std::ceil((double(hi[k]) - double(g.origin)) / double(g.scale));
+        ql = std::min(255.0, std::max(0.0, ql));
+        qh = std::min(255.0, std::max(0.0, qh));
+        // settle any rounding of the double quotient with exact float comparisons
+        uint32_t a = uint32_t(ql), b = uint32_t(qh);
+        while (a > 0 && grid_plane(g.origin, g.scale, a) > lo[k]) --a;
+        while (b < 255 && grid_plane(g.origin, g.scale, b) < hi[k]) ++b;
+        if (grid_plane(g.origin, g.scale, a) > lo[k] || grid_plane(g.origin, g.scale, b) < hi[k])
+            throw std::logic_error("quantized child box does not contain its exact box");
+        g.qlo[k] = uint8_t(a);
+        g.qhi[k] = uint8_t(b);
+    }
+    return g;
+}
+
+uint32_t pack_bytes(const uint8_t* q) { return uint32_t(q[0]) | uint32_t(q[1]) << 8 | uint32_t(q[2]) << 16 | uint32_t(q[3]) << 24; }
+
+}  // namespace
+
+GpuNodeQ quantize_node(const GpuNode4& n) {
+    int used = 0;
+    while (used < 4 && n.ref[used] != kEmptyRef) ++used;
+    GpuNodeQ q{};
+    for (int k = 0; k < 4; ++k) q.ref[k] = n.ref[k];
+    if (used == 0) return q;
+    const AxisGrid gx = quantize_axis(n.lox, n.hix, used);
+    const AxisGrid gy = quantize_axis(n.loy, n.hiy, used);
+    const AxisGrid gz = quantize_axis(n.loz, n.hiz, used);
+    q.origin[0] = gx.origin;
+    q.origin[1] = gy.origin;
+    q.origin[2] = gz.origin;
+    q.scale_x   = gx.scale;
+    q.scale_y   = gy.scale;
+    q.scale_z   = gz.scale;
+    q.qlo_x     = pack_bytes(gx.qlo);
+    q.qlo_y     = pack_bytes(gy.qlo);
+    q.qlo_z     = pack_bytes(gz.qlo);
+    q.qhi_x     = pack_bytes(gx.qhi);
+    q.qhi_y     = pack_bytes(gy.qhi);
+    q.qhi_z     = pack_bytes(gz.qhi);
+    return q;
+}
+
+void decode_child(const GpuNodeQ& q, int k, float lo[3], float hi[3]) {
+    const uint32_t sh = 8u * uint32_t(k);
+    lo[0] = grid_plane(q.origin[0], q.scale_x, (q.qlo_x >> sh) & 255u);
+    lo[1] = grid_plane(q.origin[1], q.scale_y, (q.qlo_y >> sh) & 255u);
+    lo[2] = grid_plane(q.origin[2], q.scale_z, (q.qlo_z >> sh) & 255u);
+    hi[0] = grid_plane(q.origin[0], q.scale_x, (q.qhi_x >> sh) & 255u);
+    hi[1] = grid_plane(q.origin[1], q.scale_y, (q.qhi_y >> sh) & 255u);
+    hi[2] = grid_plane(q.origin[2], q.scale_z, (q.qhi_z >> sh) & 255u);
 }
 
 }  // namespace mrg

@@ -9,7 +9,8 @@
 namespace mrg {
 
 struct BvhBuildResult {
-    std::vector<GpuNode4> nodes;
+    std::vector<GpuNode4> nodes;      // exact child boxes (host)
+    std::vector<GpuNodeQ> qnodes;     // the same nodes with quantized child boxes (uploaded)
     std::vector<uint32_t> order;      // leaf order -> scene-order primitive
     uint64_t              leaves = 0;
     uint64_t              max_depth = 0;     // binary-tree depth
@@ -21,5 +22,11 @@ struct BvhBuildResult {
 // triangles, depth <= kMaxDepth: median splits take over where SAH would go deeper), collapsed into
 // four-wide nodes whose worst-case traversal stack stays within kStackTotal.
 BvhBuildResult build_bvh(const float* positions, uint64_t count, unsigned threads);
+
+// Quantizes a node's child boxes (GpuNodeQ): every decoded box contains the exact one.
+GpuNodeQ quantize_node(const GpuNode4& n);
+
+// Child k's decoded box (origin + q * scale per plane, exact float arithmetic), as the device sees it.
+void decode_child(const GpuNodeQ& q, int k, float lo[3], float hi[3]);
 
 }  // namespace mrg

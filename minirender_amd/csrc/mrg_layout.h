@@ -1,9 +1,10 @@
 // Device data layout shared by the host builder/scheduler (C++) and the HIP kernels.
 //
 // HBM layout of one resident scene:
-//   nodes      [node_count]    GpuNode4, 128 B each (one cache line): the four child boxes of an inner
-//                              node plus the four child references, so one node fetch gives the whole
-//                              four-box test. Depth-first order, root = 0.
+//   nodes      [node_count]    GpuNodeQ, 64 B each (half a cache line): the four child boxes of an
+//                              inner node, quantized to 8 bits per plane against an exact per-node
+//                              grid, plus the four child references, so one node fetch gives the
+//                              whole four-box test. Depth-first order, root = 0.
 //   tris       [n]             GpuTri, 48 B: three float4 vertices in leaf order; v0.w carries the
 //                              scene-order primitive index (the tie-break key of bvh.cpp:273-276).
 //   normals    [n]             GpuTriNormals, 48 B: the three corner normals normalized, with the
@@ -40,22 +41,29 @@ MRG_HD inline bool     is_leaf(uint32_t ref) { return (ref & kLeafBit) != 0u; }
 MRG_HD inline uint32_t leaf_first(uint32_t ref) { return ref & 0x07ffffffu; }
 MRG_HD inline uint32_t leaf_count(uint32_t ref) { return ((ref >> 27) & 15u) + 1u; }
 
-struct alignas(16) GpuNode {
-    float    lo0[3], hi0x;     // child 0 box (lo, hi.x)
-    float    hi0yz[2], lo1xy[2];
-    float    lo1z, hi1[3];
-    uint32_t ref0, ref1, pad0, pad1;
-};
-static_assert(sizeof(GpuNode) == 64, "GpuNode must be 64 bytes");
-
-// Four-wide node, one 128 B cache line: child boxes in SoA order (lo.x[4], lo.y[4], lo.z[4],
-// hi.x[4], hi.y[4], hi.z[4]) then the four child references (kEmptyRef for unused slots).
-struct alignas(128) GpuNode4 {
+// Four-wide node as the builder produces it (host only): exact child boxes in SoA order (lo.x[4],
+// lo.y[4], lo.z[4], hi.x[4], hi.y[4], hi.z[4]) then the four child references (kEmptyRef for unused
+// slots). Uploaded as GpuNodeQ.
+struct GpuNode4 {
     float    lox[4], loy[4], loz[4], hix[4], hiy[4], hiz[4];
     uint32_t ref[4];
-    uint32_t pad[4];
 };
-static_assert(sizeof(GpuNode4) == 128, "GpuNode4 must be 128 bytes");
+
+// Four-wide node with quantized child boxes, 64 B. Per axis a, every child plane is
+// origin[a] + q * scale[a] with an 8-bit q (byte k of a q word belongs to child k): scale[a] is a
+// power of two and origin[a] a multiple of it with |origin[a] / scale[a]| + 255 <= 2^24, so each
+// such plane is an exactly representable float and the device decodes it without rounding. The
+// builder rounds q_lo down and q_hi up, so every decoded box contains its child's exact box and the
+// conservative box test never culls a box the exact one would keep.
+struct alignas(64) GpuNodeQ {
+    float    origin[3];
+    float    scale_x;
+    float    scale_y, scale_z;
+    uint32_t qlo_x, qlo_y;
+    uint32_t qlo_z, qhi_x, qhi_y, qhi_z;
+    uint32_t ref[4];
+};
+static_assert(sizeof(GpuNodeQ) == 64, "GpuNodeQ must be 64 bytes");
 
 struct alignas(16) GpuTri {
     float v0[3];
@@ -123,7 +131,7 @@ struct GpuDirectional {
 
 // Per-scene pointers and constants handed to the kernels by value.
 struct GpuScene {
-    const GpuNode4*       nodes;
+    const GpuNodeQ*       nodes;
     const GpuTri*         tris;
     const GpuTriNormals*  normals;
     const uint32_t*       slot_instance;

@@ -1,4 +1,4 @@
-// BVH traversal for one ray per lane: four-wide 128 B nodes from HBM, the current node in VGPRs, a
+// BVH traversal for one ray per lane: four-wide 64 B nodes from HBM, the current node in VGPRs, a
 // stack of (child reference, entry distance) pairs in LDS (a kernel's `depth` entries per lane) that spills to a
 // per-lane global region for deeper entries (the builder bounds the worst-case stack by kStackTotal).
 // Children of a node are visited nearest first (a five-exchange sorting network on entry distance).
@@ -6,10 +6,14 @@
 // a ray by one step (one node, or a wave-wide triangle step over parked leaves) for drivers that
 // refill lanes as their rays finish.
 //
-// Box test: conservative slab test (each slab bound computed as (bound - origin) * (1/d), then the
-// entry/exit distances widened by 2^-20 relative), so a box is never culled when the watertight
-// triangle test would accept a triangle inside it; closest-hit results therefore do not depend on
-// the tree (nearest t, ties to the lower scene-order index; bvh.cpp:273-276).
+// Box test: conservative slab test on the quantized child boxes of a node (GpuNodeQ: plane =
+// origin + q * scale, exact floats). Per node and axis, A = (origin - o) * (1/d) (two roundings,
+// at most 2^-23 |A| off) and B = scale * (1/d) (exact: scale is a power of two); a child's slab
+// distance is q * B + A in one FMA, with A moved by 2^-21 |A| toward the ray for near planes and away
+// from it for far planes, and the entry/exit distances then widened by 2^-20 relative (the FMA's and
+// the reciprocal's rounding). So a box is never culled when the watertight triangle test would accept
+// a triangle inside it; closest-hit results therefore do not depend on the tree (nearest t, ties to
+// the lower scene-order index; bvh.cpp:273-276).
 // Triangle test: the FP32 form of RayQuery::intersect (intersection.cpp:108-152), operation for
 // operation.
 #pragma once
@@ -53,20 +57,57 @@ __device__ __forceinline__ RayPre ray_setup(F3 o, F3 d) {
     return r;
 }
 
-// Conservative slab test; returns the widened entry distance or +inf for a miss.
-__device__ __forceinline__ float box_enter(const RayPre& r, float lx, float ly, float lz, float hx, float hy, float hz,
-                                           float tmin, float tmax) {
-    const float ax = (lx - r.o.x) * r.inv.x, bx = (hx - r.o.x) * r.inv.x;
-    const float ay = (ly - r.o.y) * r.inv.y, by = (hy - r.o.y) * r.inv.y;
-    const float az = (lz - r.o.z) * r.inv.z, bz = (hz - r.o.z) * r.inv.z;
-    float tn = __builtin_fmaxf(__builtin_fmaxf(__builtin_fminf(ax, bx), __builtin_fminf(ay, by)),
-                               __builtin_fmaxf(__builtin_fminf(az, bz), tmin));
-    float tf = __builtin_fminf(__builtin_fminf(__builtin_fmaxf(ax, bx), __builtin_fmaxf(ay, by)),
-                               __builtin_fminf(__builtin_fmaxf(az, bz), tmax));
+// One child of a quantized node: near/far plane bytes per axis (byte k of each word), the per-node
+// slab terms; returns the widened entry distance or +inf for a miss.
+template <int K>
+__device__ __forceinline__ float child_enter(uint32_t nx, uint32_t ny, uint32_t nz, uint32_t fx, uint32_t fy, uint32_t fz, float bx,
+                                             float by, float bz, float anx, float any_, float anz, float afx, float afy, float afz,
+                                             float tmin, float tmax) {
+    constexpr uint32_t sh = 8u * K;
+    const float n_x = __builtin_fmaf(static_cast<float>((nx >> sh) & 255u), bx, anx);
+    const float n_y = __builtin_fmaf(static_cast<float>((ny >> sh) & 255u), by, any_);
+    const float n_z = __builtin_fmaf(static_cast<float>((nz >> sh) & 255u), bz, anz);
+    const float f_x = __builtin_fmaf(static_cast<float>((fx >> sh) & 255u), bx, afx);
+    const float f_y = __builtin_fmaf(static_cast<float>((fy >> sh) & 255u), by, afy);
+    const float f_z = __builtin_fmaf(static_cast<float>((fz >> sh) & 255u), bz, afz);
+    float tn = __builtin_fmaxf(__builtin_fmaxf(n_x, n_y), __builtin_fmaxf(n_z, tmin));
+    float tf = __builtin_fminf(__builtin_fminf(f_x, f_y), __builtin_fminf(f_z, tmax));
     // widen by 2^-20 of the magnitude: one FMA with an |x| source modifier per side
     tn = __builtin_fmaf(-__builtin_fabsf(tn), 0x1p-20f, tn);
     tf = __builtin_fmaf(__builtin_fabsf(tf), 0x1p-20f, tf);
     return tn <= tf ? tn : __builtin_inff();
+}
+
+// The four child boxes of node `np` (GpuNodeQ): entry distances (+inf for a miss or an unused slot)
+// and child references.
+__device__ __forceinline__ void node_boxes(const RayPre& r, const float4* np, float tmin, float tmax, float& t0, float& t1, float& t2,
+                                           float& t3, uint32_t& r0, uint32_t& r1, uint32_t& r2, uint32_t& r3) {
+    const float4 a  = gld(np + 0);                                     // origin.xyz, scale.x
+    const float4 b  = gld(np + 1);                                     // scale.y, scale.z, qlo.x, qlo.y
+    const uint4  c  = gld(reinterpret_cast<const uint4*>(np) + 2);     // qlo.z, qhi.x, qhi.y, qhi.z
+    const uint4  rf = gld(reinterpret_cast<const uint4*>(np) + 3);
+    const float  ax = (a.x - r.o.x) * r.inv.x, ay = (a.y - r.o.y) * r.inv.y, az = (a.z - r.o.z) * r.inv.z;
+    const float  bx = a.w * r.inv.x, by = b.x * r.inv.y, bz = b.y * r.inv.z;
+    const float  anx = __builtin_fmaf(-__builtin_fabsf(ax), 0x1p-21f, ax), afx = __builtin_fmaf(__builtin_fabsf(ax), 0x1p-21f, ax);
+    const float  any_ = __builtin_fmaf(-__builtin_fabsf(ay), 0x1p-21f, ay), afy = __builtin_fmaf(__builtin_fabsf(ay), 0x1p-21f, ay);
+    const float  anz = __builtin_fmaf(-__builtin_fabsf(az), 0x1p-21f, az), afz = __builtin_fmaf(__builtin_fabsf(az), 0x1p-21f, az);
+    // near planes are the low ones along a positive direction component, the high ones along a negative one
+    const uint32_t lox = __float_as_uint(b.z), loy = __float_as_uint(b.w), loz = c.x;
+    const bool     sx = r.inv.x < 0.0f, sy = r.inv.y < 0.0f, sz = r.inv.z < 0.0f;
+    const uint32_t nx = sx ? c.y : lox, fx = sx ? lox : c.y;
+    const uint32_t ny = sy ? c.z : loy, fy = sy ? loy : c.z;
+    const uint32_t nz = sz ? c.w : loz, fz = sz ? loz : c.w;
+    t0 = child_enter<0>(nx, ny, nz, fx, fy, fz, bx, by, bz, anx, any_, anz, afx, afy, afz, tmin, tmax);
+    t1 = child_enter<1>(nx, ny, nz, fx, fy, fz, bx, by, bz, anx, any_, anz, afx, afy, afz, tmin, tmax);
+    t2 = child_enter<2>(nx, ny, nz, fx, fy, fz, bx, by, bz, anx, any_, anz, afx, afy, afz, tmin, tmax);
+    t3 = child_enter<3>(nx, ny, nz, fx, fy, fz, bx, by, bz, anx, any_, anz, afx, afy, afz, tmin, tmax);
+    r0 = rf.x;
+    r1 = rf.y;
+    r2 = rf.z;
+    r3 = rf.w;
+    if (r1 == kEmptyRef) t1 = __builtin_inff();
+    if (r2 == kEmptyRef) t2 = __builtin_inff();
+    if (r3 == kEmptyRef) t3 = __builtin_inff();
 }
 
 __device__ __forceinline__ float dop(float a, float b, float c, float d) {
@@ -193,7 +234,7 @@ __device__ __forceinline__ void finish_barycentrics(TraceResult& res) {
 
 // The pointers traversal needs from a scene.
 struct TraceScene {
-    const GpuNode4* nodes;
+    const GpuNodeQ* nodes;
     const GpuTri*   tris;
     uint32_t        node_count;
 };
@@ -300,17 +341,9 @@ __device__ __forceinline__ bool trace_step_dyn(const TraceScene& sc, TraceState&
             }
         }
     } else if (node) {
-        const float4* np = reinterpret_cast<const float4*>(sc.nodes + st.ref);
-        const float4  lx = gld(np + 0), ly = gld(np + 1), lz = gld(np + 2), hx = gld(np + 3), hy = gld(np + 4), hz = gld(np + 5);
-        const uint4   rf = gld(reinterpret_cast<const uint4*>(np) + 6);
-        float    t0 = box_enter(r, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, st.tmin, st.res.t);
-        float    t1 = box_enter(r, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, st.tmin, st.res.t);
-        float    t2 = box_enter(r, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, st.tmin, st.res.t);
-        float    t3 = box_enter(r, lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, st.tmin, st.res.t);
-        uint32_t r0 = rf.x, r1 = rf.y, r2 = rf.z, r3 = rf.w;
-        if (r1 == kEmptyRef) t1 = __builtin_inff();
-        if (r2 == kEmptyRef) t2 = __builtin_inff();
-        if (r3 == kEmptyRef) t3 = __builtin_inff();
+        float    t0, t1, t2, t3;
+        uint32_t r0, r1, r2, r3;
+        node_boxes(r, reinterpret_cast<const float4*>(sc.nodes + st.ref), st.tmin, st.res.t, t0, t1, t2, t3, r0, r1, r2, r3);
         if (STATS) bounds_tests += 4;
         // sorting network on (t, ref): nearest first, misses (+inf) last
         sort2(t0, r0, t1, r1);
@@ -363,17 +396,9 @@ __device__ __forceinline__ TraceResult trace_ray(const GpuScene& sc, const RayPr
     uint32_t ref = 0;
     for (;;) {
         if (!is_leaf(ref)) {
-            const float4* np = reinterpret_cast<const float4*>(sc.nodes + ref);
-            const float4  lx = gld(np + 0), ly = gld(np + 1), lz = gld(np + 2), hx = gld(np + 3), hy = gld(np + 4), hz = gld(np + 5);
-            const uint4   rf = gld(reinterpret_cast<const uint4*>(np) + 6);
-            float    t0 = box_enter(r, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, tmin, res.t);
-            float    t1 = box_enter(r, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, tmin, res.t);
-            float    t2 = box_enter(r, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, tmin, res.t);
-            float    t3 = box_enter(r, lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, tmin, res.t);
-            uint32_t r0 = rf.x, r1 = rf.y, r2 = rf.z, r3 = rf.w;
-            if (r1 == kEmptyRef) t1 = __builtin_inff();
-            if (r2 == kEmptyRef) t2 = __builtin_inff();
-            if (r3 == kEmptyRef) t3 = __builtin_inff();
+            float    t0, t1, t2, t3;
+            uint32_t r0, r1, r2, r3;
+            node_boxes(r, reinterpret_cast<const float4*>(sc.nodes + ref), tmin, res.t, t0, t1, t2, t3, r0, r1, r2, r3);
             if (STATS) *bounds_tests += 4;
             // sorting network on (t, ref): nearest first, misses (+inf) last
             sort2(t0, r0, t1, r1);

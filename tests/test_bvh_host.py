@@ -43,3 +43,26 @@ def test_icosphere_level6():
     s = build(sc.positions)
     assert s["bad"] == 0 and s["prims"] == sc.triangle_count and s["depth"] <= 30 and s["stack"] <= 64
     assert s["nodes"] < sc.triangle_count / 3  # four-wide nodes: about a third of a binary tree's
+
+
+@pytest.mark.parametrize("case", ["far_offset", "tiny", "mixed_scale", "flat", "negative_huge"])
+def test_quantized_boxes_contain_exact_boxes(case):
+    # The device reads child boxes quantized to 8 bits per plane on an exact per-node grid; the build
+    # check counts any decoded child box that does not contain its exact box (as "bad").
+    rng = np.random.default_rng(7)
+    n = 4000
+    small = rng.normal(size=(n, 3, 3)) * 1e-3
+    centers = rng.uniform(-1, 1, size=(n, 1, 3))
+    if case == "far_offset":
+        tri = centers * 10 + small + 1e7
+    elif case == "tiny":
+        tri = (centers + small) * 1e-30
+    elif case == "mixed_scale":
+        tri = np.concatenate([centers[: n // 2] * 1e6 + small[: n // 2], centers[n // 2:] * 1e-6 + small[n // 2:] * 1e-6])
+    elif case == "flat":
+        tri = centers + small
+        tri[..., 2] = 0.25
+    else:
+        tri = -(centers + 1.5) * 3e8 + small * 1e5
+    s = build(tri.astype(np.float32))
+    assert s["bad"] == 0 and s["prims"] == n
