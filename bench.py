@@ -159,7 +159,8 @@ def bench_render(args):
     log(f"[bench] generating workload {args.workload}")
     scene = make_scene(args.workload)
     dev = api.Device(0, schedule="plain_fill", persistent_blocks_per_cu=args.blocks_per_cu,
-                     path_capacity=args.path_capacity, traversal=args.traversal, use_graph=args.graph)
+                     path_capacity=args.path_capacity, traversal=args.traversal, use_graph=args.graph,
+                     batches_in_flight=args.batches_in_flight, hbm_budget_bytes=args.hbm_budget_gib << 30)
     info = dev.query()
     t0 = time.perf_counter()
     bvh = dev.upload(scene)
@@ -526,6 +527,8 @@ def main():
                     help="arrival rates (jobs/s) of the served stream, comma-separated; inf = all at once")
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--path-capacity", type=int, default=0, help="paths in flight per batch (0 = library default)")
+    ap.add_argument("--batches-in-flight", type=int, default=0, help="batches rendering at once (0 = library default)")
+    ap.add_argument("--hbm-budget-gib", type=int, default=0, help="device admission budget (0 = library default)")
     ap.add_argument("--traversal", default="auto", choices=["auto", "split", "fused", "refill"])
     ap.add_argument("--cpu-stride", type=int, default=1,
                     help="intersect workload: the FP64 oracle checks and times every k-th ray")

@@ -168,7 +168,7 @@ struct mrg_scene {
     int            refs = 1;
     GpuScene       gpu{};
     mrg_scene_info info{};
-    DevBuf         nodes, tris, normals, slot_instance, slot_emitter, materials, choices, cdf, areas, directional;
+    DevBuf         nodes, tris, normals, slot_emitter, materials, choices, cdf, areas, directional;
     DevBuf         uvs, textures, texels, env_cdf, env_rows;
     bool           extended = false;  // uses textures, an environment map or energy compensation
 };
@@ -465,7 +465,6 @@ mrg_scene* build_scene(mrg_device* d, const mrg_scene_desc& s, const std::string
     // leaf-order triangle records
     std::vector<GpuTri>        tris(n);
     std::vector<GpuTriNormals> nrms(n);
-    std::vector<uint32_t> slot_instance(n);
     std::vector<int32_t>  slot_emitter(n, -1);
     // light table (LightSampler<float> of the oracle, light.cpp:197-264)
     std::vector<GpuLightChoice> choices;
@@ -552,8 +551,8 @@ mrg_scene* build_scene(mrg_device* d, const mrg_scene_desc& s, const std::string
         t.v1[0] = v[3]; t.v1[1] = v[4]; t.v1[2] = v[5];
         t.v2[0] = v[6]; t.v2[1] = v[7]; t.v2[2] = v[8];
         t.prim = p;
-        t.pad1 = (m.double_sided ? 0u : 1u) | (m.mirrored ? 2u : 0u);
-        t.pad2 = 0;
+        t.flags = (m.double_sided ? 0u : 1u) | (m.mirrored ? 2u : 0u);
+        t.instance = s.instance[p];
         // surface_hit's per-triangle constants in its float operations (normalize = v * (1 / sqrt(v.v))):
         // the normalized corner normals and the geometric normal before a mirrored instance's flip
         GpuTriNormals& r  = nrms[slot];
@@ -564,7 +563,6 @@ mrg_scene* build_scene(mrg_device* d, const mrg_scene_desc& s, const std::string
         r.gx = gn.x;
         r.gy = gn.y;
         r.gz = gn.z;
-        slot_instance[slot] = s.instance[p];
         slot_emitter[slot]  = area_choice[p];
     }
     // per-corner uvs in leaf order; level-0 texels of every texture
@@ -605,7 +603,6 @@ mrg_scene* build_scene(mrg_device* d, const mrg_scene_desc& s, const std::string
     upload(sc->nodes, bvh.qnodes);
     upload(sc->tris, tris);
     upload(sc->normals, nrms);
-    upload(sc->slot_instance, slot_instance);
     upload(sc->slot_emitter, slot_emitter);
     upload(sc->materials, mats);
     upload(sc->choices, choices);
@@ -621,7 +618,6 @@ mrg_scene* build_scene(mrg_device* d, const mrg_scene_desc& s, const std::string
     g.nodes         = sc->nodes.as<GpuNodeQ>();
     g.tris          = sc->tris.as<GpuTri>();
     g.normals       = sc->normals.as<GpuTriNormals>();
-    g.slot_instance = sc->slot_instance.as<uint32_t>();
     g.slot_emitter  = sc->slot_emitter.as<int32_t>();
     g.materials     = sc->materials.as<GpuMaterial>();
     g.choices       = sc->choices.as<GpuLightChoice>();
@@ -655,7 +651,7 @@ mrg_scene* build_scene(mrg_device* d, const mrg_scene_desc& s, const std::string
     info.nodes           = bvh.nodes.size();
     info.leaves          = bvh.leaves;
     info.maximum_depth   = bvh.max_depth;
-    info.device_bytes    = sc->nodes.bytes + sc->tris.bytes + sc->normals.bytes + sc->slot_instance.bytes + sc->slot_emitter.bytes +
+    info.device_bytes    = sc->nodes.bytes + sc->tris.bytes + sc->normals.bytes + sc->slot_emitter.bytes +
                         sc->materials.bytes + sc->choices.bytes + sc->cdf.bytes + sc->areas.bytes + sc->directional.bytes +
                         sc->uvs.bytes + sc->textures.bytes + sc->texels.bytes + sc->env_cdf.bytes + sc->env_rows.bytes;
     info.build_seconds = seconds_since(t0);
@@ -967,7 +963,7 @@ void launch_bounce_loop(mrg_device* d, Track& t, bool extended, bool refill, boo
         for (uint32_t it = 0; it < t.iters; ++it) {
             check_hip(trace(it), "k_trace");
             rec(1);
-            check_hip(launch_shade(t.pool, jobs, it, extended, d->shape, t.stream), "k_shade");
+            check_hip(launch_shade(t.pool, jobs, it, extended, one_scene, d->shape, t.stream), "k_shade");
             rec(2);
         }
         // the last shading pass's shadow rays (its extension queue is empty: no path continues
@@ -978,7 +974,7 @@ void launch_bounce_loop(mrg_device* d, Track& t, bool extended, bool refill, boo
         for (uint32_t it = 0; it < t.iters; ++it) {
             check_hip(launch_extend(t.pool, jobs, it, d->shape, t.stream), "k_extend");
             rec(1);
-            check_hip(launch_shade(t.pool, jobs, it, extended, d->shape, t.stream), "k_shade");
+            check_hip(launch_shade(t.pool, jobs, it, extended, one_scene, d->shape, t.stream), "k_shade");
             rec(2);
             check_hip(launch_shadow(t.pool, jobs, it, d->shape, t.stream), "k_shadow");
             rec(3);

@@ -93,7 +93,9 @@ hipError_t launch_generate(const Pool& P, const GpuJob* jobs, const GpuChunk* ch
 hipError_t launch_extend(const Pool& P, const GpuJob* jobs, uint32_t iter, const LaunchShape& shape, hipStream_t stream);
 // extended: the shading kernel with base-color textures, environment maps and energy-compensated
 // conductors (the plain one covers scenes that use none of them).
-hipError_t launch_shade(const Pool& P, const GpuJob* jobs, uint32_t iter, bool extended, const LaunchShape& shape, hipStream_t stream);
+// one_scene: every job of the batch renders jobs[0]'s scene.
+hipError_t launch_shade(const Pool& P, const GpuJob* jobs, uint32_t iter, bool extended, bool one_scene, const LaunchShape& shape,
+                        hipStream_t stream);
 hipError_t launch_shadow(const Pool& P, const GpuJob* jobs, uint32_t iter, const LaunchShape& shape, hipStream_t stream);
 // Fused traversal: shadow rays of bounce iter - 1 and extension rays of bounce iter in one launch.
 hipError_t launch_trace(const Pool& P, const GpuJob* jobs, uint32_t iter, bool one_scene, const LaunchShape& shape,

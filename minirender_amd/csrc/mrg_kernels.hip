@@ -621,21 +621,30 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) k
         nb, nt);
 }
 
-// One path of the shading pass (the loop body of k_shade): reads the path state and its hit, adds
+// The path state and hit record of one shading thread, loaded before its job is known (they
+// depend on the path slot alone), and the hit triangle, loaded as soon as the hit slot arrives.
+struct ShadeIn {
+    float4   pv, rd, hv;
+    uint4    rg;
+    uint32_t slot;
+    float4   ta, tb, tc;  // hit triangle (vertices; v1.w = flags, v2.w = instance)
+};
+
+// One path of the shading pass (the loop body of k_shade): takes the path state and its hit, adds
 // emission, samples a light and the BSDF, and leaves the continuation and shadow-ray outputs in the
-// references. `job` is the path's job; a caller whose wave holds a single job passes a reference
-// through a wave-uniform index, so the job's scene fields load through the scalar cache.
+// references. `job` is the path's job and `sc` its scene; a caller whose wave holds a single job
+// passes references through wave-uniform indices, so their fields load through the scalar cache.
 template <bool EXT>
-__device__ __forceinline__ void shade_path(const Pool& P, const GpuJob& job, uint32_t path, float4 pv, uint32_t iter, bool& cont,
-                                           bool& emit, uint32_t& ext_key, uint32_t& sh_key, float4* s_shadow,
-                                           uint32_t& n_scatter, uint32_t& n_rr, uint32_t& n_shadow) {
+__device__ __forceinline__ void shade_path(const Pool& P, const GpuJob& job, const GpuScene& sc, uint32_t path, const ShadeIn& in,
+                                           uint32_t iter, bool& cont, bool& emit, uint32_t& ext_key, uint32_t& sh_key,
+                                           float4* s_shadow, uint32_t& n_scatter, uint32_t& n_rr, uint32_t& n_shadow) {
         // the cancel word, read first and used last so that its latency hides behind the shading; the host
         // raises it while kernels run, so it is read past the per-XCD L2s
         const bool     cancelled = __hip_atomic_load(job.error_flag + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
+        const float4   pv    = in.pv;
         const uint32_t flags = __float_as_uint(pv.w);
         const uint32_t depth = iter;
-        const GpuScene& sc  = job.scene;
-        const float4  ro = P.ray_o[path], rd = P.ray_d[path];
+        const float4  rd = in.rd;
         const F3      ray_d = mk(rd.x, rd.y, rd.z);
         // camera paths (bounce 0) start with beta = 1, pdf 0, L = 0, eta scale 1: k_generate
         // writes none of them and they are not read here
@@ -647,13 +656,12 @@ __device__ __forceinline__ void shade_path(const Pool& P, const GpuJob& job, uin
         float         eta_scale = lr.w;
         F3            prev_ref  = mk(pv.x, pv.y, pv.z);
         bool          prev_delta = (flags & kPrevDelta) != 0u;
-        const uint4   rg = P.rng[path];
+        const uint4   rg = in.rg;
         Pcg           rng{static_cast<uint64_t>(rg.x) | (static_cast<uint64_t>(rg.y) << 32),
                           static_cast<uint64_t>(rg.z) | (static_cast<uint64_t>(rg.w) << 32)};
-        const float4  hv   = P.hit[path];
-        const uint32_t slot = P.hit_slot[path];
+        const float4  hv   = in.hv;
+        const uint32_t slot = in.slot;
         bool bad = false;
-        (void)ro;
         do {
             if (!finite3(beta) || beta.x < 0.0f || beta.y < 0.0f || beta.z < 0.0f) { bad = true; break; }
             if (vmax(beta) == 0.0f) break;
@@ -667,12 +675,12 @@ __device__ __forceinline__ void shade_path(const Pool& P, const GpuJob& job, uin
             }
             // surface_hit (bvh.cpp:302-370)
             const float4* tp = reinterpret_cast<const float4*>(sc.tris + slot);
-            const float4  ta = gld(tp + 0), tb = gld(tp + 1), tc = gld(tp + 2);
+            const float4  ta = EXT ? gld(tp + 0) : in.ta, tb = EXT ? gld(tp + 1) : in.tb, tc = EXT ? gld(tp + 2) : in.tc;
             const F3      v0 = mk(ta.x, ta.y, ta.z), v1 = mk(tb.x, tb.y, tb.z), v2 = mk(tc.x, tc.y, tc.z);
             const uint32_t tflags = __float_as_uint(tb.w);
             const float4* np = reinterpret_cast<const float4*>(sc.normals + slot);
             const float4  na = gld(np + 0), nb = gld(np + 1), nc = gld(np + 2);
-            const uint32_t inst = gld(sc.slot_instance + slot);
+            const uint32_t inst = __float_as_uint(tc.w);
             const GpuMaterial& mat = sc.materials[inst];
             // corner normals arrive normalized and the geometric normal in their w components
             // (computed at upload with these same float operations)
@@ -837,7 +845,9 @@ __device__ __forceinline__ void shade_path(const Pool& P, const GpuJob& job, uin
 // and scene loads, so occupancy pays more than the registers the compiler would otherwise use.
 // EXT = false compiles out base-color textures, environment maps and energy compensation: scenes
 // without them run a kernel with fewer live registers.
-template <bool EXT>
+// ONE_SCENE: every job of the batch renders jobs[0]'s scene, whose pointers are read once (scalar),
+// so the hit triangle's loads wait for the hit slot alone.
+template <bool EXT, bool ONE_SCENE>
 __global__ void __launch_bounds__(kShadeThreads) __attribute__((amdgpu_waves_per_eu(4))) k_shade(Pool P, const GpuJob* __restrict__ jobs, uint32_t iter) {
     const uint32_t  n       = P.counters[kCtrExt + iter];
     const uint32_t* queue   = ext_queue(P, iter);
@@ -858,13 +868,28 @@ __global__ void __launch_bounds__(kShadeThreads) __attribute__((amdgpu_waves_per
         uint32_t       path     = 0;
         if (i < n) {
             path = queue ? queue[i] : i;
-            const float4   pv  = P.prev[path];
-            const uint32_t jid = __float_as_uint(pv.w) >> 16;
+            // everything that depends on the path slot alone is in flight before the job is known
+            ShadeIn in;
+            in.pv   = P.prev[path];
+            in.rd   = P.ray_d[path];
+            in.hv   = P.hit[path];
+            in.slot = P.hit_slot[path];
+            in.rg   = P.rng[path];
+            const uint32_t jid = __float_as_uint(in.pv.w) >> 16;
             const uint32_t j0  = __builtin_amdgcn_readfirstlane(jid);
-            if (__ballot(jid != j0) == 0)
-                shade_path<EXT>(P, jobs[j0], path, pv, iter, cont, emit, ext_key, sh_key, s_shadow, n_scatter, n_rr, n_shadow);
+            const bool     uniform = __ballot(jid != j0) == 0;
+            const GpuScene& sc = ONE_SCENE ? jobs[0].scene : (uniform ? jobs[j0].scene : jobs[jid].scene);
+            // the hit triangle, fetched before the shading decides it needs it (a
+            // missed ray reads the first triangle instead, unused; a scene without triangles has no hits)
+            if (!EXT && sc.tri_count != 0u) {  // (the extended kernel has no registers to spare for it)
+                const uint32_t t  = in.slot != 0xffffffffu ? in.slot : 0u;
+                const float4*  tp = reinterpret_cast<const float4*>(sc.tris + t);
+                in.ta = gld(tp + 0), in.tb = gld(tp + 1), in.tc = gld(tp + 2);
+            }
+            if (uniform)
+                shade_path<EXT>(P, jobs[j0], sc, path, in, iter, cont, emit, ext_key, sh_key, s_shadow, n_scatter, n_rr, n_shadow);
             else
-                shade_path<EXT>(P, jobs[jid], path, pv, iter, cont, emit, ext_key, sh_key, s_shadow, n_scatter, n_rr, n_shadow);
+                shade_path<EXT>(P, jobs[jid], sc, path, in, iter, cont, emit, ext_key, sh_key, s_shadow, n_scatter, n_rr, n_shadow);
         }
         const GroupedSlots g = block_reserve_grouped<kShadeThreads / 64>(next_n, cont, ext_key, sh_n, emit, sh_key, s_off, s_base);
         const uint32_t cpos = g.ext, spos = g.shadow;
@@ -1012,10 +1037,16 @@ hipError_t launch_extend(const Pool& P, const GpuJob* jobs, uint32_t iter, const
     return hipGetLastError();
 }
 
-hipError_t launch_shade(const Pool& P, const GpuJob* jobs, uint32_t iter, bool extended, const LaunchShape& shape, hipStream_t stream) {
+hipError_t launch_shade(const Pool& P, const GpuJob* jobs, uint32_t iter, bool extended, bool one_scene, const LaunchShape& shape,
+                        hipStream_t stream) {
     const dim3 grid(shape.shade_blocks * 256 / kShadeThreads), block(kShadeThreads);
-    if (extended) hipLaunchKernelGGL(k_shade<true>, grid, block, 0, stream, P, jobs, iter);
-    else hipLaunchKernelGGL(k_shade<false>, grid, block, 0, stream, P, jobs, iter);
+    if (extended) {
+        if (one_scene) hipLaunchKernelGGL((k_shade<true, true>), grid, block, 0, stream, P, jobs, iter);
+        else hipLaunchKernelGGL((k_shade<true, false>), grid, block, 0, stream, P, jobs, iter);
+    } else {
+        if (one_scene) hipLaunchKernelGGL((k_shade<false, true>), grid, block, 0, stream, P, jobs, iter);
+        else hipLaunchKernelGGL((k_shade<false, false>), grid, block, 0, stream, P, jobs, iter);
+    }
     return hipGetLastError();
 }
 

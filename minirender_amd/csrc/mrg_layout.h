@@ -6,10 +6,11 @@
 //                              grid, plus the four child references, so one node fetch gives the
 //                              whole four-box test. Depth-first order, root = 0.
 //   tris       [n]             GpuTri, 48 B: three float4 vertices in leaf order; v0.w carries the
-//                              scene-order primitive index (the tie-break key of bvh.cpp:273-276).
+//                              scene-order primitive index (the tie-break key of bvh.cpp:273-276),
+//                              v1.w the sidedness flags and v2.w the instance (material) index.
 //   normals    [n]             GpuTriNormals, 48 B: the three corner normals normalized, with the
 //                              triangle's unit geometric normal in the w components, in leaf order.
-//   slot_info  [n]             uint2 (instance, area-light choice or -1) in leaf order.
+//   slot_emitter [n]           area-light choice of each triangle or -1, in leaf order.
 //   materials  [instances]     GpuMaterial.
 //   uvs        [n]             per-corner texture coordinates in leaf order (3 x float2), if any.
 //   textures                   level-0 texels of the base-color textures (float4 each).
@@ -69,9 +70,9 @@ struct alignas(16) GpuTri {
     float v0[3];
     uint32_t prim;
     float v1[3];
-    uint32_t pad1;
+    uint32_t flags;     // bit 0 single-sided, bit 1 mirrored instance
     float v2[3];
-    uint32_t pad2;
+    uint32_t instance;  // material index
 };
 static_assert(sizeof(GpuTri) == 48, "GpuTri must be 48 bytes");
 
@@ -134,7 +135,6 @@ struct GpuScene {
     const GpuNodeQ*       nodes;
     const GpuTri*         tris;
     const GpuTriNormals*  normals;
-    const uint32_t*       slot_instance;
     const int32_t*        slot_emitter;
     const GpuMaterial*    materials;
     const GpuLightChoice* choices;

@@ -164,7 +164,7 @@ __device__ __forceinline__ bool tri_test(const RayPre& r, F3 v0, F3 v1, F3 v2, f
     return true;
 }
 
-// Triangle record flags (GpuTri::pad1): bit 0 single-sided, bit 1 mirrored instance.
+// Triangle record flags (GpuTri::flags): bit 0 single-sided, bit 1 mirrored instance.
 constexpr uint32_t kTriSingleSided = 1u;
 constexpr uint32_t kTriMirrored    = 2u;
 
