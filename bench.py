@@ -339,7 +339,7 @@ def bench_intersect(args):
     n = 1 << 24
     log("[bench] intersect: generating 16M rays")
     rays = scenegen.intersection_rays(scene, n)
-    dev = api.Device(0, persistent_blocks_per_cu=args.blocks_per_cu)
+    dev = api.Device(0, persistent_blocks_per_cu=args.blocks_per_cu, query_precision=args.query_precision)
     bvh = dev.upload(scene)
     hip = ctypes.CDLL("libamdhip64.so")
     d_rays, d_hits = ctypes.c_void_p(), ctypes.c_void_p()
@@ -361,9 +361,11 @@ def bench_intersect(args):
     value = n / avg / 1e3
     result = {"metric": "Mrays/s closest hit (intersection microbench)", "value": round(value, 1), "unit": "Mrays/s",
               "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(avg, 3),
-              "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+              "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+              "dtype": "f32 traversal, f64 hit decisions" if args.query_precision == 64 else "f32", "data": "synthetic",
               "config": {"workload": f"icosphere level 8 ({scene.triangle_count} tris, 5% value-noise displacement), "
-                                     f"{n} rays, closest hit", "triangles": scene.triangle_count, "rays": n},
+                                     f"{n} rays, closest hit", "triangles": scene.triangle_count, "rays": n,
+                         "query_precision": args.query_precision},
               "pcie_inclusive_mrays_per_s": round(n / pcie_s / 1e6, 1)}
     if not args.no_cpu:
         from oracle import oracle as O
@@ -389,10 +391,13 @@ def bench_intersect(args):
                 edge[i] = bool(va & {tuple(v) for v in scene.positions[ref["prim"][i]].tolist()})
         both = (~mism) & (ref["prim"] >= 0)
         rel = np.abs(hits["t"][idx][both] - ref["t"][both]) / ref["t"][both]
+        hit64 = ref["prim"] >= 0
         result["parity"] = {"sample_rays": int(len(idx)), "id_mismatch": int(mism.sum()),
                             "near_ties": int(near.sum()), "id_mismatch_outside_near_ties": int((mism & ~near).sum()),
                             "of_which_shared_edge_crossings": int(edge.sum()),
-                            "max_t_rel": float(rel.max(initial=0))}
+                            "max_t_rel": float(rel.max(initial=0)),
+                            "t_equal_to_fp64_rounded": int((hits["t"][idx][hit64] == ref["t"][hit64].astype(np.float32)).sum()),
+                            "hits": int(hit64.sum())}
         result["roofline"] = roofline_entry("k_query", "intersect", avg / 1e3, n, 0, b, tt, 0, 0)
         result["cpu_baseline"] = {"value": round(len(idx) / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads,
                                   "kind": "port", "host": host,
@@ -529,6 +534,8 @@ def main():
     ap.add_argument("--path-capacity", type=int, default=0, help="paths in flight per batch (0 = library default)")
     ap.add_argument("--batches-in-flight", type=int, default=0, help="batches rendering at once (0 = library default)")
     ap.add_argument("--hbm-budget-gib", type=int, default=0, help="device admission budget (0 = library default)")
+    ap.add_argument("--query-precision", type=int, default=64, choices=[32, 64],
+                    help="intersect workload: hit decisions in FP64 (the reference intersector's) or the render kernels' FP32")
     ap.add_argument("--traversal", default="auto", choices=["auto", "split", "fused", "refill"])
     ap.add_argument("--cpu-stride", type=int, default=1,
                     help="intersect workload: the FP64 oracle checks and times every k-th ray")

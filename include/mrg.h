@@ -198,6 +198,11 @@ typedef struct mrg_device_options {
     uint32_t batches_in_flight;  /* 2: a second path pool renders the next batch while the previous one
                                     drains its tail of late bounces (films still commit in batch order);
                                     0 or 1: one batch at a time */
+    uint32_t query_precision;    /* hit decisions of the ray-query entry points: 64 (or 0, the default) =
+                                    FP64, the reference intersector's arithmetic (RayQuery in double,
+                                    intersection.cpp:108-152; bvh.cpp:259-276), so ids, distances and
+                                    barycentrics are its own rounded to float; 32 = the render kernels'
+                                    FP32 watertight test (the FP32 twin's) */
 } mrg_device_options;
 
 /* Hit of the intersection entry points: closest t and scene-order primitive index (-1 = miss). */

@@ -81,7 +81,8 @@ class Progress(C.Structure):
 class DeviceOptions(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_uint64), ("path_capacity", C.c_uint64), ("schedule", C.c_uint32),
                 ("max_jobs_in_flight", C.c_uint32), ("use_graph", C.c_uint32), ("block_size", C.c_uint32),
-                ("persistent_blocks_per_cu", C.c_uint32), ("traversal", C.c_uint32), ("batches_in_flight", C.c_uint32)]
+                ("persistent_blocks_per_cu", C.c_uint32), ("traversal", C.c_uint32), ("batches_in_flight", C.c_uint32),
+                ("query_precision", C.c_uint32)]
 
 
 class Hit(C.Structure):

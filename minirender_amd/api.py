@@ -77,7 +77,7 @@ class Device:
     """One MI355X (or any gfx950 device) with its own scheduler thread inside libmrg."""
 
     def __init__(self, ordinal=0, schedule="plain", hbm_budget_bytes=0, path_capacity=0, max_jobs_in_flight=0,
-                 use_graph=False, persistent_blocks_per_cu=0, traversal="auto", batches_in_flight=0):
+                 use_graph=False, persistent_blocks_per_cu=0, traversal="auto", batches_in_flight=0, query_precision=64):
         L = N.lib()
         o = N.DeviceOptions()
         o.hbm_budget_bytes = hbm_budget_bytes
@@ -88,6 +88,7 @@ class Device:
         o.persistent_blocks_per_cu = persistent_blocks_per_cu
         o.traversal = {"auto": 0, "split": 1, "fused": 2, "refill": 3}[traversal]
         o.batches_in_flight = batches_in_flight
+        o.query_precision = query_precision
         h = C.c_void_p()
         N.check(L.mrg_device_open(ordinal, C.byref(o), C.byref(h)))
         self.handle = h

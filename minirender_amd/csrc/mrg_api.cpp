@@ -1329,6 +1329,8 @@ int mrg_device_open(int ordinal, const mrg_device_options* options, mrg_device**
         if (d->opt.max_jobs_in_flight == 0) d->opt.max_jobs_in_flight = 64;
         if (d->opt.max_jobs_in_flight > 4096) throw std::invalid_argument("max_jobs_in_flight too large");
         if (d->opt.batches_in_flight > 2) throw std::invalid_argument("batches_in_flight must be 0 (automatic), 1 or 2");
+        if (d->opt.query_precision == 0) d->opt.query_precision = 64;
+        if (d->opt.query_precision != 32 && d->opt.query_precision != 64) throw std::invalid_argument("query_precision must be 0, 32 or 64");
         alloc_pool(d.get());
         // the allocation-free query's workspace: one static array per GPU, looked up with the device current
         check_hip(noalloc_scratch_address(&d->noalloc_scratch), "hipGetSymbolAddress(scratch)");
@@ -1503,7 +1505,8 @@ static void query_locked(mrg_scene* s, const void* d_rays, uint64_t n, float tmi
     hipEvent_t e0 = d->query_events[0], e1 = d->query_events[1];
     check_hip(hipEventRecord(e0, d->query_stream), "hipEventRecord");
     check_hip(launch_query(s->gpu, static_cast<const float*>(d_rays), uint32_t(n), tmin, tmax, static_cast<mrg_hit_raw*>(d_hits), d_bary,
-                           d_occ, work, d->query_pool, stats_dev, any, stats != nullptr, d->shape, d->query_stream),
+                           d_occ, work, d->query_pool, stats_dev, any, stats != nullptr, d->opt.query_precision == 64, d->shape,
+                           d->query_stream),
               "k_query");
     check_hip(hipEventRecord(e1, d->query_stream), "hipEventRecord");
     uint64_t h[2] = {0, 0};

@@ -85,8 +85,9 @@ struct LaunchShape {
     uint32_t query_blocks;      // persistent grid of the ray-query kernel (k_query)
 };
 
+// exact: hit decisions in FP64 (the reference intersector's arithmetic); otherwise the render kernels' FP32 test.
 hipError_t launch_query(const GpuScene& sc, const float* rays, uint32_t n, float tmin, float tmax, mrg_hit_raw* hits, float* bary,
-                        uint8_t* occluded, uint32_t* work, const Pool& P, uint64_t* stats, bool any, bool with_stats,
+                        uint8_t* occluded, uint32_t* work, const Pool& P, uint64_t* stats, bool any, bool with_stats, bool exact,
                         const LaunchShape& shape, hipStream_t stream);
 hipError_t launch_reset(const Pool& P, hipStream_t stream);
 hipError_t launch_generate(const Pool& P, const GpuJob* jobs, const GpuChunk* chunks, const LaunchShape& shape, hipStream_t stream);

@@ -207,7 +207,7 @@ constexpr int      kQueryLeafBatch    = 24;
 constexpr int      kBounceLeafBatch   = 24;
 constexpr uint32_t kRefillClaim = 128;  // entries per atomic claim (>= 64; 128 measured best, +61% over an atomic per refill)
 
-template <int REFILL_LANES, int LEAF_BATCH, bool STATS, bool UNIFORM, class Start, class Finish, class Keep>
+template <int REFILL_LANES, int LEAF_BATCH, bool STATS, bool UNIFORM, bool EXACT, class Start, class Finish, class Keep>
 __device__ __forceinline__ void persistent_trace_mixed(const StackView& stk, uint32_t n, uint32_t* work, const TraceScene& uscene,
                                                        Start start, Finish finish, const Keep& keep, uint64_t& nb, uint64_t& nt) {
     TraceState st;
@@ -254,7 +254,7 @@ __device__ __forceinline__ void persistent_trace_mixed(const StackView& stk, uin
             }
         }
         if (__ballot(active) == 0) break;
-        if (active && trace_step_dyn<LEAF_BATCH, STATS>(UNIFORM ? uscene : ts, st, stk, nb, nt, any, keep)) {
+        if (active && trace_step_dyn<LEAF_BATCH, STATS, EXACT>(UNIFORM ? uscene : ts, st, stk, nb, nt, any, keep)) {
             active  = false;
             pending = true;
         }
@@ -263,10 +263,10 @@ __device__ __forceinline__ void persistent_trace_mixed(const StackView& stk, uin
 }
 
 // Every item of one kind against one scene: closest hit (ANY = false) or any hit (ANY = true).
-template <bool ANY, bool STATS, class Start, class Finish>
+template <bool ANY, bool STATS, bool EXACT, class Start, class Finish>
 __device__ __forceinline__ void persistent_trace(const StackView& stk, uint32_t n, uint32_t* work, const TraceScene& scene,
                                                  Start start, Finish finish, uint64_t& nb, uint64_t& nt) {
-    persistent_trace_mixed<kQueryRefillLanes, kQueryLeafBatch, STATS, true>(
+    persistent_trace_mixed<kQueryRefillLanes, kQueryLeafBatch, STATS, true, EXACT>(
         stk, n, work, scene,
         [&](uint32_t i, TraceState& st, bool& any) {
             any = ANY;
@@ -296,17 +296,19 @@ __device__ __forceinline__ float refine_distance(const GpuScene& sc, uint32_t sl
 // Intersection microbenchmark / SceneBvh::intersect and ::occluded over a packed ray array.
 // Six waves per SIMD (at most 80 VGPRs, 26 KB of LDS stack per block: the incoherent rays of queries
 // stack deeper than path rays, and measured 2.5% slower at seven waves with 22 KB stacks).
-template <bool ANY, bool STATS>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k_query(GpuScene sc, const float* __restrict__ rays, uint32_t n, float tmin, float tmax,
-                                               mrg_hit_raw* __restrict__ hits, float* __restrict__ bary,
-                                               uint8_t* __restrict__ occluded, uint32_t* __restrict__ work, Pool P,
-                                               uint64_t* __restrict__ stats) {
+// EXACT: hit decisions in FP64 as the reference's intersector (tri_test64): ids, distances and
+// barycentrics are the FP64 ones rounded to float. Otherwise the FP32 watertight test of the render
+// kernels, with the winner's distance refined in FP64 (refine_distance).
+template <bool ANY, bool STATS, bool EXACT>
+__device__ __forceinline__ void query_body(const GpuScene& sc, const float* __restrict__ rays, uint32_t n, float tmin, float tmax,
+                                           mrg_hit_raw* __restrict__ hits, float* __restrict__ bary, uint8_t* __restrict__ occluded,
+                                           uint32_t* __restrict__ work, const Pool& P, uint64_t* __restrict__ stats) {
     __shared__ uint32_t s_ref[kStackLdsQuery * kTraversalBlock];
     __shared__ float    s_t[kStackLdsQuery * kTraversalBlock];
     const StackView stk = make_stack(s_ref, s_t, P, kStackLdsQuery);
     uint64_t        nb = 0, nt = 0;
     const TraceScene ts = trace_scene(sc);
-    persistent_trace<ANY, STATS>(
+    persistent_trace<ANY, STATS, EXACT>(
         stk, n, work, ts,
         [&](uint32_t i, TraceState& st) {
             const float* r = rays + static_cast<size_t>(i) * 6;
@@ -316,6 +318,28 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k
             TraceResult res = st.res;
             if (ANY) {
                 occluded[i] = res.slot != 0xffffffffu ? 1 : 0;
+            } else if (EXACT) {
+                mrg_hit_raw h;
+                h.t         = res.slot != 0xffffffffu ? static_cast<float>(st.best64) : __builtin_inff();
+                h.primitive = res.slot != 0xffffffffu ? static_cast<int32_t>(res.prim) : -1;
+                hits[i]     = h;
+                if (bary) {
+                    float b0 = 0.0f, b1 = 0.0f, b2 = 0.0f;
+                    if (res.slot != 0xffffffffu) {
+                        // the winner's FP64 hit again (same inputs, so it passes with t = best64)
+                        const float4* tp = reinterpret_cast<const float4*>(sc.tris + res.slot);
+                        const float4  a = gld(tp + 0), b = gld(tp + 1), c = gld(tp + 2);
+                        TriHit64      w;
+                        if (tri_test64(st.r, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), mk(c.x, c.y, c.z), st.tmin, st.best64, w)) {
+                            b0 = static_cast<float>(w.a / w.det);
+                            b1 = static_cast<float>(w.b / w.det);
+                            b2 = static_cast<float>(w.c / w.det);
+                        }
+                    }
+                    bary[static_cast<size_t>(i) * 3 + 0] = b0;
+                    bary[static_cast<size_t>(i) * 3 + 1] = b1;
+                    bary[static_cast<size_t>(i) * 3 + 2] = b2;
+                }
             } else {
                 mrg_hit_raw h;
                 h.t         = res.slot != 0xffffffffu ? refine_distance(sc, res.slot, st.r, res.t) : __builtin_inff();
@@ -334,6 +358,24 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k
         atomicAdd(reinterpret_cast<unsigned long long*>(stats), static_cast<unsigned long long>(nb));
         atomicAdd(reinterpret_cast<unsigned long long*>(stats + 1), static_cast<unsigned long long>(nt));
     }
+}
+
+template <bool ANY, bool STATS>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k_query(GpuScene sc, const float* __restrict__ rays, uint32_t n, float tmin, float tmax,
+                                               mrg_hit_raw* __restrict__ hits, float* __restrict__ bary,
+                                               uint8_t* __restrict__ occluded, uint32_t* __restrict__ work, Pool P,
+                                               uint64_t* __restrict__ stats) {
+    query_body<ANY, STATS, false>(sc, rays, n, tmin, tmax, hits, bary, occluded, work, P, stats);
+}
+
+// FP64 hit decisions at five waves per SIMD (96 VGPRs: the out-of-line FP64 test's call needs the
+// registers; measured 2.71 ms per 16.7M-ray launch against 2.96 ms at six waves).
+template <bool ANY, bool STATS>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_query_exact(GpuScene sc, const float* __restrict__ rays, uint32_t n, float tmin, float tmax,
+                                               mrg_hit_raw* __restrict__ hits, float* __restrict__ bary,
+                                               uint8_t* __restrict__ occluded, uint32_t* __restrict__ work, Pool P,
+                                               uint64_t* __restrict__ stats) {
+    query_body<ANY, STATS, true>(sc, rays, n, tmin, tmax, hits, bary, occluded, work, P, stats);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -583,7 +625,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) k
     uint64_t        nb = 0, nt = 0;
     uint32_t        path = 0;
     const TraceScene one = ONE_SCENE ? trace_scene(jobs[0].scene) : TraceScene{nullptr, nullptr, 0};
-    persistent_trace_mixed<kBounceRefillLanes, kBounceLeafBatch, false, ONE_SCENE>(
+    persistent_trace_mixed<kBounceRefillLanes, kBounceLeafBatch, false, ONE_SCENE, false>(
         stk, n_sh + n_ext, work, one,
         [&](uint32_t i, TraceState& st, bool& any) {
             any = i < n_sh;
@@ -1009,16 +1051,28 @@ static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 hipError_t noalloc_scratch_address(void** out) { return hipGetSymbolAddress(out, HIP_SYMBOL(scratch)); }
 
 hipError_t launch_query(const GpuScene& sc, const float* rays, uint32_t n, float tmin, float tmax, mrg_hit_raw* hits, float* bary,
-                        uint8_t* occluded, uint32_t* work, const Pool& P, uint64_t* stats, bool any, bool with_stats,
+                        uint8_t* occluded, uint32_t* work, const Pool& P, uint64_t* stats, bool any, bool with_stats, bool exact,
                         const LaunchShape& shape, hipStream_t stream) {
     const dim3 grid(shape.query_blocks), block(256);
-    if (any) {
-        if (with_stats) hipLaunchKernelGGL((k_query<true, true>), grid, block, 0, stream, sc, rays, n, tmin, tmax, hits, bary, occluded, work, P, stats);
-        else hipLaunchKernelGGL((k_query<true, false>), grid, block, 0, stream, sc, rays, n, tmin, tmax, hits, bary, occluded, work, P, stats);
+#define MRG_QUERY(A, S, E) hipLaunchKernelGGL((E ? k_query_exact<A, S> : k_query<A, S>), grid, block, 0, stream, sc, rays, n, tmin, tmax, hits, bary, occluded, work, P, stats)
+    if (exact) {
+        if (any) {
+            if (with_stats) MRG_QUERY(true, true, true);
+            else MRG_QUERY(true, false, true);
+        } else {
+            if (with_stats) MRG_QUERY(false, true, true);
+            else MRG_QUERY(false, false, true);
+        }
     } else {
-        if (with_stats) hipLaunchKernelGGL((k_query<false, true>), grid, block, 0, stream, sc, rays, n, tmin, tmax, hits, bary, occluded, work, P, stats);
-        else hipLaunchKernelGGL((k_query<false, false>), grid, block, 0, stream, sc, rays, n, tmin, tmax, hits, bary, occluded, work, P, stats);
+        if (any) {
+            if (with_stats) MRG_QUERY(true, true, false);
+            else MRG_QUERY(true, false, false);
+        } else {
+            if (with_stats) MRG_QUERY(false, true, false);
+            else MRG_QUERY(false, false, false);
+        }
     }
+#undef MRG_QUERY
     return hipGetLastError();
 }
 

@@ -164,6 +164,134 @@ __device__ __forceinline__ bool tri_test(const RayPre& r, F3 v0, F3 v1, F3 v2, f
     return true;
 }
 
+// ---- FP64 hit decisions for ray queries
+// The reference's RayQuery::intersect (intersection.cpp:108-152) in double on the float ray and
+// vertices widened to double, operation for operation as the FP64 oracle computes it, so a query's
+// hit ids, distances and barycentrics are the FP64 intersector's bit for bit. Traversal stays in
+// float: the slab test is conservative (it never culls a box holding a hit the FP64 test accepts at a
+// distance within the float bound, which is rounded up from the FP64 best hit), and tri_clear_miss32
+// discards, in float, only triangles the FP64 test provably rejects.
+struct TriHit64 {
+    double t, a, b, c, det;
+};
+
+__device__ __forceinline__ double dop64(double a, double b, double c, double d) {
+    const double cd   = c * d;
+    const double corr = __builtin_fma(-c, d, cd);
+    return __builtin_fma(a, b, -cd) + corr;
+}
+
+__device__ __forceinline__ double comp64(double x, double y, double z, int i) { return i == 0 ? x : (i == 1 ? y : z); }
+
+__device__ __forceinline__ bool tri_test64_body(const RayPre& r, F3 v0, F3 v1, F3 v2, double tmin, double tmax, TriHit64& out) {
+    const int    kx = r.kx(), ky = r.ky(), kz = r.kz();
+    const double dkz = comp(r.d, kz);
+    const double sx = double(comp(r.d, kx)) / dkz, sy = double(comp(r.d, ky)) / dkz, sz = 1.0 / dkz;
+    double p[3][3];
+    const F3 v[3] = {v0, v1, v2};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const double tx = double(v[i].x) - double(r.o.x), ty = double(v[i].y) - double(r.o.y), tz = double(v[i].z) - double(r.o.z);
+        const double c  = comp64(tx, ty, tz, kz);
+        p[i][0] = comp64(tx, ty, tz, kx) - sx * c;
+        p[i][1] = comp64(tx, ty, tz, ky) - sy * c;
+        p[i][2] = c * sz;
+    }
+    const double a = dop64(p[1][0], p[2][1], p[1][1], p[2][0]);
+    const double b = dop64(p[2][0], p[0][1], p[2][1], p[0][0]);
+    const double c = dop64(p[0][0], p[1][1], p[0][1], p[1][0]);
+    if ((a < 0.0 || b < 0.0 || c < 0.0) && (a > 0.0 || b > 0.0 || c > 0.0)) return false;
+    const double det = (a + b) + c;
+    if (det == 0.0) return false;
+    const double t = ((a * p[0][2] + b * p[1][2]) + c * p[2][2]) / det;
+    if (t < tmin || t > tmax) return false;
+    // gamma_bound(n) = n (eps / 2) / (1 - n (eps / 2)) with the double epsilon
+    constexpr double e2 = 2.0 * 0x1p-53, e3 = 3.0 * 0x1p-53, e5 = 5.0 * 0x1p-53;
+    constexpr double g2 = e2 / (1.0 - e2), g3 = e3 / (1.0 - e3), g5 = e5 / (1.0 - e5);
+    const double x    = __builtin_fmax(__builtin_fmax(__builtin_fabs(p[0][0]), __builtin_fabs(p[1][0])), __builtin_fabs(p[2][0]));
+    const double y    = __builtin_fmax(__builtin_fmax(__builtin_fabs(p[0][1]), __builtin_fabs(p[1][1])), __builtin_fabs(p[2][1]));
+    const double z    = __builtin_fmax(__builtin_fmax(__builtin_fabs(p[0][2]), __builtin_fabs(p[1][2])), __builtin_fabs(p[2][2]));
+    const double ex   = g5 * (x + z);
+    const double ey   = g5 * (y + z);
+    const double ee   = 2.0 * ((g2 * x * y + ey * x) + ex * y);
+    const double edge = __builtin_fmax(__builtin_fmax(__builtin_fabs(a), __builtin_fabs(b)), __builtin_fabs(c));
+    const double et   = 3.0 * ((g3 * edge * z + ee * z) + g3 * z * edge) / __builtin_fabs(det);
+    if (t <= et) return false;
+    out.t   = t;
+    out.a   = a;
+    out.b   = b;
+    out.c   = c;
+    out.det = det;
+    return true;
+}
+
+// Out of line, with its inputs by value: the call keeps the FP64 temporaries out of the register budget
+// of the traversal loop (inlined, they raise it by about 30 VGPRs); it runs only for the triangles the
+// float pre-test cannot reject, about one or two per ray.
+__device__ __attribute__((noinline)) TriHit64 tri_test64_call(float ox, float oy, float oz, float dx, float dy, float dz, uint32_t axes,
+                                                              float v0x, float v0y, float v0z, float v1x, float v1y, float v1z,
+                                                              float v2x, float v2y, float v2z, double tmin, double tmax) {
+    RayPre r;
+    r.o    = mk(ox, oy, oz);
+    r.d    = mk(dx, dy, dz);
+    r.axes = axes;
+    TriHit64 out;
+    if (!tri_test64_body(r, mk(v0x, v0y, v0z), mk(v1x, v1y, v1z), mk(v2x, v2y, v2z), tmin, tmax, out)) out.det = 0.0;  // det == 0: a miss
+    return out;
+}
+
+__device__ __forceinline__ bool tri_test64(const RayPre& r, F3 v0, F3 v1, F3 v2, double tmin, double tmax, TriHit64& out) {
+    out = tri_test64_call(r.o.x, r.o.y, r.o.z, r.d.x, r.d.y, r.d.z, r.axes, v0.x, v0.y, v0.z, v1.x, v1.y, v1.z, v2.x, v2.y, v2.z, tmin,
+                          tmax);
+    return out.det != 0.0;
+}
+
+// Single-sided culling of the FP64 traversal (bvh.cpp:259-271): true when the ray reaches the front
+// of the triangle (dot(d, outward) < 0), outward = (v1 - v0) x (v2 - v0), negated for a mirrored instance.
+__device__ __forceinline__ bool front_side64(const RayPre& r, F3 v0, F3 v1, F3 v2, bool mirrored) {
+    const double e1x = double(v1.x) - double(v0.x), e1y = double(v1.y) - double(v0.y), e1z = double(v1.z) - double(v0.z);
+    const double e2x = double(v2.x) - double(v0.x), e2y = double(v2.y) - double(v0.y), e2z = double(v2.z) - double(v0.z);
+    const double s   = mirrored ? -1.0 : 1.0;
+    const double ox = (e1y * e2z - e2y * e1z) * s, oy = (e1z * e2x - e2z * e1x) * s, oz = (e1x * e2y - e2x * e1y) * s;
+    return ((double(r.d.x) * ox + double(r.d.y) * oy) + double(r.d.z) * oz) < 0.0;
+}
+
+// Float pre-test of the FP64 test: true only when the FP64 edge functions provably have mixed signs
+// (a miss). The sheared float coordinates of corner i differ from the FP64 ones by at most about
+// 4 u M_i (u = 2^-24, M_i = max(|t_kx|, |t_ky|) + max(|sx|, |sy|) |t_kz| bounds them), so a float edge
+// function differs from the FP64 one by at most about 19 u M_i M_j; an edge function beyond
+// 2^-17 M_i M_j (128 u) has the FP64 one's sign. Outside [2^-100, 2^100] for M_i M_j (float
+// underflow or overflow) the pre-test gives no verdict.
+__device__ __forceinline__ void permute_bound(const RayPre& r, F3 v, float& px, float& py, float& m) {
+    const F3    t  = v - r.o;
+    const float tx = comp(t, r.kx()), ty = comp(t, r.ky()), tz = comp(t, r.kz());
+    px = tx - r.sx * tz;
+    py = ty - r.sy * tz;
+    m  = __builtin_fmaxf(__builtin_fabsf(tx), __builtin_fabsf(ty)) +
+        __builtin_fmaxf(__builtin_fabsf(r.sx), __builtin_fabsf(r.sy)) * __builtin_fabsf(tz);
+}
+
+__device__ __forceinline__ bool in_bound_range(float x) { return x >= 0x1p-100f && x <= 0x1p100f; }
+
+__device__ __forceinline__ bool tri_clear_miss32(const RayPre& r, F3 v0, F3 v1, F3 v2) {
+    float p0x, p0y, m0, p1x, p1y, m1, p2x, p2y, m2;
+    permute_bound(r, v0, p0x, p0y, m0);
+    permute_bound(r, v1, p1x, p1y, m1);
+    permute_bound(r, v2, p2x, p2y, m2);
+    const float m12 = m1 * m2, m20 = m2 * m0, m01 = m0 * m1;
+    if (!(in_bound_range(m12) && in_bound_range(m20) && in_bound_range(m01))) return false;
+    const float a = dop(p1x, p2y, p1y, p2x), ea = 0x1p-17f * m12;
+    const float b = dop(p2x, p0y, p2y, p0x), eb = 0x1p-17f * m20;
+    const float c = dop(p0x, p1y, p0y, p1x), ec = 0x1p-17f * m01;
+    return (a < -ea || b < -eb || c < -ec) && (a > ea || b > eb || c > ec);
+}
+
+// Smallest float >= x (x >= 0): the float cull bound of an FP64 distance.
+__device__ __forceinline__ float float_at_least(double x) {
+    const float f = static_cast<float>(x);
+    return static_cast<double>(f) < x ? __uint_as_float(__float_as_uint(f) + 1u) : f;
+}
+
 // Triangle record flags (GpuTri::flags): bit 0 single-sided, bit 1 mirrored instance.
 constexpr uint32_t kTriSingleSided = 1u;
 constexpr uint32_t kTriMirrored    = 2u;
@@ -251,6 +379,7 @@ struct TraceState {
     uint32_t    ref;
     uint32_t    leaf;  // parked leaf, kEmptyRef when none (kEmptyRef is never a leaf of a scene below 2^27 triangles)
     uint32_t    sp;
+    double      best64;  // FP64 hit decisions (EXACT): the best hit's FP64 distance; res.t is the float at least it
 };
 
 // `skip`: for an any-hit ray, a scene-order primitive to ignore (the emitter a shadow segment ends
@@ -266,6 +395,7 @@ __device__ __forceinline__ void trace_init(TraceState& st, F3 o, F3 d, float tmi
     st.ref  = 0;  // the root is always an inner node
     st.leaf = kEmptyRef;
     st.sp   = 0;
+    st.best64 = tmax;
 }
 
 // Leaf batching of the step driver: a lane that reaches a leaf parks it (one parked leaf per lane)
@@ -296,8 +426,9 @@ struct KeepInResult {
 // has an inner node to visit. Returns true once the ray is finished: closest hit (any = false) or any
 // hit (any = true) in [tmin, tmax], result in st.res; a closest hit's edge functions go to `keep`
 // undivided (with KeepInResult the caller applies finish_barycentrics where it needs them). `any` may differ between the lanes of a
-// wave.
-template <int LEAF_BATCH, bool STATS, class Keep>
+// wave. EXACT: hit decisions in FP64 (tri_test64; st.best64 holds the best distance, `keep` is not
+// called: the caller recomputes the winner's FP64 hit where it needs its barycentrics).
+template <int LEAF_BATCH, bool STATS, bool EXACT, class Keep>
 __device__ __forceinline__ bool trace_step_dyn(const TraceScene& sc, TraceState& st, const StackView& stk, uint64_t& bounds_tests,
                                                uint64_t& tri_tests, const bool any, const Keep& keep) {
     if (sc.node_count == 0) return true;
@@ -319,8 +450,25 @@ __device__ __forceinline__ bool trace_step_dyn(const TraceScene& sc, TraceState&
                 const float4   b    = gld(tp + 1);
                 const float4   c    = gld(tp + 2);
                 if (STATS) tri_tests += 1;
-                TriHit h;
                 const F3 v0 = mk(a.x, a.y, a.z), v1 = mk(b.x, b.y, b.z), v2 = mk(c.x, c.y, c.z);
+                if (EXACT) {
+                    if (tri_clear_miss32(r, v0, v1, v2)) continue;
+                    TriHit64 h;
+                    if (!tri_test64(r, v0, v1, v2, st.tmin, st.best64, h)) continue;
+                    const uint32_t flags = __float_as_uint(b.w);
+                    if ((flags & kTriSingleSided) && !front_side64(r, v0, v1, v2, (flags & kTriMirrored) != 0u)) continue;
+                    const uint32_t prim = __float_as_uint(a.w);
+                    if (any && prim == st.res.prim) continue;
+                    if (st.res.slot == 0xffffffffu || h.t < st.best64 || (h.t == st.best64 && prim < st.res.prim)) {
+                        st.best64   = h.t;
+                        st.res.t    = float_at_least(h.t);
+                        st.res.slot = slot;
+                        st.res.prim = prim;
+                        if (any) return true;
+                    }
+                    continue;
+                }
+                TriHit h;
                 if (!tri_test(r, v0, v1, v2, st.tmin, st.res.t, h)) continue;
                 const uint32_t flags = __float_as_uint(b.w);
                 if (flags & kTriSingleSided) {

@@ -48,14 +48,15 @@ def _samples(count=1000):
     return [(float(s.next_1d()), float(s.next_1d())) for _ in range(count)]
 
 
-def test_emitter_segments_through_mrg_occluded_match_the_twin(device, oracle):
-    """The FP32 twin's shadow segments of the reference's emitter case, traced by mrg_occluded (the plain
-    any-hit query, which skips nothing): the GPU reports the same false endpoint occlusions as the twin's
-    any-hit query, segment for segment (37 of 1000: the reason the render path's shadow rays skip their
-    emitter triangle)."""
+def test_emitter_segments_through_mrg_occluded_match_the_twin(oracle):
+    """The FP32 twin's shadow segments of the reference's emitter case, traced by mrg_occluded with the
+    render kernels' FP32 hit decisions (query_precision 32; the plain any-hit query, which skips nothing):
+    the GPU reports the same false endpoint occlusions as the twin's any-hit query, segment for segment
+    (37 of 1000: the reason the render path's shadow rays skip their emitter triangle)."""
     tri, mat = _emitter()
     scene = concat([(tri, TRI_N)], [mat], environment=(0, 0, 0))
     o = oracle.OracleScene(scene)
+    device = api.Device(0, query_precision=32)
     bvh = device.upload(scene)
     origin = np.array([0.5, 0.25, 4.0])
     gpu, twin = [], []
@@ -67,6 +68,7 @@ def test_emitter_segments_through_mrg_occluded_match_the_twin(device, oracle):
         twin.append(bool(o.occluded(ray.astype(np.float64), 0.0, seg["limit"], precision=32)[0][0]))
         assert not seg["occluded"]  # the render path's segment (emitter skipped)
     bvh.release()
+    device.close()
     print(f"  false occlusions without the skip: gpu={sum(gpu)} twin={sum(twin)}", flush=True)
     assert gpu == twin and sum(twin) == 37
 

@@ -13,11 +13,20 @@ G = os.path.join(os.path.dirname(__file__), "golden")
 def test_hits_match_fixture(device):
     f = np.load(os.path.join(G, "icosphere5_hits.npz"))
     print("[golden] icosphere5 4096 rays", flush=True)
-    hits = device.upload(scenegen.icosphere(level=5)).intersect(f["rays"])
+    scene = scenegen.icosphere(level=5)
+    # FP64 hit decisions (the default): the FP64 fixture's ids and distances, near-ties included
+    hits = device.upload(scene).intersect(f["rays"])
+    np.testing.assert_array_equal(hits["primitive"], f["prim64"])
+    hit = f["prim64"] >= 0
+    np.testing.assert_array_equal(hits["t"][hit], f["t64"][hit].astype(np.float32))
+    # FP32 decisions: the FP32 twin's fixture; ids equal FP64 outside near-ties
+    with api.Device(0, query_precision=32) as d32:
+        hits = d32.upload(scene).intersect(f["rays"])
     np.testing.assert_array_equal(hits["primitive"], f["prim32"])
     hit = f["prim32"] >= 0
     np.testing.assert_array_equal(hits["t"][hit], f["t32"][hit].astype(np.float32))
-    near = np.abs(f["t2_64"] - f["t64"]) <= 1e-6 * np.abs(f["t64"])
+    with np.errstate(invalid="ignore"):
+        near = np.abs(f["t2_64"] - f["t64"]) <= 1e-6 * np.abs(f["t64"])
     assert np.all((hits["primitive"] == f["prim64"]) | near)
 
 
