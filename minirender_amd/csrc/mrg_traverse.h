@@ -15,7 +15,8 @@
 // a triangle inside it; closest-hit results therefore do not depend on the tree (nearest t, ties to
 // the lower scene-order index; bvh.cpp:273-276).
 // Triangle test: the FP32 form of RayQuery::intersect (intersection.cpp:108-152), operation for
-// operation.
+// operation, in the render kernels; the ray-query kernels decide hits with its FP64 form
+// (tri_test64) after a float pre-test that only discards provable FP64 misses.
 #pragma once
 
 #include "mrg_layout.h"
