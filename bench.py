@@ -339,7 +339,8 @@ def bench_intersect(args):
     n = 1 << 24
     log("[bench] intersect: generating 16M rays")
     rays = scenegen.intersection_rays(scene, n)
-    dev = api.Device(0, persistent_blocks_per_cu=args.blocks_per_cu, query_precision=args.query_precision)
+    dev = api.Device(0, persistent_blocks_per_cu=args.blocks_per_cu, query_precision=args.query_precision,
+                     path_capacity=1 << 16)  # ray queries only: a small render pool
     bvh = dev.upload(scene)
     hip = ctypes.CDLL("libamdhip64.so")
     d_rays, d_hits = ctypes.c_void_p(), ctypes.c_void_p()
@@ -398,7 +399,8 @@ def bench_intersect(args):
                             "max_t_rel": float(rel.max(initial=0)),
                             "t_equal_to_fp64_rounded": int((hits["t"][idx][hit64] == ref["t"][hit64].astype(np.float32)).sum()),
                             "hits": int(hit64.sum())}
-        result["roofline"] = roofline_entry("k_query", "intersect", avg / 1e3, n, 0, b, tt, 0, 0)
+        kernel = "k_query_exact" if args.query_precision == 64 else "k_query"
+        result["roofline"] = roofline_entry(kernel, "intersect", avg / 1e3, n, 0, b, tt, 0, 0)
         result["cpu_baseline"] = {"value": round(len(idx) / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads,
                                   "kind": "port", "host": host,
                                   "sample": (f"all {len(idx)} rays" if args.cpu_stride == 1 else

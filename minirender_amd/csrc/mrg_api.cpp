@@ -279,7 +279,11 @@ struct mrg_device {
 
 namespace {
 
-uint64_t budget(const mrg_device* d) { return d->opt.hbm_budget_bytes ? d->opt.hbm_budget_bytes : (uint64_t(64) << 30); }
+// An explicit budget covers path pools, scenes and films; the default gives scenes and films 64 GiB
+// on top of the path pools (the pools are allocated once, at device open).
+uint64_t budget(const mrg_device* d) {
+    return d->opt.hbm_budget_bytes ? d->opt.hbm_budget_bytes : (uint64_t(64) << 30) + d->pool_bytes;
+}
 
 void alloc_track(mrg_device* d, Track& t) {
     const uint64_t P = d->opt.path_capacity;
@@ -1313,16 +1317,19 @@ int mrg_device_open(int ordinal, const mrg_device_options* options, mrg_device**
         const uint32_t per_q  = d->opt.persistent_blocks_per_cu ? d->opt.persistent_blocks_per_cu : 6u;  // k_query: six
         d->shape              = LaunchShape{cus, cus * per, cus * 8, cus * per_rf, cus * per_q};
         // Automatic pool: 2^28 paths (47 GB of path state, within the default 64 GiB budget) on a
-        // device of at least 192 GiB, 2^26 (12 GB) from 64 GiB, else 2^23. Fewer, fuller batches pay
-        // each batch's latency-bound tail of late bounces (a few ms, whatever the batch size) less
-        // often. Measured on the 1080p terrain job (530.8M samples): 2^23 -> 2^26 paths +30%
-        // (per-ray traversal); with the refill traversal 2^26: 2,303, 2^27: 2,486, 2^28: 2,602
-        // Msamples/s. Latency-oriented callers (progressive serving) pass a smaller pool.
+        // device of at least 192 GiB, 2^29 (94.5 GB) from 256 GiB with one batch in flight, 2^26
+        // (12 GB) from 64 GiB, else 2^23. Fewer, fuller batches pay each batch's latency-bound tail
+        // of late bounces (a few ms, whatever the batch size) and each launch's drain less often.
+        // Measured on the 1080p terrain job (530.8M samples): 2^23 -> 2^26 paths +30% (per-ray
+        // traversal); with the refill traversal 2^26: 2,303, 2^27: 2,486, 2^28: 2,602 Msamples/s
+        // (earlier kernels); 2^27: 2,735, 2^28: 2,917, 2^29 (the job in one batch): 2,982.
+        // Latency-oriented callers (progressive serving) pass a smaller pool.
         if (d->opt.path_capacity == 0) {
             const uint64_t mem = d->props.totalGlobalMem;
-            d->opt.path_capacity = mem >= (uint64_t(192) << 30)  ? uint64_t(1) << 28
-                                   : mem >= (uint64_t(64) << 30) ? uint64_t(1) << 26
-                                                                 : uint64_t(1) << 23;
+            d->opt.path_capacity = mem >= (uint64_t(256) << 30) && d->opt.batches_in_flight != 2 ? uint64_t(1) << 29
+                                   : mem >= (uint64_t(192) << 30)                                ? uint64_t(1) << 28
+                                   : mem >= (uint64_t(64) << 30)                                 ? uint64_t(1) << 26
+                                                                                                 : uint64_t(1) << 23;
         }
         if (d->opt.path_capacity > (uint64_t(1) << 30)) throw std::invalid_argument("path capacity too large");
         if (d->opt.traversal > MRG_TRAVERSAL_REFILL) throw std::invalid_argument("unknown traversal kernel mode");

@@ -56,7 +56,7 @@ def test_emitter_segments_through_mrg_occluded_match_the_twin(oracle):
     tri, mat = _emitter()
     scene = concat([(tri, TRI_N)], [mat], environment=(0, 0, 0))
     o = oracle.OracleScene(scene)
-    device = api.Device(0, query_precision=32)
+    device = api.Device(0, query_precision=32, path_capacity=1 << 16)  # queries only: no render pool
     bvh = device.upload(scene)
     origin = np.array([0.5, 0.25, 4.0])
     gpu, twin = [], []
@@ -107,7 +107,8 @@ def headline():
 
 def test_headline_scene_in_its_benchmark_shape_matches_the_twin(headline):
     """1920 x 1080 x 256 spp, depth 16, default device options (the benchmark's job): rows 516-519 against
-    the FP32 twin within 1e-4; the kernel times show the refill traversal in two 2^28-path batches."""
+    the FP32 twin within 1e-4; the kernel times show the refill traversal in the automatic pool (the whole
+    job in one 2^29-path batch on a 288 GB device)."""
     scene, o = headline
     w, h, spp, depth, rows = 1920, 1080, 256, 16, (516, 520)
     print(f"[config 3] {w}x{h} spp={spp} depth={depth}, default options, rows {rows}", flush=True)

@@ -20,7 +20,7 @@ def test_hits_match_fixture(device):
     hit = f["prim64"] >= 0
     np.testing.assert_array_equal(hits["t"][hit], f["t64"][hit].astype(np.float32))
     # FP32 decisions: the FP32 twin's fixture; ids equal FP64 outside near-ties
-    with api.Device(0, query_precision=32) as d32:
+    with api.Device(0, query_precision=32, path_capacity=1 << 16) as d32:
         hits = d32.upload(scene).intersect(f["rays"])
     np.testing.assert_array_equal(hits["primitive"], f["prim32"])
     hit = f["prim32"] >= 0

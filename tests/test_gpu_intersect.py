@@ -25,7 +25,7 @@ SHARED_EDGE_RATE = 4e-6  # bound on id mismatches outside near-ties (all shared-
 
 @pytest.fixture(scope="module")
 def device32():
-    d = api.Device(0, query_precision=32)
+    d = api.Device(0, query_precision=32, path_capacity=1 << 16)  # queries only: no render pool
     yield d
     d.close()
 

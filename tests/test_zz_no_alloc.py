@@ -46,7 +46,7 @@ def _same(bvh, rays, tmin=0.0, tmax=np.inf, stats=True):
 
 def test_noalloc_query_equals_the_allocating_query():
     scene = scenegen.icosphere(level=6, seed=3)
-    with api.Device(0) as dev:
+    with api.Device(0, path_capacity=1 << 16) as dev:  # queries only: a small render pool
         bvh = dev.upload(scene)
         print("[noalloc] 262144 rays, then 4096 rays, then 1048576 rays", flush=True)
         big = scenegen.intersection_rays(scene, 1 << 18, seed=11)
