@@ -997,7 +997,7 @@ void enqueue_batch(mrg_device* d, Track& t, hipEvent_t before) {
         const uint32_t w = b.gjobs[c.job].width;
         c.tile_rows      = c.pixel_begin % w == 0 && c.pixel_count % w == 0 ? c.pixel_count / w : 0u;
     }
-    const uint32_t hdr[kHdrWords] = {uint32_t(b.chunks.size()), b.slots, b.pixels, t.iters};
+    const uint32_t hdr[kHdrWords] = {uint32_t(b.chunks.size()), b.slots, b.pixels, t.iters, b.gjobs.size() == 1 ? 1u : 0u, 0u, 0u, 0u};
     std::memcpy(t.up_jobs.ptr, b.gjobs.data(), b.gjobs.size() * sizeof(GpuJob));
     std::memcpy(t.up_chunks.ptr, b.chunks.data(), b.chunks.size() * sizeof(GpuChunk));
     std::memcpy(t.up_header.ptr, hdr, sizeof hdr);

@@ -461,6 +461,7 @@ __device__ __forceinline__ uint32_t find_chunk_by_pixel(const GpuChunk* chunks, 
 __global__ void __launch_bounds__(256) k_generate(Pool P, const GpuJob* __restrict__ jobs, const GpuChunk* __restrict__ chunks) {
     const uint32_t chunk_count = P.header[kHdrChunks];
     const uint32_t total       = P.header[kHdrSlots];
+    const bool     one_job     = P.header[kHdrOneJob] != 0u;
     for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < total; slot += gridDim.x * blockDim.x) {
         const uint32_t  ci = find_chunk_by_slot(chunks, chunk_count, slot);
         const GpuChunk  ch = chunks[ci];
@@ -493,8 +494,9 @@ __global__ void __launch_bounds__(256) k_generate(Pool P, const GpuJob* __restri
         const float cosine = dot(d, fwd);
         P.ray_o[slot] = make_float4(o.x, o.y, o.z, cam.near_plane / cosine);
         P.ray_d[slot] = make_float4(d.x, d.y, d.z, cam.far_plane / cosine);
-        // beta = 1, pdf 0, radiance 0 and eta scale 1 are implied for bounce 0 (k_shade)
-        P.prev[slot]  = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(kPrevDelta | (ch.job << 16)));
+        // beta = 1, pdf 0, radiance 0 and eta scale 1 are implied for bounce 0 (k_shade), and so is the
+        // previous-event record of a one-job batch (job 0, a camera path's flags)
+        if (!one_job) P.prev[slot] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(kPrevDelta | (ch.job << 16)));
         P.rng[slot]   = make_uint4(static_cast<uint32_t>(rng.state), static_cast<uint32_t>(rng.state >> 32),
                                    static_cast<uint32_t>(rng.inc), static_cast<uint32_t>(rng.inc >> 32));
     }
@@ -509,14 +511,15 @@ __device__ __forceinline__ const uint32_t* ext_queue(const Pool& P, uint32_t ite
 // One extension ray: closest hit of path queue[i] (path i at bounce 0, whose queue is the identity:
 // a null queue). ONE_SCENE: every job of the batch renders
 // jobs[0]'s scene, read through a uniform index (scalar loads) without the path's job lookup.
+// job0: the path's job is 0 without a record to read (bounce 0 of a one-job batch).
 template <bool ONE_SCENE>
 __device__ __forceinline__ void extend_one(const Pool& P, const GpuJob* __restrict__ jobs, const uint32_t* queue, uint32_t i,
-                                           const StackView& stk) {
+                                           const StackView& stk, bool job0) {
     uint64_t       dummy = 0;
     const uint32_t path  = queue ? queue[i] : i;
     const float4   ro    = P.ray_o[path];
     const float4   rd    = P.ray_d[path];
-    const uint32_t job   = ONE_SCENE ? 0u : __float_as_uint(P.prev[path].w) >> 16;
+    const uint32_t job   = ONE_SCENE || job0 ? 0u : __float_as_uint(P.prev[path].w) >> 16;
     const RayPre   rp    = ray_setup(mk(ro.x, ro.y, ro.z), mk(rd.x, rd.y, rd.z));
     const TraceResult res = trace_ray<false, false>(jobs[job].scene, rp, ro.w, rd.w, stk, &dummy, &dummy);
     P.hit[path]      = make_float4(res.b0, res.b1, res.b2, 1.0f);  // already divided: their sum stands in as 1
@@ -551,10 +554,11 @@ __global__ void __launch_bounds__(256) k_extend(Pool P, const GpuJob* __restrict
     const uint32_t  n     = P.counters[kCtrExt + iter];
     const uint32_t* queue = ext_queue(P, iter);
     uint32_t*       work  = P.counters + kCtrWorkExt + iter;
+    const bool      job0  = iter == 0 && P.header[kHdrOneJob] != 0u;
     WaveWork<kExtendClaim> w;
     for (w.begin(); w.base < n; w.next(work, n)) {
         const uint32_t i = w.base + lane_id();
-        if (i < n) extend_one<false>(P, jobs, queue, i, stk);
+        if (i < n) extend_one<false>(P, jobs, queue, i, stk, job0);
     }
 }
 
@@ -574,6 +578,7 @@ __global__ void __launch_bounds__(256) k_trace(Pool P, const GpuJob* __restrict_
     const uint32_t  n      = sh_end + n_ext;
     const uint32_t* queue  = ext_queue(P, iter);
     uint32_t*       work   = P.counters + kCtrWorkExt + iter;
+    const bool      job0   = iter == 0 && P.header[kHdrOneJob] != 0u;
     WaveWork<kTraceClaim> w;
     for (w.begin(); w.base < n; w.next(work, n)) {
         if (w.base < sh_end) {
@@ -581,7 +586,7 @@ __global__ void __launch_bounds__(256) k_trace(Pool P, const GpuJob* __restrict_
             if (i < n_sh) shadow_one<ONE_SCENE>(P, jobs, i, stk);
         } else {
             const uint32_t i = w.base - sh_end + lane_id();
-            if (i < n_ext) extend_one<ONE_SCENE>(P, jobs, queue, i, stk);
+            if (i < n_ext) extend_one<ONE_SCENE>(P, jobs, queue, i, stk, job0);
         }
     }
 }
@@ -590,9 +595,9 @@ __global__ void __launch_bounds__(256) k_trace(Pool P, const GpuJob* __restrict_
 // bounce iter - 1 first, then the extension rays of bounce iter), but lanes that finish their ray
 // take the next queue entry at once, so a wave can trace shadow and extension rays side by side.
 // ONE_SCENE: every job of the batch renders the same scene (jobs[0]'s), read once per wave.
-// Seven waves per SIMD (at most 72 VGPRs, 22 KB of LDS stack per block): seven resident blocks per CU.
-// The kernel is bound by latency, not by VALU issue (the SIMDs issue two VALU instructions in a quarter
-// of their issue slots), so a seventh wave pays: 137.3 -> 131.3 ms of traversal per 1080p job.
+// Seven waves per SIMD (at most 72 VGPRs, 22 KB of LDS stack per block): seven resident blocks per CU;
+// the seventh wave hides more of the dependent node-fetch latency: 137.3 -> 131.3 ms of traversal per
+// 1080p job.
 template <bool ONE_SCENE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) k_trace_refill(Pool P, const GpuJob* __restrict__ jobs, uint32_t iter) {
     __shared__ uint32_t s_ref[kStackLds * kTraversalBlock];
@@ -604,6 +609,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) k
     uint32_t*       work  = P.counters + kCtrWorkExt + iter;
     uint64_t        nb = 0, nt = 0;
     uint32_t        path = 0;
+    const bool      job0 = iter == 0 && P.header[kHdrOneJob] != 0u;
     const TraceScene one = ONE_SCENE ? trace_scene(jobs[0].scene) : TraceScene{nullptr, nullptr, 0};
     persistent_trace_mixed<kBounceRefillLanes, kBounceLeafBatch, false, ONE_SCENE, false>(
         stk, n_sh + n_ext, work, one,
@@ -621,7 +627,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) k
                 trace_init(st, mk(ro.x, ro.y, ro.z), mk(rd.x, rd.y, rd.z), ro.w, rd.w);
             }
             if (ONE_SCENE) return one;
-            const uint32_t job = __float_as_uint(P.prev[path].w) >> 16;
+            const uint32_t job = job0 ? 0u : __float_as_uint(P.prev[path].w) >> 16;
             return trace_scene(jobs[job].scene);
         },
         [&](uint32_t i, const TraceState& st) {
@@ -875,6 +881,7 @@ __global__ void __launch_bounds__(kShadeThreads) __attribute__((amdgpu_waves_per
     uint32_t*       next_n  = P.counters + kCtrExt + iter + 1;
     uint32_t*       sh_n    = P.counters + kCtrShadow + iter;
     const uint32_t  stride  = gridDim.x * blockDim.x;
+    const bool      job0    = iter == 0 && P.header[kHdrOneJob] != 0u;  // bounce 0 of a one-job batch: no prev record
     uint32_t        n_scatter = 0, n_rr = 0, n_shadow = 0;
     __shared__ uint32_t s_off[2 * (kShadeThreads / 64)];
     __shared__ uint32_t s_base[2];
@@ -889,7 +896,7 @@ __global__ void __launch_bounds__(kShadeThreads) __attribute__((amdgpu_waves_per
             path = queue ? queue[i] : i;
             // everything that depends on the path slot alone is in flight before the job is known
             ShadeIn in;
-            in.pv   = P.prev[path];
+            in.pv   = job0 ? make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(kPrevDelta)) : P.prev[path];
             in.rd   = P.ray_d[path];
             in.hv   = P.hit[path];
             in.slot = P.hit_slot[path];
