@@ -189,7 +189,8 @@ typedef struct mrg_device_options {
                                     scenes and films on top of the path pools */
     uint64_t path_capacity;      /* paths in flight per batch; 0 = automatic (2^29 with >= 256 GiB HBM and
                                     one batch in flight, 2^28 with >= 192 GiB, 2^26 with >= 64 GiB,
-                                    else 2^23); 176 B of HBM per path */
+                                    else 2^23; halved while the pools would take more than half of the
+                                    memory free at open); 176 B of HBM per path */
     uint32_t schedule;           /* MRG_SCHEDULE_PLAIN, _BATCHED or _PLAIN_FILL */
     uint32_t max_jobs_in_flight; /* batched: jobs co-scheduled per batch; 0 = 64 */
     uint32_t use_graph;          /* capture the bounce loop in a hipGraph */

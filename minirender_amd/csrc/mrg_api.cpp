@@ -285,10 +285,12 @@ uint64_t budget(const mrg_device* d) {
     return d->opt.hbm_budget_bytes ? d->opt.hbm_budget_bytes : (uint64_t(64) << 30) + d->pool_bytes;
 }
 
+// Pool bytes per path slot: ray_o, ray_d, hit, beta, rad, prev, rng, sh_o, sh_d, sh_c (16 B each) + 4 u32.
+constexpr uint64_t kPathSlotBytes = 10 * 16 + 4 * 4;
+
 void alloc_track(mrg_device* d, Track& t) {
-    const uint64_t P = d->opt.path_capacity;
-    // per slot: ray_o, ray_d, hit, beta, rad, prev, rng, sh_o, sh_d, sh_c (16 B each) + 4 u32
-    const uint64_t per = 10 * 16 + 4 * 4;
+    const uint64_t P   = d->opt.path_capacity;
+    const uint64_t per = kPathSlotBytes;
     t.pool_mem.alloc(P * per);
     char* base = t.pool_mem.as<char>();
     auto take  = [&](uint64_t bytes) {
@@ -1324,12 +1326,19 @@ int mrg_device_open(int ordinal, const mrg_device_options* options, mrg_device**
         // traversal); with the refill traversal 2^26: 2,303, 2^27: 2,486, 2^28: 2,602 Msamples/s
         // (earlier kernels); 2^27: 2,735, 2^28: 2,917, 2^29 (the job in one batch): 2,982.
         // Latency-oriented callers (progressive serving) pass a smaller pool.
+        // The automatic pools (one per batch in flight) also keep within half of the memory free now,
+        // so that a second device handle in the process, or another process, still fits.
         if (d->opt.path_capacity == 0) {
             const uint64_t mem = d->props.totalGlobalMem;
-            d->opt.path_capacity = mem >= (uint64_t(256) << 30) && d->opt.batches_in_flight != 2 ? uint64_t(1) << 29
-                                   : mem >= (uint64_t(192) << 30)                                ? uint64_t(1) << 28
-                                   : mem >= (uint64_t(64) << 30)                                 ? uint64_t(1) << 26
-                                                                                                 : uint64_t(1) << 23;
+            uint64_t       cap = mem >= (uint64_t(256) << 30) && d->opt.batches_in_flight != 2 ? uint64_t(1) << 29
+                                 : mem >= (uint64_t(192) << 30)                                ? uint64_t(1) << 28
+                                 : mem >= (uint64_t(64) << 30)                                 ? uint64_t(1) << 26
+                                                                                               : uint64_t(1) << 23;
+            size_t free_bytes = 0, total_bytes = 0;
+            check_hip(hipMemGetInfo(&free_bytes, &total_bytes), "hipMemGetInfo");
+            const uint64_t tracks = d->opt.batches_in_flight == 2 ? 2u : 1u;
+            while (cap > (uint64_t(1) << 20) && cap * kPathSlotBytes * tracks > free_bytes / 2) cap >>= 1;
+            d->opt.path_capacity = cap;
         }
         if (d->opt.path_capacity > (uint64_t(1) << 30)) throw std::invalid_argument("path capacity too large");
         if (d->opt.traversal > MRG_TRAVERSAL_REFILL) throw std::invalid_argument("unknown traversal kernel mode");
