@@ -258,11 +258,16 @@ __device__ __forceinline__ bool front_side64(const RayPre& r, F3 v0, F3 v1, F3 v
 }
 
 // Float pre-test of the FP64 test: true only when the FP64 edge functions provably have mixed signs
-// (a miss). The sheared float coordinates of corner i differ from the FP64 ones by at most about
-// 4 u M_i (u = 2^-24, M_i = max(|t_kx|, |t_ky|) + max(|sx|, |sy|) |t_kz| bounds them), so a float edge
-// function differs from the FP64 one by at most about 19 u M_i M_j; an edge function beyond
-// 2^-17 M_i M_j (128 u) has the FP64 one's sign. Outside [2^-100, 2^100] for M_i M_j (float
-// underflow or overflow) the pre-test gives no verdict.
+// (a miss). With -ffp-contract=off the sheared float coordinates of corner i differ from the exact
+// ones (exact arithmetic on the same float ray and vertices) by at most 4 u M_i (u = 2^-24,
+// M_i = max(|t_kx|, |t_ky|) + max(|sx|, |sy|) |t_kz| bounds every term they are made of); the
+// FP64 ones by 4 u' M_i (u' = 2^-53). An edge function is then off by at most
+// 4 u (M_j A_k + M_k A_j) + 96 u^2 M_j M_k (A_i = |px_i| + |py_i|), plus 2 u |e| for the rounding of
+// dop (Kahan's difference of products); each term below takes at least twice that, which also covers
+// the FP64 errors and the rounding of the bound itself. The sensitivity to a corner's error is the
+// other corner's distance A from the ray, not its distance M from the origin, so the bound stays
+// small for a small triangle near the ray far from the origin. For M_i outside [2^-40, 2^40] the
+// pre-test gives no verdict (no overflow, and float underflow stays far below every bound term).
 __device__ __forceinline__ void permute_bound(const RayPre& r, F3 v, float& px, float& py, float& m) {
     const F3    t  = v - r.o;
     const float tx = comp(t, r.kx()), ty = comp(t, r.ky()), tz = comp(t, r.kz());
@@ -272,18 +277,26 @@ __device__ __forceinline__ void permute_bound(const RayPre& r, F3 v, float& px, 
         __builtin_fmaxf(__builtin_fabsf(r.sx), __builtin_fabsf(r.sy)) * __builtin_fabsf(tz);
 }
 
-__device__ __forceinline__ bool in_bound_range(float x) { return x >= 0x1p-100f && x <= 0x1p100f; }
+__device__ __forceinline__ bool in_bound_range(float x) { return x >= 0x1p-40f && x <= 0x1p40f; }
+
+// Bound on |float edge - FP64 edge| for the edge function e of corners j, k (mj, mk pre-scaled by 2^-20).
+__device__ __forceinline__ float edge_bound(float e, float mj, float aj, float mk, float ak) {
+    return 0.5f * (mj * ak + mk * aj) + (0x1p-21f * __builtin_fabsf(e) + mj * mk);
+}
 
 __device__ __forceinline__ bool tri_clear_miss32(const RayPre& r, F3 v0, F3 v1, F3 v2) {
     float p0x, p0y, m0, p1x, p1y, m1, p2x, p2y, m2;
     permute_bound(r, v0, p0x, p0y, m0);
     permute_bound(r, v1, p1x, p1y, m1);
     permute_bound(r, v2, p2x, p2y, m2);
-    const float m12 = m1 * m2, m20 = m2 * m0, m01 = m0 * m1;
-    if (!(in_bound_range(m12) && in_bound_range(m20) && in_bound_range(m01))) return false;
-    const float a = dop(p1x, p2y, p1y, p2x), ea = 0x1p-17f * m12;
-    const float b = dop(p2x, p0y, p2y, p0x), eb = 0x1p-17f * m20;
-    const float c = dop(p0x, p1y, p0y, p1x), ec = 0x1p-17f * m01;
+    const float mlo = __builtin_fminf(__builtin_fminf(m0, m1), m2), mhi = __builtin_fmaxf(__builtin_fmaxf(m0, m1), m2);
+    if (!(in_bound_range(mlo) && in_bound_range(mhi))) return false;
+    const float a0 = __builtin_fabsf(p0x) + __builtin_fabsf(p0y), s0 = 0x1p-20f * m0;
+    const float a1 = __builtin_fabsf(p1x) + __builtin_fabsf(p1y), s1 = 0x1p-20f * m1;
+    const float a2 = __builtin_fabsf(p2x) + __builtin_fabsf(p2y), s2 = 0x1p-20f * m2;
+    const float a = dop(p1x, p2y, p1y, p2x), ea = edge_bound(a, s1, a1, s2, a2);
+    const float b = dop(p2x, p0y, p2y, p0x), eb = edge_bound(b, s2, a2, s0, a0);
+    const float c = dop(p0x, p1y, p0y, p1x), ec = edge_bound(c, s0, a0, s1, a1);
     return (a < -ea || b < -eb || c < -ec) && (a > ea || b > eb || c > ec);
 }
 

@@ -137,8 +137,10 @@ def test_shared_edges_resolved_in_fp64(device, device32, oracle):
         tris = np.stack([np.broadcast_to(centre, (k, 3)), ring[:-1], ring[1:]], 1).astype(np.float32)
         normals = np.tile(np.array([0, 0, 1], np.float32), (k, 3, 1))
         scene = concat([(tris, normals)], [material()])
-        # points on the spokes (shared edges) and near them, seen from above at random tilts
-        a = ang[rng.integers(0, k, 4096)] + rng.normal(0, 1e-7, 4096)
+        # points on the spokes (shared edges) and near them, seen from above at random tilts; the angular
+        # offsets are log-uniform over 1e-11..1e-3 so that, at every scale, some rays pass just inside
+        # and just outside the float pre-test's error band (about 2^-19 of the origin distance)
+        a = ang[rng.integers(0, k, 4096)] + rng.choice([-1.0, 1.0], 4096) * 10 ** rng.uniform(-11, -3, 4096)
         r = rng.uniform(0.05, 0.95, 4096) * scale
         target = np.stack([np.cos(a) * r, np.sin(a) * r, np.zeros(4096)], -1) + offset
         d = np.stack([rng.normal(0, 0.3, 4096), rng.normal(0, 0.3, 4096), -np.ones(4096)], -1)
