@@ -948,7 +948,11 @@ __global__ void __launch_bounds__(256) k_shadow(Pool P, const GpuJob* __restrict
 
 // Film validation of a batch (Film::commit checks the whole batch before publishing any of it,
 // render_session.cpp:39-51): nonfinite or negative sample radiance, or a pixel mean above FLT_MAX,
-// flags the job; k_accumulate then leaves that job's film untouched.
+// flags the job; k_accumulate then leaves that job's film untouched. The validation already forms
+// each pixel's new FP64 sums (film + samples in index order), so it stages them for k_accumulate in
+// the batch's ray arrays, which no kernel reads again before the next batch's k_generate rewrites
+// them: batch pixel g's sums go to ray_o[g] (x, y) and ray_d[g] (z) as doubles (a batch has at most
+// one pixel per path slot, so g < capacity).
 __global__ void __launch_bounds__(256) k_validate(Pool P, const GpuJob* __restrict__ jobs, const GpuChunk* __restrict__ chunks) {
     const uint32_t chunk_count  = P.header[kHdrChunks];
     const uint32_t total_pixels = P.header[kHdrPixels];
@@ -977,6 +981,8 @@ __global__ void __launch_bounds__(256) k_validate(Pool P, const GpuJob* __restri
         const double fm    = 3.4028234663852886e38;
         if (!(sx / count <= fm && sy / count <= fm && sz / count <= fm)) bad |= 2u;
         if (bad) atomicOr(job.error_flag, bad);
+        reinterpret_cast<double2*>(P.ray_o)[g] = make_double2(sx, sy);
+        reinterpret_cast<double2*>(P.ray_d)[g] = make_double2(sz, 0.0);
     }
 }
 
@@ -993,7 +999,8 @@ __global__ void k_commit_gate(const GpuJob* __restrict__ jobs, uint32_t job_coun
     }
 }
 
-// Film commit: per pixel, samples in index order, FP64 sums (render_session.cpp:39-74).
+// Film commit: per pixel, samples in index order, FP64 sums (render_session.cpp:39-74), as k_validate
+// formed and staged them.
 __global__ void __launch_bounds__(256) k_accumulate(Pool P, const GpuJob* __restrict__ jobs, const GpuChunk* __restrict__ chunks) {
     const uint32_t chunk_count  = P.header[kHdrChunks];
     const uint32_t total_pixels = P.header[kHdrPixels];
@@ -1006,18 +1013,11 @@ __global__ void __launch_bounds__(256) k_accumulate(Pool P, const GpuJob* __rest
         uint32_t       px, py;
         chunk_xy(ch, job.width, p, px, py);
         const uint32_t pixel = py * job.width + px;
-        double sx = job.film_sums[static_cast<size_t>(pixel) * 3 + 0];
-        double sy = job.film_sums[static_cast<size_t>(pixel) * 3 + 1];
-        double sz = job.film_sums[static_cast<size_t>(pixel) * 3 + 2];
-        for (uint32_t s = 0; s < ch.sample_count; ++s) {
-            const float4 v = P.rad[chunk_slot(ch, s, p)];
-            sx = sx + static_cast<double>(v.x);
-            sy = sy + static_cast<double>(v.y);
-            sz = sz + static_cast<double>(v.z);
-        }
-        job.film_sums[static_cast<size_t>(pixel) * 3 + 0] = sx;
-        job.film_sums[static_cast<size_t>(pixel) * 3 + 1] = sy;
-        job.film_sums[static_cast<size_t>(pixel) * 3 + 2] = sz;
+        const double2  sxy   = reinterpret_cast<const double2*>(P.ray_o)[g];
+        const double2  sz    = reinterpret_cast<const double2*>(P.ray_d)[g];
+        job.film_sums[static_cast<size_t>(pixel) * 3 + 0] = sxy.x;
+        job.film_sums[static_cast<size_t>(pixel) * 3 + 1] = sxy.y;
+        job.film_sums[static_cast<size_t>(pixel) * 3 + 2] = sz.x;
         job.film_counts[pixel] = job.film_counts[pixel] + ch.sample_count;
     }
 }
