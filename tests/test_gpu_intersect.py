@@ -126,7 +126,7 @@ def test_cornell_single_sided_rays(device, device32, oracle):
 def test_shared_edges_resolved_in_fp64(device, device32, oracle):
     """Rays aimed at the shared edges of a fan of triangles at three scales (the reference's watertight
     shared-edge cases, intersection_tests.cpp:161-180, 213-237): FP64 decisions give the FP64 ids on
-    every ray; the float pre-test rejects nothing the FP64 test would accept."""
+    every ray, including those that pass within float rounding of an edge."""
     from minirender_amd.scene import concat, material
     rng = np.random.default_rng(5)
     for scale, offset in ((1.0, 0.0), (1e-3, 0.0), (1e3, 1e7)):
@@ -138,8 +138,8 @@ def test_shared_edges_resolved_in_fp64(device, device32, oracle):
         normals = np.tile(np.array([0, 0, 1], np.float32), (k, 3, 1))
         scene = concat([(tris, normals)], [material()])
         # points on the spokes (shared edges) and near them, seen from above at random tilts; the angular
-        # offsets are log-uniform over 1e-11..1e-3 so that, at every scale, some rays pass just inside
-        # and just outside the float pre-test's error band (about 2^-19 of the origin distance)
+        # offsets are log-uniform over 1e-11..1e-3 so that, at every scale, some rays pass closer to an
+        # edge than float rounding resolves (about 2^-19 of the origin distance) and some farther
         a = ang[rng.integers(0, k, 4096)] + rng.choice([-1.0, 1.0], 4096) * 10 ** rng.uniform(-11, -3, 4096)
         r = rng.uniform(0.05, 0.95, 4096) * scale
         target = np.stack([np.cos(a) * r, np.sin(a) * r, np.zeros(4096)], -1) + offset
