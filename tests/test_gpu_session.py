@@ -111,6 +111,50 @@ def test_overflow_fails_without_publishing(device):
     assert s.wait_for(10000) and s.progress().state == "completed"
 
 
+def _overflow_scene():
+    """A film overflow that only the film check catches (the scene itself uploads): a single-sided
+    area light of radiance 1.8e38 (power 2.8e38, finite in float) seen from below through a glass
+    sheet; leaving the glass scales radiance by eta^2 = 2.25, past FLT_MAX (session_tests.cpp:203-242
+    builds the same overflow from a 2e38 environment, whose selection power 4 pi 2e38 float cannot
+    hold)."""
+    from minirender_amd.scene import Camera, material
+    g = np.array([[(-1, 0, -1), (-1, 0, 1), (1, 0, 1)], [(-1, 0, -1), (1, 0, 1), (1, 0, -1)]], np.float64)
+    a, y = 0.35, 0.3
+    lamp = np.array([[(-a, y, -a), (a, y, -a), (a, y, a)], [(-a, y, -a), (a, y, a), (-a, y, a)]], np.float64)  # faces -y
+    return concat([(g, np.tile([0, 1.0, 0], (2, 3, 1))), (lamp, np.tile([0, -1.0, 0], (2, 3, 1)))],
+                  [material("dielectric", (1, 1, 1)), material("diffuse", (0, 0, 0), emission=(1.8e38, 1.8e38, 1.8e38),
+                                                            double_sided=False)],
+                  camera=Camera(position=(0, -1, 0), target=(0, 0, 0), up=(0, 0, 1)))
+
+
+def test_failing_job_in_a_batch_leaves_the_other_films():
+    """A job that fails the film check inside a co-scheduled batch publishes nothing, and the job beside
+    it commits the film it gets alone (the commit stages each pixel's sums per batch pixel)."""
+    good_scene, bad_scene = scenegen.cornell_box(), _overflow_scene()
+    rs, ins = api.RenderSettings(40, 24, 16, 5), api.IntegratorSettings("mis", 8)
+    films = {}
+    for mode in ("plain", "batched"):
+        print(f"[commit] {mode}", flush=True)
+        with api.Device(0, schedule=mode, path_capacity=1 << 16) as d:
+            good = api.Job(d.upload(good_scene), good_scene.camera, rs, ins, pass_samples=4)
+            if mode == "batched":
+                bads = [api.Job(d.upload(bad_scene), bad_scene.camera, api.RenderSettings(4, 4, 64, k),
+                                api.IntegratorSettings("mis", 2, False, 3), pass_samples=16) for k in range(3)]
+                for b in bads:
+                    b.wait()
+                    p = b.progress()
+                    assert p.state == "failed" and p.error and p.recorded_samples == 0
+                    sums, counts = b.film()
+                    assert (counts == 0).all() and (sums == 0).all()
+                    b.release()
+            good.wait()
+            assert good.progress().state == "completed"
+            films[mode] = good.film()
+            good.release()
+    np.testing.assert_array_equal(films["plain"][1], films["batched"][1])
+    np.testing.assert_array_equal(films["plain"][0], films["batched"][0])
+
+
 def test_collapsed_scene_fails_then_recovers(device):
     scene = scenegen.floor_scene()
     scene.positions[:] = np.float32(1e10) + scene.positions * np.float32(1e-4)
