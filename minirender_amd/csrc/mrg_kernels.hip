@@ -457,48 +457,83 @@ __device__ __forceinline__ uint32_t find_chunk_by_pixel(const GpuChunk* chunks, 
     return lo;
 }
 
-// Camera rays (PathTracer::sample_pixel, path_tracer.cpp:368-386; Camera::generate_lens_ray).
+// The camera ray of one path slot, drawn from the path's sample stream, and the stream's state after
+// the four camera draws (PathTracer::sample_pixel, path_tracer.cpp:368-386; Camera::generate_lens_ray).
+__device__ __forceinline__ void start_camera_path(const Pool& P, const GpuJob& job, uint32_t job_index, uint32_t slot, uint32_t x,
+                                                  uint32_t y, Pcg rng, bool one_job) {
+    const float      fx     = next_1d(rng);
+    const float      fy     = next_1d(rng);
+    const float      lx     = next_1d(rng);
+    const float      ly     = next_1d(rng);
+    const GpuCamera& cam    = job.camera;
+    const float      uvx    = (static_cast<float>(x) + fx) / static_cast<float>(job.width);
+    const float      uvy    = (static_cast<float>(y) + fy) / static_cast<float>(job.height);
+    const float      aspect = static_cast<float>(job.width) / static_cast<float>(job.height);
+    const F3         fwd    = ld3(cam.forward), right = ld3(cam.right), up = ld3(cam.up), pos = ld3(cam.position);
+    const F3         pd = normalize((fwd + (((2.0f * uvx - 1.0f) * aspect) * cam.half_height) * right) +
+                                    ((1.0f - 2.0f * uvy) * cam.half_height) * up);
+    F3 o = pos, d = pd;
+    if (cam.aperture != 0.0f) {
+        float dx, dy;
+        sample_disk(lx, ly, dx, dy);
+        const float kx = dx * cam.aperture, ky = dy * cam.aperture;
+        const F3    focus_point = pos + pd * (cam.focus / dot(pd, fwd));
+        o                       = (pos + kx * right) + ky * up;
+        d                       = normalize(focus_point - o);
+    }
+    const float cosine = dot(d, fwd);
+    P.ray_o[slot] = make_float4(o.x, o.y, o.z, cam.near_plane / cosine);
+    P.ray_d[slot] = make_float4(d.x, d.y, d.z, cam.far_plane / cosine);
+    // beta = 1, pdf 0, radiance 0 and eta scale 1 are implied for bounce 0 (k_shade), and so is the
+    // previous-event record of a one-job batch (job 0, a camera path's flags)
+    if (!one_job) P.prev[slot] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(kPrevDelta | (job_index << 16)));
+    P.rng[slot] = make_uint4(static_cast<uint32_t>(rng.state), static_cast<uint32_t>(rng.state >> 32), static_cast<uint32_t>(rng.inc),
+                             static_cast<uint32_t>(rng.inc >> 32));
+}
+
+// Camera rays of every path slot of the batch. A batch that is one chunk of whole image rows (a
+// frame, or a band of rows, of one job) runs pixel-stationary: a thread keeps one pixel and one
+// position within the sample groups and walks the groups, so the pixel's position, its share of the
+// stream set-up (pixel_stream) and the job's camera are formed once per thread instead of once per
+// path; the 64 lanes of a wave still write 64 consecutive slots per group. Other batches map each
+// slot back to its chunk, sample and pixel.
 __global__ void __launch_bounds__(256) k_generate(Pool P, const GpuJob* __restrict__ jobs, const GpuChunk* __restrict__ chunks) {
     const uint32_t chunk_count = P.header[kHdrChunks];
     const uint32_t total       = P.header[kHdrSlots];
     const bool     one_job     = P.header[kHdrOneJob] != 0u;
-    for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < total; slot += gridDim.x * blockDim.x) {
-        const uint32_t  ci = find_chunk_by_slot(chunks, chunk_count, slot);
-        const GpuChunk  ch = chunks[ci];
-        uint32_t        s, p;
-        chunk_sample(ch, slot - ch.slot_begin, s, p);
-        const GpuJob&   job    = jobs[ch.job];
-        uint32_t        x, y;
-        chunk_xy(ch, job.width, p, x, y);
-        Pcg             rng    = for_sample(job.seed, static_cast<uint64_t>(y) * job.width + x, ch.sample_begin + s);
-        const float     fx     = next_1d(rng);
-        const float     fy     = next_1d(rng);
-        const float     lx     = next_1d(rng);
-        const float     ly     = next_1d(rng);
-        const GpuCamera& cam   = job.camera;
-        const float     uvx    = (static_cast<float>(x) + fx) / static_cast<float>(job.width);
-        const float     uvy    = (static_cast<float>(y) + fy) / static_cast<float>(job.height);
-        const float     aspect = static_cast<float>(job.width) / static_cast<float>(job.height);
-        const F3        fwd    = ld3(cam.forward), right = ld3(cam.right), up = ld3(cam.up), pos = ld3(cam.position);
-        const F3        pd = normalize((fwd + (((2.0f * uvx - 1.0f) * aspect) * cam.half_height) * right) +
-                                       ((1.0f - 2.0f * uvy) * cam.half_height) * up);
-        F3 o = pos, d = pd;
-        if (cam.aperture != 0.0f) {
-            float dx, dy;
-            sample_disk(lx, ly, dx, dy);
-            const float kx = dx * cam.aperture, ky = dy * cam.aperture;
-            const F3    focus_point = pos + pd * (cam.focus / dot(pd, fwd));
-            o                       = (pos + kx * right) + ky * up;
-            d                       = normalize(focus_point - o);
+    const uint32_t first       = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t stride      = gridDim.x * blockDim.x;
+    if (chunk_count == 1u && chunks[0].tile_rows != 0u) {
+        const GpuChunk ch     = chunks[0];
+        const GpuJob&  job    = jobs[ch.job];
+        const uint32_t n      = ch.pixel_count;
+        const uint32_t groups = ch.sample_count / kSampleGroup;  // grouped_samples(ch) / kSampleGroup
+        const uint32_t rest   = ch.sample_count - groups * kSampleGroup;
+        const uint32_t pitch  = n * kSampleGroup;                // slots of one sample group (chunk_slot)
+        for (uint32_t u = first; u < pitch; u += stride) {
+            const uint32_t q = u / kSampleGroup, j = u & (kSampleGroup - 1u);
+            uint32_t       x, y;
+            chunk_xy(ch, job.width, q, x, y);
+            const PixelStream px   = pixel_stream(job.seed, static_cast<uint64_t>(y) * job.width + x);
+            uint32_t          slot = ch.slot_begin + u;
+            for (uint32_t g = 0; g < groups; ++g, slot += pitch)
+                start_camera_path(P, job, ch.job, slot, x, y, for_sample(px, ch.sample_begin + g * kSampleGroup + j), one_job);
+            if (j < rest)  // the samples after the last whole group, one over all pixel positions each
+                start_camera_path(P, job, ch.job, ch.slot_begin + groups * pitch + j * n + q, x, y,
+                                  for_sample(px, ch.sample_begin + groups * kSampleGroup + j), one_job);
         }
-        const float cosine = dot(d, fwd);
-        P.ray_o[slot] = make_float4(o.x, o.y, o.z, cam.near_plane / cosine);
-        P.ray_d[slot] = make_float4(d.x, d.y, d.z, cam.far_plane / cosine);
-        // beta = 1, pdf 0, radiance 0 and eta scale 1 are implied for bounce 0 (k_shade), and so is the
-        // previous-event record of a one-job batch (job 0, a camera path's flags)
-        if (!one_job) P.prev[slot] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(kPrevDelta | (ch.job << 16)));
-        P.rng[slot]   = make_uint4(static_cast<uint32_t>(rng.state), static_cast<uint32_t>(rng.state >> 32),
-                                   static_cast<uint32_t>(rng.inc), static_cast<uint32_t>(rng.inc >> 32));
+        return;
+    }
+    for (uint32_t slot = first; slot < total; slot += stride) {
+        const uint32_t ci = find_chunk_by_slot(chunks, chunk_count, slot);
+        const GpuChunk ch = chunks[ci];
+        uint32_t       s, p;
+        chunk_sample(ch, slot - ch.slot_begin, s, p);
+        const GpuJob& job = jobs[ch.job];
+        uint32_t      x, y;
+        chunk_xy(ch, job.width, p, x, y);
+        start_camera_path(P, job, ch.job, slot, x, y, for_sample(job.seed, static_cast<uint64_t>(y) * job.width + x, ch.sample_begin + s),
+                          one_job);
     }
 }
 

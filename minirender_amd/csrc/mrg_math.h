@@ -141,8 +141,23 @@ __device__ __forceinline__ Pcg pcg_init(uint64_t seed, uint64_t sequence) {
     return p;
 }
 
+// The pixel's share of a sample stream's set-up (three of its four SplitMix finalisers): a thread
+// that draws several samples of one pixel forms it once.
+struct PixelStream {
+    uint64_t seed;      // PCG seed of every sample of the pixel
+    uint64_t sequence;  // the pixel's term of the PCG sequence
+};
+
+__device__ __forceinline__ PixelStream pixel_stream(uint64_t seed, uint64_t pixel) {
+    return PixelStream{mix64(seed ^ mix64(pixel + 0x9e3779b97f4a7c15ull)), mix64(pixel)};
+}
+
+__device__ __forceinline__ Pcg for_sample(const PixelStream& px, uint64_t sample) {
+    return pcg_init(px.seed, mix64(sample + 0xd1b54a32d192ed03ull) ^ px.sequence);
+}
+
 __device__ __forceinline__ Pcg for_sample(uint64_t seed, uint64_t pixel, uint64_t sample) {
-    return pcg_init(mix64(seed ^ mix64(pixel + 0x9e3779b97f4a7c15ull)), mix64(sample + 0xd1b54a32d192ed03ull) ^ mix64(pixel));
+    return for_sample(pixel_stream(seed, pixel), sample);
 }
 
 // 53 random bits rounded to float and clamped below one.
