@@ -95,6 +95,27 @@ def test_pixel_split_batches_match_fp32_twin(oracle, width):
     assert prog.statistics["extension_rays"] == st["extension_rays"]
 
 
+@pytest.mark.parametrize("pass_samples", [0, 3, 6])
+def test_one_chunk_camera_rays_match_fp32_twin(oracle, pass_samples):
+    """A batch that is one chunk of whole rows takes the pixel-stationary camera-ray kernel (a thread
+    walks the sample groups of one pixel). 37 x 23 leaves partial 4x4 tiles in the last tile row and
+    column, and 7 spp leaves 3 samples after the last whole group of 4; progressive passes of 3 or 6
+    samples start later passes mid-stream (sample_begin > 0, with and without whole groups). Each
+    pass is one batch; the image equals the FP32 twin's."""
+    scene = scenegen.cornell_box()
+    width, height, spp, depth, seed = 37, 23, 7, 6, 29
+    print(f"[render parity] one-chunk cornell {width}x{height} spp={spp} pass={pass_samples}", flush=True)
+    with api.Device(0) as dev:
+        dev.kernel_times(reset=True)
+        img, prog = api.render(dev, scene, api.RenderSettings(width, height, spp, seed), api.IntegratorSettings("mis", depth, True, 3),
+                               pass_samples=pass_samples)
+        assert dev.kernel_times()["batches"] == (-(-spp // pass_samples) if pass_samples else 1)
+    twin, _, st = oracle.OracleScene(scene).render(scene.camera, width, height, spp, seed=seed, mode="mis", max_depth=depth,
+                                                   rr=True, precision=32, sums=True)
+    _check(img, twin)
+    assert prog.statistics["extension_rays"] == st["extension_rays"]
+
+
 def test_default_refill_batch_matches_fp32_twin(oracle):
     """The library's default for a large batch: AUTO traversal picks the refill launch (a scene of at
     least 2^14 triangles in one batch of 1024 x 1024 x 8 = 2^23 paths), at the library's block counts
