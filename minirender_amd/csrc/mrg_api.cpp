@@ -995,11 +995,14 @@ void enqueue_batch(mrg_device* d, Track& t, hipEvent_t before) {
     t.iters      = b.max_depth + 1;
     t.timed      = d->opt.use_graph == 0;
     t.marks.clear();
+    uint32_t units = 0;
     for (GpuChunk& c : b.chunks) {
         const uint32_t w = b.gjobs[c.job].width;
         c.tile_rows      = c.pixel_begin % w == 0 && c.pixel_count % w == 0 ? c.pixel_count / w : 0u;
+        c.unit_prefix    = units;
+        units += c.tile_rows ? c.pixel_count * 4u : c.pixel_count;  // k_generate's units (4 per pixel: kSampleGroup)
     }
-    const uint32_t hdr[kHdrWords] = {uint32_t(b.chunks.size()), b.slots, b.pixels, t.iters, b.gjobs.size() == 1 ? 1u : 0u, 0u, 0u, 0u};
+    const uint32_t hdr[kHdrWords] = {uint32_t(b.chunks.size()), b.slots, b.pixels, t.iters, b.gjobs.size() == 1 ? 1u : 0u, units, 0u, 0u};
     std::memcpy(t.up_jobs.ptr, b.gjobs.data(), b.gjobs.size() * sizeof(GpuJob));
     std::memcpy(t.up_chunks.ptr, b.chunks.data(), b.chunks.size() * sizeof(GpuChunk));
     std::memcpy(t.up_header.ptr, hdr, sizeof hdr);

@@ -45,6 +45,7 @@ constexpr uint32_t kHdrSlots  = 1;
 constexpr uint32_t kHdrPixels = 2;
 constexpr uint32_t kHdrOneJob = 4;  // 1 when every chunk of the batch belongs to one job: bounce 0 has no path records of
                                    // `prev` (the job is 0, the flags are a camera path's), k_generate writes none
+constexpr uint32_t kHdrUnits  = 5;  // camera-ray work units of the batch (GpuChunk::unit_prefix)
 constexpr uint32_t kHdrWords  = 8;
 
 // Batch statistics (uint64).

@@ -381,24 +381,7 @@ __device__ __forceinline__ uint32_t grouped_samples(const GpuChunk& ch) {
     return ch.tile_rows ? ch.sample_count & ~(kSampleGroup - 1u) : 0u;
 }
 
-// Sample s and pixel position q of slot `local` (counted from the chunk's first slot).
-__device__ __forceinline__ void chunk_sample(const GpuChunk& ch, uint32_t local, uint32_t& s, uint32_t& q) {
-    const uint32_t n       = ch.pixel_count;
-    const uint32_t grouped = grouped_samples(ch);
-    if (local < grouped * n) {
-        const uint32_t g   = local / (n * kSampleGroup);
-        const uint32_t rem = local - g * (n * kSampleGroup);
-        q = rem / kSampleGroup;
-        s = g * kSampleGroup + (rem & (kSampleGroup - 1u));
-    } else {
-        const uint32_t rest = local - grouped * n;
-        const uint32_t k    = rest / n;
-        s = grouped + k;
-        q = rest - k * n;
-    }
-}
-
-// Slot of sample s at pixel position q (the inverse of chunk_sample).
+// Slot of sample s at pixel position q.
 __device__ __forceinline__ uint32_t chunk_slot(const GpuChunk& ch, uint32_t s, uint32_t q) {
     const uint32_t n       = ch.pixel_count;
     const uint32_t grouped = grouped_samples(ch);
@@ -409,7 +392,7 @@ __device__ __forceinline__ uint32_t chunk_slot(const GpuChunk& ch, uint32_t s, u
 // Pixel (x, y) of pixel position q of a chunk. A chunk of whole image rows (tile_rows != 0) lays its
 // pixels out in kTileW x kTileH tiles, tile rows top to bottom and tiles left to right (the last tile
 // row and column hold the remainder rows and columns); with kSampleGroup samples of each pixel side
-// by side (chunk_sample), the 64 paths of a wave start from one compact patch of the image instead
+// by side (chunk_slot), the 64 paths of a wave start from one compact patch of the image instead
 // of a 64-pixel strip. Other chunks keep scanline order. k_generate, k_validate and k_accumulate
 // use the same maps, so every sample lands on its own pixel.
 __device__ __forceinline__ void chunk_xy(const GpuChunk& ch, uint32_t width, uint32_t q, uint32_t& x, uint32_t& y) {
@@ -437,15 +420,6 @@ __device__ __forceinline__ void chunk_xy(const GpuChunk& ch, uint32_t width, uin
         x = full + (w - y * rest);
     }
     y += y0 + ch.pixel_begin / width;
-}
-
-__device__ __forceinline__ uint32_t find_chunk_by_slot(const GpuChunk* chunks, uint32_t count, uint32_t slot) {
-    uint32_t lo = 0, hi = count;
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (chunks[mid].slot_begin <= slot) lo = mid; else hi = mid;
-    }
-    return lo;
 }
 
 __device__ __forceinline__ uint32_t find_chunk_by_pixel(const GpuChunk* chunks, uint32_t count, uint32_t pix) {
@@ -491,49 +465,51 @@ __device__ __forceinline__ void start_camera_path(const Pool& P, const GpuJob& j
                              static_cast<uint32_t>(rng.inc >> 32));
 }
 
-// Camera rays of every path slot of the batch. A batch that is one chunk of whole image rows (a
-// frame, or a band of rows, of one job) runs pixel-stationary: a thread keeps one pixel and one
-// position within the sample groups and walks the groups, so the pixel's position, its share of the
-// stream set-up (pixel_stream) and the job's camera are formed once per thread instead of once per
-// path; the 64 lanes of a wave still write 64 consecutive slots per group. Other batches map each
-// slot back to its chunk, sample and pixel.
+__device__ __forceinline__ uint32_t find_chunk_by_unit(const GpuChunk* chunks, uint32_t count, uint32_t unit) {
+    uint32_t lo = 0, hi = count;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (chunks[mid].unit_prefix <= unit) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// Camera rays of every path slot of the batch, pixel-stationary: a thread takes a work unit (one
+// pixel of a chunk in scanline order, or one pixel and one position within the sample groups of a
+// chunk of whole rows) and walks that unit's samples, so the pixel's position, its share of the
+// stream set-up (pixel_stream) and the job's camera are formed once per unit instead of once per
+// path. Consecutive units are consecutive slots of each sample (group), so the 64 lanes of a wave
+// write 64 consecutive slots per step.
 __global__ void __launch_bounds__(256) k_generate(Pool P, const GpuJob* __restrict__ jobs, const GpuChunk* __restrict__ chunks) {
     const uint32_t chunk_count = P.header[kHdrChunks];
-    const uint32_t total       = P.header[kHdrSlots];
+    const uint32_t units       = P.header[kHdrUnits];
     const bool     one_job     = P.header[kHdrOneJob] != 0u;
-    const uint32_t first       = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t stride      = gridDim.x * blockDim.x;
-    if (chunk_count == 1u && chunks[0].tile_rows != 0u) {
-        const GpuChunk ch     = chunks[0];
-        const GpuJob&  job    = jobs[ch.job];
-        const uint32_t n      = ch.pixel_count;
-        const uint32_t groups = ch.sample_count / kSampleGroup;  // grouped_samples(ch) / kSampleGroup
-        const uint32_t rest   = ch.sample_count - groups * kSampleGroup;
-        const uint32_t pitch  = n * kSampleGroup;                // slots of one sample group (chunk_slot)
-        for (uint32_t u = first; u < pitch; u += stride) {
-            const uint32_t q = u / kSampleGroup, j = u & (kSampleGroup - 1u);
+    for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < units; u += gridDim.x * blockDim.x) {
+        const GpuChunk ch    = chunks[chunk_count == 1u ? 0u : find_chunk_by_unit(chunks, chunk_count, u)];
+        const GpuJob&  job   = jobs[ch.job];
+        const uint32_t n     = ch.pixel_count;
+        const uint32_t local = u - ch.unit_prefix;
+        if (ch.tile_rows != 0u) {
+            const uint32_t groups = ch.sample_count / kSampleGroup;  // grouped_samples(ch) / kSampleGroup
+            const uint32_t rest   = ch.sample_count - groups * kSampleGroup;
+            const uint32_t pitch  = n * kSampleGroup;                // slots of one sample group (chunk_slot)
+            const uint32_t q = local / kSampleGroup, j = local & (kSampleGroup - 1u);
             uint32_t       x, y;
             chunk_xy(ch, job.width, q, x, y);
             const PixelStream px   = pixel_stream(job.seed, static_cast<uint64_t>(y) * job.width + x);
-            uint32_t          slot = ch.slot_begin + u;
+            uint32_t          slot = ch.slot_begin + local;
             for (uint32_t g = 0; g < groups; ++g, slot += pitch)
                 start_camera_path(P, job, ch.job, slot, x, y, for_sample(px, ch.sample_begin + g * kSampleGroup + j), one_job);
             if (j < rest)  // the samples after the last whole group, one over all pixel positions each
                 start_camera_path(P, job, ch.job, ch.slot_begin + groups * pitch + j * n + q, x, y,
                                   for_sample(px, ch.sample_begin + groups * kSampleGroup + j), one_job);
+        } else {  // scanline order, sample-major: slot = first slot + s * n + q (chunk_slot)
+            uint32_t x, y;
+            chunk_xy(ch, job.width, local, x, y);
+            const PixelStream px = pixel_stream(job.seed, static_cast<uint64_t>(y) * job.width + x);
+            for (uint32_t s = 0; s < ch.sample_count; ++s)
+                start_camera_path(P, job, ch.job, ch.slot_begin + s * n + local, x, y, for_sample(px, ch.sample_begin + s), one_job);
         }
-        return;
-    }
-    for (uint32_t slot = first; slot < total; slot += stride) {
-        const uint32_t ci = find_chunk_by_slot(chunks, chunk_count, slot);
-        const GpuChunk ch = chunks[ci];
-        uint32_t       s, p;
-        chunk_sample(ch, slot - ch.slot_begin, s, p);
-        const GpuJob& job = jobs[ch.job];
-        uint32_t      x, y;
-        chunk_xy(ch, job.width, p, x, y);
-        start_camera_path(P, job, ch.job, slot, x, y, for_sample(job.seed, static_cast<uint64_t>(y) * job.width + x, ch.sample_begin + s),
-                          one_job);
     }
 }
 

@@ -188,7 +188,7 @@ struct GpuJob {
 };
 
 // A contiguous range of samples for a contiguous range of pixels of one job. The kernels map a path
-// slot of the chunk to its (pixel, sample) with chunk_slot / chunk_sample / chunk_xy: a chunk of whole
+// (pixel, sample) of the chunk to its path slot with chunk_slot / chunk_xy: a chunk of whole
 // image rows (tile_rows != 0) holds its samples in groups of 4, each group pixel-major with the 4
 // samples of a pixel side by side and the pixels in 4x4 tiles (tile rows top to bottom, tiles left to
 // right, the remainder rows and columns in the last tile row and column); samples past the last whole
@@ -203,6 +203,8 @@ struct GpuChunk {
     uint32_t slot_begin;    // first path slot of this chunk in the pool
     uint32_t pixel_prefix;  // pixels of all earlier chunks of the batch
     uint32_t tile_rows;     // image rows of a chunk of whole rows (4x4 pixel tiles x 4 samples), else 0 (scanline order)
+    uint32_t unit_prefix;   // camera-ray work units of all earlier chunks of the batch (k_generate: 4 per pixel of a
+                            // chunk of whole rows, 1 per pixel otherwise)
 };
 
 }  // namespace mrg
